@@ -1,0 +1,39 @@
+"""acoular_amd -- Acoular's frequency-domain beamforming path, rebuilt for AMD MI355X (gfx950).
+
+The public names mirror Acoular's (``PowerSpectra``, ``PowerSpectraImport``, ``SteeringVector``,
+``BeamformerBase``, ``RectGrid``, ``MicGeom``, ``Environment``, ``L_p``, and the ``fastFuncs``
+kernels) so that code using that path can switch imports.  All compute runs in hand-written
+HIP kernels in ``libabf.so`` (C ABI in ``include/abf.h``); PyTorch supplies device memory and
+streams.
+"""
+
+from .environments import Environment, dist_mat
+from .fastFuncs import beamformerFreq, calcCSM, calcTransfer
+from .fbeamform import BeamformerBase, LazyBfResult, L_p, SteeringVector, beamform_device, integrate
+from .grids import Grid, ImportGrid, RectGrid, RectGrid3D
+from .microphones import MicGeom
+from .sources import TimeSamples
+from .spectra import BaseSpectra, PowerSpectra, PowerSpectraImport
+
+__all__ = [
+    'BaseSpectra',
+    'BeamformerBase',
+    'Environment',
+    'Grid',
+    'ImportGrid',
+    'L_p',
+    'LazyBfResult',
+    'MicGeom',
+    'PowerSpectra',
+    'PowerSpectraImport',
+    'RectGrid',
+    'RectGrid3D',
+    'SteeringVector',
+    'TimeSamples',
+    'beamform_device',
+    'beamformerFreq',
+    'calcCSM',
+    'calcTransfer',
+    'dist_mat',
+    'integrate',
+]

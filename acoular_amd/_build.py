@@ -1,0 +1,45 @@
+"""Build of the HIP library ``libabf.so`` (gfx950) from ``csrc/``, in-tree."""
+
+from __future__ import annotations
+
+import os
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, 'csrc')
+LIB = os.path.join(HERE, 'libabf.so')
+SOURCES = ['abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip', 'abf_api.hip']
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+# packed fp32 instructions are disabled on purpose (build flags)
+FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-Xclang', '-target-feature', '-Xclang',
+         '-packed-fp32-ops', '-I', os.path.join(HERE, '..', 'include')]
+
+
+def _compile(src: str, build_dir: str) -> str:
+    obj = os.path.join(build_dir, src.replace('.hip', '.o'))
+    cmd = [HIPCC, *FLAGS, '-c', os.path.join(CSRC, src), '-o', obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f'hipcc failed for {src}:\n{r.stderr}')
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    build_dir = os.path.join(HERE, '_build')
+    os.makedirs(build_dir, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        objs = list(ex.map(lambda s: _compile(s, build_dir), SOURCES))
+    tmp = LIB + '.tmp'
+    cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f'link failed:\n{r.stderr}')
+    os.replace(tmp, LIB)
+    if verbose:
+        print('built', LIB)
+    return LIB
+
+
+if __name__ == '__main__':
+    build(verbose=True)

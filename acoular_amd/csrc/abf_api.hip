@@ -1,0 +1,638 @@
+// C-ABI entry points of libabf.so (declared in include/abf.h).  Host-side planning,
+// validation and launches; the kernels live in abf_bf.hip, abf_steer.hip and abf_csm.hip.
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "abf_common.h"
+#include "abf_csm_kernels.h"
+#include "abf_kernels.h"
+#include "abf_steer_kernels.h"
+
+using namespace abf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_status(const char *what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ABF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return ABF_OK;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr size_t kLdsMax = 163840; // gfx950 LDS per workgroup
+constexpr int kPhaseMask = 0xff;
+constexpr int kFastTrigFlag = 0x100;
+
+int check_steer(int st)
+{
+    if (st < ABF_STEER_CLASSIC || st > ABF_STEER_TRUE_LOCATION)
+        return fail(ABF_ERR_INVALID, "unknown steer_type " + std::to_string(st));
+    return ABF_OK;
+}
+
+int check_phase(int pm)
+{
+    const int m = pm & kPhaseMask;
+    if (m != ABF_PHASE_REFERENCE && m != ABF_PHASE_FP64) return fail(ABF_ERR_INVALID, "unknown phase_mode");
+    return ABF_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// MFMA planning
+// ---------------------------------------------------------------------------------------------
+struct MfPlan {
+    int Mp, CT, CW, FB, nK, nUp;
+    int64_t ntiles, wgs;
+    size_t prep_bytes, lds;
+    bool bf16;
+};
+
+size_t mfma_lds(bool bf16, int BN, int Mp)
+{
+    const size_t KS = bf16 ? Mp + 8 : Mp + 4;
+    const size_t planes = bf16 ? 4 * (size_t)BN * KS * 2 : 2 * (size_t)BN * KS * 4;
+    return planes + 16 * (size_t)BN + 4 * 8 * (size_t)BN;
+}
+
+int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
+int plan_mfma(const abf_grid_job *j, MfPlan &p)
+{
+    p.bf16 = j->form == ABF_FORM_MFMA_BF16X3;
+    p.Mp = round_up(j->geom.num_mics, 32);
+    p.nK = p.Mp / 32;
+    p.nUp = p.nK * (p.nK + 1) / 2;
+    // columns needed by this grid, as a power of two tile count
+    int ct_cap = 16;
+    const int64_t tiles16 = cdiv(j->geom.num_grid, 16);
+    while (ct_cap > 1 && ct_cap / 2 >= tiles16) ct_cap /= 2;
+    const int ct_env = env_int("ABF_MFMA_CT", 0);
+    p.CT = 0;
+    for (int ct = 16; ct >= 1; ct /= 2) {
+        if (ct > ct_cap) continue;
+        if (ct_env > 0 && ct > ct_env) continue;
+        const int BN = 16 * ct;
+        const int TPC = 512 / BN;
+        if (p.Mp / 8 > TPC * 5) continue; // steering prologue covers at most 5 groups of 8 per thread
+        if (mfma_lds(p.bf16, BN, p.Mp) > kLdsMax) continue;
+        p.CT = ct;
+        break;
+    }
+    if (p.CT == 0)
+        return fail(ABF_ERR_UNSUPPORTED, "MFMA form supports at most ~1150 microphones; use the plain form");
+    p.CW = std::min(p.CT, 4);
+    const int fb = env_int("ABF_MFMA_FB", 8);
+    p.FB = std::max(1, std::min(fb, j->num_freqs));
+    p.ntiles = cdiv(j->geom.num_grid, 16 * p.CT);
+    p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
+    p.prep_bytes = (size_t)j->num_freqs * p.nUp * 8192;
+    p.lds = mfma_lds(p.bf16, 16 * p.CT, p.Mp);
+    return ABF_OK;
+}
+
+size_t plain_ut_bytes(int M, int F) { return (size_t)F * round_up(M, 16) * round_up(M, 16) * sizeof(float2); }
+
+// sizes and enums only (enough for workspace queries)
+int validate_job(const abf_grid_job *j)
+{
+    if (!j) return fail(ABF_ERR_INVALID, "null job");
+    if (j->geom.num_mics < 1 || j->geom.num_grid < 0 || j->num_freqs < 0)
+        return fail(ABF_ERR_INVALID, "bad job sizes");
+    if (int r = check_steer(j->steer_type)) return r;
+    if (int r = check_phase(j->phase_mode)) return r;
+    if (j->form < ABF_FORM_PLAIN_F32 || j->form > ABF_FORM_MFMA_F32) return fail(ABF_ERR_INVALID, "unknown form");
+    return ABF_OK;
+}
+
+int validate_job_pointers(const abf_grid_job *j)
+{
+    if (j->geom.num_grid > 0 && j->num_freqs > 0) {
+        if (!j->csm || !j->wavenumber || !j->geom.gpos || !j->geom.mpos || !j->out)
+            return fail(ABF_ERR_INVALID, "null device pointer in job");
+    }
+    return ABF_OK;
+}
+
+size_t job_workspace(const abf_grid_job *j)
+{
+    if (j->form == ABF_FORM_PLAIN_F32) return align256(plain_ut_bytes(j->geom.num_mics, j->num_freqs));
+    MfPlan p;
+    if (plan_mfma(j, p)) return 0;
+    return align256(p.prep_bytes);
+}
+
+// descriptors travel to the device as kernel arguments of this copy kernel, so a launch
+// sequence needs no host-to-device memcpy (and can be captured into a graph)
+constexpr int kPack = 16;
+struct JobPack {
+    MfJob jobs[kPack];
+    int n;
+};
+
+__global__ void write_jobs_kernel(JobPack pack, MfJob *dst)
+{
+    const int i = threadIdx.x;
+    if (i < pack.n) dst[i] = pack.jobs[i];
+}
+
+__global__ void set_double_kernel(double *d, double v)
+{
+    if (threadIdx.x == 0) d[0] = v;
+}
+
+__global__ void zero_c64_kernel(float2 *p)
+{
+    if (threadIdx.x == 0) p[0] = make_float2(0.f, 0.f);
+}
+
+__global__ void twiddle_kernel(float2 *tw, int bs)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= bs / 2) return;
+    double s, c;
+    sincospi(2.0 * j / bs, &s, &c);
+    tw[j] = make_float2((float)c, (float)-s);
+}
+
+template <bool B, int CW> void launch_mfma(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)bf_mfma_kernel<B, CW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLdsMax);
+        attr = true;
+    }
+    hipLaunchKernelGGL((bf_mfma_kernel<B, CW>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
+}
+
+void launch_mfma_group(bool bf16, int CW, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+{
+    if (bf16) {
+        if (CW == 1) launch_mfma<true, 1>(dev, n, wgs, lds, s);
+        else if (CW == 2) launch_mfma<true, 2>(dev, n, wgs, lds, s);
+        else launch_mfma<true, 4>(dev, n, wgs, lds, s);
+    } else {
+        if (CW == 1) launch_mfma<false, 1>(dev, n, wgs, lds, s);
+        else if (CW == 2) launch_mfma<false, 2>(dev, n, wgs, lds, s);
+        else launch_mfma<false, 4>(dev, n, wgs, lds, s);
+    }
+}
+
+void prep_plain(const abf_grid_job *j, void *ws, hipStream_t s)
+{
+    const int M = j->geom.num_mics, Mp = round_up(M, 16), F = j->num_freqs;
+    hipLaunchKernelGGL(prep_plain_kernel, dim3((unsigned)cdiv((int64_t)Mp * Mp, 256), F), dim3(256), 0, s,
+                       reinterpret_cast<const float2 *>(j->csm), M, Mp, j->r_diag, reinterpret_cast<float2 *>(ws));
+}
+
+int run_plain_only(const abf_grid_job *j, void *ws, hipStream_t s)
+{
+    const int M = j->geom.num_mics, Mp = round_up(M, 16), F = j->num_freqs;
+    const int64_t G = j->geom.num_grid;
+    float2 *ut = reinterpret_cast<float2 *>(ws);
+    PlainArgs a{};
+    a.ut = ut;
+    a.gpos = j->geom.gpos;
+    a.mpos = j->geom.mpos;
+    a.wavenumber = j->wavenumber;
+    a.out = j->out;
+    a.out_norm = j->out_norm;
+    for (int i = 0; i < 3; ++i) a.ref[i] = j->geom.ref[i];
+    a.ref_dist = j->geom.ref_dist;
+    a.norm = j->norm_factor;
+    a.G = G;
+    a.M = M;
+    a.Mp = Mp;
+    a.steer_type = j->steer_type;
+    a.phase_mode = j->phase_mode & kPhaseMask;
+    a.fast_trig = (j->phase_mode & kFastTrigFlag) != 0;
+    a.clip = j->clip_negative;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)bf_plain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
+        attr = true;
+    }
+    const size_t lds = (size_t)std::min(256, Mp) * 64 * sizeof(float2);
+    hipLaunchKernelGGL(bf_plain_kernel, dim3((unsigned)cdiv(G, 64), F), dim3(256), lds, s, a);
+    return hip_status("plain beamformer launch");
+}
+
+int run_plain(const abf_grid_job *j, void *ws, hipStream_t s)
+{
+    prep_plain(j, ws, s);
+    return run_plain_only(j, ws, s);
+}
+
+MfJob make_mfjob(const abf_grid_job *j, const MfPlan &p, const void *prep, int64_t wg_begin)
+{
+    MfJob d{};
+    d.prep = prep;
+    d.gpos = j->geom.gpos;
+    d.mpos = j->geom.mpos;
+    d.wavenumber = j->wavenumber;
+    d.out = j->out;
+    d.out_norm = j->out_norm;
+    for (int i = 0; i < 3; ++i) d.ref[i] = j->geom.ref[i];
+    d.ref_dist = j->geom.ref_dist;
+    d.norm = j->norm_factor;
+    d.G = j->geom.num_grid;
+    d.wg_begin = wg_begin;
+    d.M = j->geom.num_mics;
+    d.Mp = p.Mp;
+    d.CT = p.CT;
+    d.F = j->num_freqs;
+    d.FB = p.FB;
+    d.ntiles = (int)p.ntiles;
+    d.steer_type = j->steer_type;
+    d.phase_mode = j->phase_mode & kPhaseMask;
+    d.fast_trig = (j->phase_mode & kFastTrigFlag) != 0;
+    d.clip = j->clip_negative;
+    return d;
+}
+
+void launch_prep_mfma(const abf_grid_job *j, const MfPlan &p, void *prep, hipStream_t s)
+{
+    hipLaunchKernelGGL(prep_mfma_kernel, dim3(p.nK * p.nK, 2, j->num_freqs), dim3(64), 0, s,
+                       reinterpret_cast<const float2 *>(j->csm), j->geom.num_mics, p.Mp, j->r_diag, p.bf16 ? 1 : 0,
+                       reinterpret_cast<char *>(prep));
+}
+
+} // namespace
+
+extern "C" {
+
+const char *abf_last_error(void) { return g_err.c_str(); }
+const char *abf_version(void) { return "abf 0.1 (gfx950)"; }
+
+size_t abf_beamform_workspace_size(const abf_grid_job *job)
+{
+    if (validate_job(job)) return 0;
+    return abf_beamform_batch_workspace_size(job, 1, nullptr);
+}
+
+size_t abf_beamform_batch_workspace_size(const abf_grid_job *jobs, int32_t num_jobs, size_t *offsets)
+{
+    size_t off = align256((size_t)std::max(num_jobs, 1) * sizeof(MfJob));
+    for (int i = 0; i < num_jobs; ++i) {
+        if (validate_job(&jobs[i])) return 0;
+        if (offsets) offsets[i] = off;
+        off += job_workspace(&jobs[i]);
+    }
+    return off;
+}
+
+} // extern "C"
+
+namespace {
+constexpr int kPrep = 1, kRun = 2;
+
+int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *workspace, size_t workspace_bytes,
+                        void *stream, int mode)
+{
+    if (num_jobs < 0 || (num_jobs > 0 && !jobs)) return fail(ABF_ERR_INVALID, "bad job list");
+    const hipStream_t s = S(stream);
+    std::vector<size_t> offs(num_jobs);
+    const size_t need = abf_beamform_batch_workspace_size(jobs, num_jobs, offs.data());
+    if (num_jobs > 0 && need == 0) return g_err.empty() ? fail(ABF_ERR_INVALID, "invalid job") : ABF_ERR_INVALID;
+    if (!workspace || workspace_bytes < need)
+        return fail(ABF_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+    char *ws = reinterpret_cast<char *>(workspace);
+    MfJob *dev_jobs = reinterpret_cast<MfJob *>(ws);
+
+    // plain jobs run one after the other; MFMA jobs are grouped per (form, CW) into one launch
+    struct Group {
+        bool bf16;
+        int CW;
+        std::vector<int> idx;
+    };
+    std::vector<Group> groups;
+    std::vector<MfPlan> plans(num_jobs);
+    for (int i = 0; i < num_jobs; ++i) {
+        const abf_grid_job *j = &jobs[i];
+        if (int r = validate_job_pointers(j)) return r;
+        if (j->geom.num_grid == 0 || j->num_freqs == 0) continue;
+        if (j->form == ABF_FORM_PLAIN_F32) {
+            if (mode & kPrep) prep_plain(j, ws + offs[i], s);
+            if (mode & kRun)
+                if (int r = run_plain_only(j, ws + offs[i], s)) return r;
+            continue;
+        }
+        if (int r = plan_mfma(j, plans[i])) return r;
+        if (mode & kPrep) launch_prep_mfma(j, plans[i], ws + offs[i], s);
+        if (!(mode & kRun)) continue;
+        auto it = std::find_if(groups.begin(), groups.end(),
+                               [&](const Group &g) { return g.bf16 == plans[i].bf16 && g.CW == plans[i].CW; });
+        if (it == groups.end()) {
+            groups.push_back({plans[i].bf16, plans[i].CW, {}});
+            it = groups.end() - 1;
+        }
+        it->idx.push_back(i);
+    }
+    if (int r = hip_status("MFMA re-layout launch")) return r;
+    int desc_base = 0;
+    for (const Group &gr : groups) {
+        int64_t wg = 0;
+        size_t lds = 0;
+        std::vector<MfJob> descs;
+        for (int i : gr.idx) {
+            descs.push_back(make_mfjob(&jobs[i], plans[i], ws + offs[i], wg));
+            wg += plans[i].wgs;
+            lds = std::max(lds, plans[i].lds);
+        }
+        for (size_t b = 0; b < descs.size(); b += kPack) {
+            JobPack pk{};
+            pk.n = (int)std::min<size_t>(kPack, descs.size() - b);
+            for (int q = 0; q < pk.n; ++q) pk.jobs[q] = descs[b + q];
+            hipLaunchKernelGGL(write_jobs_kernel, dim3(1), dim3(64), 0, s, pk, dev_jobs + desc_base + b);
+        }
+        launch_mfma_group(gr.bf16, gr.CW, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+        if (int r = hip_status("MFMA beamformer launch")) return r;
+        desc_base += (int)descs.size();
+    }
+    return ABF_OK;
+}
+
+int beamform_single(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream, int mode)
+{
+    if (int r = validate_job(job)) return r;
+    if (int r = validate_job_pointers(job)) return r;
+    if (job->geom.num_grid == 0 || job->num_freqs == 0) return ABF_OK;
+    return beamform_batch_impl(job, 1, workspace, workspace_bytes, stream, mode);
+}
+} // namespace
+
+extern "C" {
+
+int abf_beamform_batch(const abf_grid_job *jobs, int32_t num_jobs, void *workspace, size_t workspace_bytes,
+                       void *stream)
+{
+    return beamform_batch_impl(jobs, num_jobs, workspace, workspace_bytes, stream, kPrep | kRun);
+}
+
+int abf_beamform(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream)
+{
+    return beamform_single(job, workspace, workspace_bytes, stream, kPrep | kRun);
+}
+
+int abf_beamform_prepare(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream)
+{
+    return beamform_single(job, workspace, workspace_bytes, stream, kPrep);
+}
+
+int abf_beamform_run(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream)
+{
+    return beamform_single(job, workspace, workspace_bytes, stream, kRun);
+}
+
+size_t abf_beamform_table_workspace_size(int32_t num_mics)
+{
+    return align256(plain_ut_bytes(num_mics, 1)) + 256;
+}
+
+int abf_beamform_table(const void *csm, const double *r0, const double *rm, int64_t num_grid, int32_t num_mics,
+                       double wavenumber, int32_t steer_type, int32_t r_diag, double norm_factor, int32_t phase_mode,
+                       float *out, float *out_norm, void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (int r = check_steer(steer_type)) return r;
+    if (int r = check_phase(phase_mode)) return r;
+    if (num_mics < 1 || num_grid < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (num_grid == 0) return ABF_OK;
+    if (!csm || !r0 || !rm || !out) return fail(ABF_ERR_INVALID, "null device pointer");
+    const size_t need = abf_beamform_table_workspace_size(num_mics);
+    if (!workspace || workspace_bytes < need) return fail(ABF_ERR_WORKSPACE, "workspace too small");
+    const hipStream_t s = S(stream);
+    const int M = num_mics, Mp = round_up(M, 16);
+    char *ws = reinterpret_cast<char *>(workspace);
+    float2 *ut = reinterpret_cast<float2 *>(ws);
+    double *kdev = reinterpret_cast<double *>(ws + align256(plain_ut_bytes(M, 1)));
+    hipLaunchKernelGGL(prep_plain_kernel, dim3((unsigned)cdiv((int64_t)Mp * Mp, 256), 1), dim3(256), 0, s,
+                       reinterpret_cast<const float2 *>(csm), M, Mp, r_diag, ut);
+    // the wavenumber reaches the device through a one-element fill (no host memcpy)
+    hipLaunchKernelGGL(set_double_kernel, dim3(1), dim3(64), 0, s, kdev, wavenumber);
+    PlainArgs a{};
+    a.ut = ut;
+    a.rm_table = rm;
+    a.r0_table = r0;
+    a.wavenumber = kdev;
+    a.out = out;
+    a.out_norm = out_norm;
+    a.norm = norm_factor;
+    a.G = num_grid;
+    a.M = M;
+    a.Mp = Mp;
+    a.steer_type = steer_type;
+    a.phase_mode = phase_mode & kPhaseMask;
+    a.fast_trig = (phase_mode & kFastTrigFlag) != 0;
+    a.clip = 0;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)bf_plain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
+        attr = true;
+    }
+    const size_t lds = (size_t)std::min(256, Mp) * 64 * sizeof(float2);
+    hipLaunchKernelGGL(bf_plain_kernel, dim3((unsigned)cdiv(num_grid, 64), 1), dim3(256), lds, s, a);
+    return hip_status("table beamformer launch");
+}
+
+size_t abf_beamform_eig_workspace_size(int32_t, int32_t) { return 0; }
+
+int abf_beamform_eig(const double *eigval, const void *eigvec, int32_t num_eig, const double *r0, const double *rm,
+                     int64_t num_grid, int32_t num_mics, double wavenumber, int32_t steer_type, int32_t r_diag,
+                     double norm_factor, int32_t phase_mode, float *out, float *out_norm, void *, size_t,
+                     void *stream)
+{
+    if (int r = check_steer(steer_type)) return r;
+    if (int r = check_phase(phase_mode)) return r;
+    if (num_mics < 1 || num_grid < 0 || num_eig < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (num_grid == 0) return ABF_OK;
+    if (!r0 || !rm || !out || (num_eig > 0 && (!eigval || !eigvec))) return fail(ABF_ERR_INVALID, "null pointer");
+    EigArgs a{};
+    a.eigval = eigval;
+    a.eigvec_t = reinterpret_cast<const float2 *>(eigvec);
+    a.steer = nullptr;
+    a.r0_table = r0;
+    a.rm_table = rm;
+    a.out = out;
+    a.out_norm = out_norm;
+    a.wavenumber = wavenumber;
+    a.norm = norm_factor;
+    a.G = num_grid;
+    a.M = num_mics;
+    a.K = num_eig;
+    a.steer_type = steer_type;
+    a.r_diag = r_diag;
+    a.phase_mode = phase_mode & kPhaseMask;
+    a.fast_trig = (phase_mode & kFastTrigFlag) != 0;
+    hipLaunchKernelGGL(bf_eig_kernel, dim3((unsigned)cdiv(num_grid, 256)), dim3(256), 0, S(stream), a);
+    return hip_status("eig beamformer launch");
+}
+
+int abf_beamform_eig_custom(const double *eigval, const void *eigvec, int32_t num_eig, const void *steer,
+                            int64_t num_grid, int32_t num_mics, int32_t r_diag, double norm_factor, float *out,
+                            float *out_norm, void *, size_t, void *stream)
+{
+    if (num_mics < 1 || num_grid < 0 || num_eig < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (num_grid == 0) return ABF_OK;
+    if (!steer || !out || (num_eig > 0 && (!eigval || !eigvec))) return fail(ABF_ERR_INVALID, "null pointer");
+    EigArgs a{};
+    a.eigval = eigval;
+    a.eigvec_t = reinterpret_cast<const float2 *>(eigvec);
+    a.steer = reinterpret_cast<const float2 *>(steer);
+    a.out = out;
+    a.out_norm = out_norm;
+    a.norm = norm_factor;
+    a.G = num_grid;
+    a.M = num_mics;
+    a.K = num_eig;
+    a.steer_type = ABF_STEER_CLASSIC;
+    a.r_diag = r_diag;
+    hipLaunchKernelGGL(bf_eig_kernel, dim3((unsigned)cdiv(num_grid, 256)), dim3(256), 0, S(stream), a);
+    return hip_status("eig custom beamformer launch");
+}
+
+int abf_beamform_custom(const void *csm, const void *steer, int64_t num_grid, int32_t num_mics, int32_t r_diag,
+                        double norm_factor, float *out, float *out_norm, void *, size_t, void *stream)
+{
+    if (num_mics < 1 || num_grid < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (num_grid == 0) return ABF_OK;
+    if (!csm || !steer || !out) return fail(ABF_ERR_INVALID, "null pointer");
+    CustomArgs a{};
+    a.csm = reinterpret_cast<const float2 *>(csm);
+    a.steer = reinterpret_cast<const float2 *>(steer);
+    a.out = out;
+    a.out_norm = out_norm;
+    a.norm = norm_factor;
+    a.G = num_grid;
+    a.M = num_mics;
+    a.r_diag = r_diag;
+    hipLaunchKernelGGL(bf_custom_kernel, dim3((unsigned)cdiv(num_grid, 256)), dim3(256), 0, S(stream), a);
+    return hip_status("custom beamformer launch");
+}
+
+int abf_dist_mat(const double *gpos, const double *mpos, int64_t num_grid, int32_t num_mics, double *rm,
+                 void *stream)
+{
+    if (num_mics < 1 || num_grid < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (num_grid == 0) return ABF_OK;
+    if (!gpos || !mpos || !rm) return fail(ABF_ERR_INVALID, "null pointer");
+    hipLaunchKernelGGL(dist_mat_kernel, dim3((unsigned)cdiv(num_grid * num_mics, 256)), dim3(256), 0, S(stream),
+                       gpos, mpos, num_grid, (int)num_mics, rm);
+    return hip_status("dist_mat launch");
+}
+
+int abf_transfer(const double *r0, const double *rm, int64_t num_grid, int32_t num_mics, double wavenumber,
+                 int32_t phase_mode, int32_t steer_mode, int32_t out_c128, void *out, void *stream)
+{
+    if (num_mics < 1 || num_grid < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (steer_mode < 0 || steer_mode > 4) return fail(ABF_ERR_INVALID, "unknown steer mode");
+    if (int r = check_phase(phase_mode)) return r;
+    if (num_grid == 0) return ABF_OK;
+    if (!r0 || !rm || !out) return fail(ABF_ERR_INVALID, "null pointer");
+    hipLaunchKernelGGL(transfer_kernel, dim3((unsigned)cdiv(num_grid, 64)), dim3(64), 0, S(stream), r0, rm, num_grid,
+                       (int)num_mics, wavenumber, phase_mode & kPhaseMask, steer_mode, out_c128, out);
+    return hip_status("transfer launch");
+}
+
+int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mics, int32_t is_c128, void *stream)
+{
+    if (num_mics < 1 || num_freqs < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (num_freqs == 0) return ABF_OK;
+    if (!csm || !spec) return fail(ABF_ERR_INVALID, "null pointer");
+    const int64_t n = (int64_t)num_freqs * num_mics * num_mics;
+    if (is_c128)
+        hipLaunchKernelGGL(calc_csm_c128_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, S(stream),
+                           reinterpret_cast<double2 *>(csm), reinterpret_cast<const double2 *>(spec), num_freqs,
+                           num_mics);
+    else
+        hipLaunchKernelGGL(calc_csm_c64_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, S(stream),
+                           reinterpret_cast<float2 *>(csm), reinterpret_cast<const float2 *>(spec), num_freqs,
+                           num_mics);
+    return hip_status("calc_csm launch");
+}
+
+size_t abf_welch_csm_workspace_size(int32_t num_mics, int32_t block_size, int32_t blocks_per_pass)
+{
+    if (num_mics < 1 || block_size < 2 || blocks_per_pass < 1) return 0;
+    return align256((size_t)block_size / 2 * sizeof(float2)) +
+           align256((size_t)(block_size / 2 + 1) * blocks_per_pass * num_mics * sizeof(float2));
+}
+
+int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int32_t block_size, int32_t hop,
+                  int32_t num_blocks, const float *window, double scale, void *csm_out, int32_t blocks_per_pass,
+                  void *workspace, size_t workspace_bytes, void *stream)
+{
+    const int bs = block_size;
+    if (num_mics < 1 || bs < 2 || (bs & (bs - 1)) || hop < 1 || num_blocks < 0 || blocks_per_pass < 1)
+        return fail(ABF_ERR_INVALID, "bad Welch parameters");
+    if (bs > 16384) return fail(ABF_ERR_UNSUPPORTED, "block_size above 16384 is not supported by the LDS FFT");
+    if (num_blocks > 0 && (int64_t)(num_blocks - 1) * hop + bs > num_samples)
+        return fail(ABF_ERR_INVALID, "blocks exceed the time data");
+    if (!csm_out || (num_blocks > 0 && (!time || !window))) return fail(ABF_ERR_INVALID, "null pointer");
+    const size_t need = abf_welch_csm_workspace_size(num_mics, bs, blocks_per_pass);
+    if (!workspace || workspace_bytes < need) return fail(ABF_ERR_WORKSPACE, "workspace too small");
+    const hipStream_t s = S(stream);
+    const int M = num_mics, F = bs / 2 + 1;
+    int logbs = 0;
+    while ((1 << logbs) < bs) ++logbs;
+    char *ws = reinterpret_cast<char *>(workspace);
+    float2 *tw = reinterpret_cast<float2 *>(ws);
+    float2 *spec = reinterpret_cast<float2 *>(ws + align256((size_t)bs / 2 * sizeof(float2)));
+    const int64_t n4 = (int64_t)F * M * M / 2; // complex64 pairs per float4
+    const int64_t total = (int64_t)F * M * M;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)cdiv(cdiv(total, 2), 256)), dim3(256), 0, s,
+                       reinterpret_cast<float4 *>(csm_out), n4);
+    if (total & 1) // odd element count: clear the last complex64 separately
+        hipLaunchKernelGGL(zero_c64_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<float2 *>(csm_out) + (total - 1));
+    hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)welch_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
+        attr = true;
+    }
+    const int P = std::max(1, std::min({16, 16384 / bs, (M + 1) / 2}));
+    const int nT = (M + 63) / 64;
+    for (int b0 = 0; b0 < num_blocks; b0 += blocks_per_pass) {
+        const int nb = std::min(blocks_per_pass, num_blocks - b0);
+        WelchFftArgs a{};
+        a.time = time;
+        a.window = window;
+        a.twiddle = tw;
+        a.spec = spec;
+        a.N = num_samples;
+        a.M = M;
+        a.bs = bs;
+        a.logbs = logbs;
+        a.hop = hop;
+        a.b0 = b0;
+        a.nb = nb;
+        a.P = P;
+        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)cdiv(M, 2 * P), nb), dim3(256),
+                           (size_t)P * bs * sizeof(float2), s, a);
+        hipLaunchKernelGGL(welch_herk_kernel, dim3(nT * nT, F), dim3(64), 0, s, spec, nb, M, nT,
+                           reinterpret_cast<float2 *>(csm_out));
+    }
+    hipLaunchKernelGGL(welch_finalize_kernel, dim3((unsigned)cdiv((int64_t)M * M, 256), F), dim3(256), 0, s,
+                       reinterpret_cast<float2 *>(csm_out), M, (float)scale);
+    return hip_status("Welch CSM launch");
+}
+
+} // extern "C"

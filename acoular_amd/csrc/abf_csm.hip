@@ -1,0 +1,178 @@
+// Cross-spectral matrix kernels for gfx950.
+//
+//  * calc_csm_kernel: the in-place upper-triangle rank-1 update of calcCSM
+//    (acoular/fastFuncs.py:30-58), complex64 or complex128.
+//  * Welch CSM (PowerSpectra.calc_csm, acoular/spectra.py:366-416):
+//      welch_fft_kernel      windowed real FFT of every (block, channel), two channels packed
+//                            into one complex radix-2 FFT in LDS;
+//      welch_herk_kernel     C_f += X_f X_f^H over the blocks of a pass on fp32 MFMA
+//                            (16x16x4), upper 64x64 tiles only;
+//      welch_finalize_kernel Hermitian completion from the upper triangle with a real
+//                            diagonal (spectra.py:412-414) and the 2/(bs w num_blocks) scaling
+//                            (spectra.py:416).
+#include "abf_common.h"
+#include "abf_csm_kernels.h"
+
+namespace abf {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__global__ void fill_zero_kernel(float4 *__restrict__ p, int64_t n4)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void calc_csm_c64_kernel(float2 *__restrict__ csm, const float2 *__restrict__ spec, int F, int M)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t mm = (int64_t)M * M;
+    if (idx >= (int64_t)F * mm) return;
+    const int f = (int)(idx / mm);
+    const int r = (int)((idx / M) % M), c = (int)(idx % M);
+    if (r > c) return;
+    const float2 sr = spec[(int64_t)f * M + r], sc = spec[(int64_t)f * M + c];
+    float2 v = csm[idx];
+    v.x += sc.x * sr.x + sc.y * sr.y;
+    v.y += sc.x * sr.y - sc.y * sr.x;
+    csm[idx] = v;
+}
+
+__global__ void calc_csm_c128_kernel(double2 *__restrict__ csm, const double2 *__restrict__ spec, int F, int M)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t mm = (int64_t)M * M;
+    if (idx >= (int64_t)F * mm) return;
+    const int f = (int)(idx / mm);
+    const int r = (int)((idx / M) % M), c = (int)(idx % M);
+    if (r > c) return;
+    const double2 sr = spec[(int64_t)f * M + r], sc = spec[(int64_t)f * M + c];
+    double2 v = csm[idx];
+    v.x += sc.x * sr.x + sc.y * sr.y;
+    v.y += sc.x * sr.y - sc.y * sr.x;
+    csm[idx] = v;
+}
+
+__device__ __forceinline__ int bitrev(int n, int bits) { return (int)(__builtin_bitreverse32((uint32_t)n) >> (32 - bits)); }
+
+// grid: (ceil(M / 2P), nb); P channel pairs of one Welch block per workgroup
+__global__ __launch_bounds__(256) void welch_fft_kernel(WelchFftArgs a)
+{
+    extern __shared__ float2 z[]; // [P][bs]
+    const int bs = a.bs, P = a.P;
+    const int bl = blockIdx.y;
+    const int m0 = blockIdx.x * 2 * P;
+    const int64_t t0 = (int64_t)(a.b0 + bl) * a.hop;
+    for (int idx = threadIdx.x; idx < P * bs; idx += blockDim.x) {
+        const int p = idx % P, n = idx / P;
+        const int ma = m0 + 2 * p, mb = ma + 1;
+        const float w = a.window[n];
+        const float *row = a.time + (t0 + n) * a.M;
+        const float xa = ma < a.M ? row[ma] : 0.f;
+        const float xb = mb < a.M ? row[mb] : 0.f;
+        z[p * bs + bitrev(n, a.logbs)] = make_float2(w * xa, w * xb);
+    }
+    __syncthreads();
+    const int nbf = bs / 2;
+    for (int s = 1; s <= a.logbs; ++s) {
+        const int half = 1 << (s - 1);
+        const int tstride = bs >> s;
+        for (int idx = threadIdx.x; idx < P * nbf; idx += blockDim.x) {
+            const int p = idx / nbf, bi = idx % nbf;
+            const int grp = bi / half, pos = bi % half;
+            const int i0 = p * bs + grp * 2 * half + pos, i1 = i0 + half;
+            const float2 w = a.twiddle[pos * tstride];
+            const float2 x0 = z[i0], x1 = z[i1];
+            const float tr = x1.x * w.x - x1.y * w.y;
+            const float ti = x1.x * w.y + x1.y * w.x;
+            z[i0] = make_float2(x0.x + tr, x0.y + ti);
+            z[i1] = make_float2(x0.x - tr, x0.y - ti);
+        }
+        __syncthreads();
+    }
+    const int F = bs / 2 + 1;
+    for (int idx = threadIdx.x; idx < P * F; idx += blockDim.x) {
+        const int p = idx % P, k = idx / P;
+        const float2 zk = z[p * bs + k];
+        const float2 zn = z[p * bs + ((bs - k) & (bs - 1))];
+        const int ma = m0 + 2 * p, mb = ma + 1;
+        float2 *dst = a.spec + ((int64_t)k * a.nb + bl) * a.M;
+        if (ma < a.M) dst[ma] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+        if (mb < a.M) dst[mb] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+    }
+}
+
+// grid: (nT * nT, F), one wave per upper 64x64 tile of one frequency
+__global__ __launch_bounds__(64) void welch_herk_kernel(const float2 *__restrict__ spec, int nb, int M, int nT,
+                                                        float2 *__restrict__ csm)
+{
+    const int I = blockIdx.x / nT, J = blockIdx.x % nT;
+    if (J < I) return;
+    const int f = blockIdx.y;
+    const int l = threadIdx.x;
+    const float2 *X = spec + (int64_t)f * nb * M;
+    f4 accr[4][4], acci[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            accr[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+            acci[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    for (int k0 = 0; k0 < nb; k0 += 4) {
+        const int b = k0 + (l >> 4);
+        const bool bv = b < nb;
+        float2 ra[4], ca[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int r = I * 64 + t * 16 + (l & 15);
+            const int c = J * 64 + t * 16 + (l & 15);
+            ra[t] = (bv && r < M) ? X[(int64_t)b * M + r] : make_float2(0.f, 0.f);
+            ca[t] = (bv && c < M) ? X[(int64_t)b * M + c] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                // C_rc = sum_b X_r conj(X_c)
+                accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ra[ti].x, ca[tj].x, accr[ti][tj], 0, 0, 0);
+                accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ra[ti].y, ca[tj].y, accr[ti][tj], 0, 0, 0);
+                acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ra[ti].y, ca[tj].x, acci[ti][tj], 0, 0, 0);
+                acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ra[ti].x, ca[tj].y, acci[ti][tj], 0, 0, 0);
+            }
+    }
+    float2 *C = csm + (int64_t)f * M * M;
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = I * 64 + ti * 16 + (l >> 4) * 4 + i;
+                const int c = J * 64 + tj * 16 + (l & 15);
+                if (r < M && c < M && r <= c) {
+                    float2 v = C[(int64_t)r * M + c];
+                    v.x += accr[ti][tj][i];
+                    v.y += acci[ti][tj][i];
+                    C[(int64_t)r * M + c] = v;
+                }
+            }
+}
+
+__global__ void welch_finalize_kernel(float2 *__restrict__ csm, int M, float scale)
+{
+    const int f = blockIdx.y;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)M * M) return;
+    const int r = (int)(idx / M), c = (int)(idx % M);
+    if (r > c) return;
+    float2 *C = csm + (int64_t)f * M * M;
+    float2 v = C[(int64_t)r * M + c];
+    v.x *= scale;
+    v.y *= scale;
+    if (r == c) v.y = 0.f;
+    C[(int64_t)r * M + c] = v;
+    if (r != c) C[(int64_t)c * M + r] = make_float2(v.x, -v.y);
+}
+
+} // namespace abf

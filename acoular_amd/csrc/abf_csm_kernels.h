@@ -1,0 +1,23 @@
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace abf {
+
+struct WelchFftArgs {
+    const float *time;     // [N][M]
+    const float *window;   // [bs]
+    const float2 *twiddle; // [bs/2]  exp(-2 pi i j / bs)
+    float2 *spec;          // [bs/2+1][nb][M]
+    int64_t N;
+    int M, bs, logbs, hop, b0, nb, P;
+};
+
+__global__ void fill_zero_kernel(float4 *p, int64_t n4);
+__global__ void calc_csm_c64_kernel(float2 *csm, const float2 *spec, int F, int M);
+__global__ void calc_csm_c128_kernel(double2 *csm, const double2 *spec, int F, int M);
+__global__ void welch_fft_kernel(WelchFftArgs a);
+__global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nT, float2 *csm);
+__global__ void welch_finalize_kernel(float2 *csm, int M, float scale);
+
+} // namespace abf

@@ -1,0 +1,71 @@
+// Kernel argument blocks and launch declarations shared between the kernel sources and the
+// C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace abf {
+
+struct PlainArgs {
+    const float2 *ut;          // [F][Mp][Mp] U^T
+    const double *gpos;        // [3][G] or null when rm_table is used
+    const double *mpos;        // [3][M]
+    const double *rm_table;    // [G][M] or null
+    const double *r0_table;    // [G] or null
+    const double *wavenumber;  // [F]
+    float *out;                // [F][G]
+    float *out_norm;           // [F][G] or null
+    double ref[3];
+    double ref_dist;
+    double norm;
+    int64_t G;
+    int M, Mp;
+    int steer_type, phase_mode, clip, fast_trig;
+};
+
+// one beamforming job of an MFMA launch
+struct MfJob {
+    const void *prep;          // U in fragment order, [F][nUpper][2][4 KiB]
+    const double *gpos;        // [3][G]
+    const double *mpos;        // [3][M]
+    const double *wavenumber;  // [F]
+    float *out;                // [F][G]
+    float *out_norm;           // [F][G] or null
+    double ref[3];
+    double ref_dist;
+    double norm;
+    int64_t G;
+    int64_t wg_begin;          // first workgroup of this job in the launch
+    int M, Mp, CT, F, FB, ntiles;
+    int steer_type, phase_mode, clip, fast_trig;
+};
+
+struct EigArgs {
+    const double *eigval;      // [K]
+    const float2 *eigvec_t;    // [K][M] (transposed eigenvectors)
+    const float2 *steer;       // [G][M] explicit steering or null
+    const double *r0_table;    // [G]
+    const double *rm_table;    // [G][M]
+    float *out, *out_norm;
+    double wavenumber, norm;
+    int64_t G;
+    int M, K, steer_type, r_diag, phase_mode, fast_trig;
+};
+
+struct CustomArgs {
+    const float2 *csm;         // [M][M]
+    const float2 *steer;       // [G][M]
+    float *out, *out_norm;
+    double norm;
+    int64_t G;
+    int M, r_diag;
+};
+
+__global__ void prep_plain_kernel(const float2 *csm, int M, int Mp, int r_diag, float2 *ut);
+__global__ void bf_plain_kernel(PlainArgs a);
+__global__ void prep_mfma_kernel(const float2 *csm, int M, int Mp, int r_diag, int bf16x3, char *prep);
+template <bool BF16X3, int CW> __global__ void bf_mfma_kernel(const MfJob *jobs, int njobs);
+__global__ void bf_eig_kernel(EigArgs a);
+__global__ void bf_custom_kernel(CustomArgs a);
+
+} // namespace abf

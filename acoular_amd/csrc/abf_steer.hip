@@ -1,0 +1,74 @@
+// Geometry and steering-vector kernels: dist_mat (acoular/environments.py:65-103) and
+// calcTransfer + steer_vector (acoular/fastFuncs.py:859-897, acoular/fbeamform.py:145-153).
+#include "abf_common.h"
+#include "abf_steer_kernels.h"
+
+namespace abf {
+
+__global__ void dist_mat_kernel(const double *__restrict__ gpos, const double *__restrict__ mpos, int64_t G, int M,
+                                double *__restrict__ rm)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= G * M) return;
+    const int64_t g = idx / M;
+    const int m = (int)(idx % M);
+    rm[idx] = dist3(gpos[g], gpos[G + g], gpos[2 * G + g], mpos[m], mpos[M + m], mpos[2 * M + m]);
+}
+
+// One grid point per thread: a[g][m] = exp(-1j f32(k (rm - r0))) r0 / rm, then the optional
+// steer_vector normalisation; float64 except for the float32 phase (as the reference).
+__global__ void transfer_kernel(const double *__restrict__ r0, const double *__restrict__ rm, int64_t G, int M,
+                                double k, int phase_mode, int steer_mode, int out_c128, void *__restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const double rr0 = r0[g];
+    double s2 = 0.0; // sum |a|^2 for true level / true location
+    if (steer_mode == 3 || steer_mode == 4) {
+        for (int m = 0; m < M; ++m) {
+            const double q = rr0 / rm[g * M + m];
+            s2 += q * q;
+        }
+    }
+    for (int m = 0; m < M; ++m) {
+        const double r = rm[g * M + m];
+        float c, s;
+        phasor(k * (r - rr0), phase_mode, c, s);
+        const double amp = rr0 / r;
+        double ar = (double)c * amp, ai = -(double)s * amp;
+        switch (steer_mode) {
+        case 1: { // x / |x| / M
+            const double mag = sqrt(ar * ar + ai * ai);
+            ar = ar / mag / M;
+            ai = ai / mag / M;
+            break;
+        }
+        case 2: { // 1 / conj(x) / M
+            const double d = ar * ar + ai * ai;
+            const double br = ar / d, bi = ai / d; // 1/conj(x) = x / |x|^2
+            ar = br / M;
+            ai = bi / M;
+            break;
+        }
+        case 3:
+            ar /= s2;
+            ai /= s2;
+            break;
+        case 4: {
+            const double d = sqrt(s2 * M);
+            ar /= d;
+            ai /= d;
+            break;
+        }
+        default:
+            break;
+        }
+        if (out_c128) {
+            reinterpret_cast<double2 *>(out)[g * M + m] = make_double2(ar, ai);
+        } else {
+            reinterpret_cast<float2 *>(out)[g * M + m] = make_float2((float)ar, (float)ai);
+        }
+    }
+}
+
+} // namespace abf

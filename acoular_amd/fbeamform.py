@@ -1,0 +1,385 @@
+"""Frequency-domain beamforming (``acoular/fbeamform.py``): ``SteeringVector``,
+``BeamformerBase`` with lazy per-frequency results, ``L_p`` and ``integrate``.
+
+``BeamformerBase._calc`` batches every missing frequency of a request into one GPU job
+(``abf_beamform``): the CSM stays resident on the device, steering vectors are generated inside
+the kernels from grid and microphone coordinates, and only the map comes back to the host.
+"""
+
+from __future__ import annotations
+
+from warnings import warn as _warn
+
+import numpy as np
+
+from . import _lib
+from ._base import HasParams
+from .environments import Environment
+from .fastFuncs import beamformerFreq, calcTransfer
+from .grids import Grid
+from .microphones import MicGeom
+from .spectra import PowerSpectra
+
+STEER_TYPES = ('true level', 'true location', 'classic', 'inverse')
+
+#: defaults of the GPU path (not reference traits)
+config = {'form': 'mfma_bf16x3', 'phase_mode': 'reference', 'fast_trig': True}
+
+
+class SteeringVector(HasParams):
+    """Monopole steering vectors of the four formulations (``fbeamform.py:105-258``)."""
+
+    _params = ('grid', 'mics', 'steer_type', 'env', 'ref')
+
+    def __init__(self, **kwargs):
+        self.grid = None
+        self.mics = None
+        self.steer_type = 'true level'
+        self.env = Environment()
+        self.ref = np.array([0.0, 0.0, 0.0])
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name == 'steer_type' and value not in STEER_TYPES:
+            raise ValueError(f'steer_type must be one of {STEER_TYPES}')
+        if name == 'grid' and value is not None and not isinstance(value, Grid):
+            raise TypeError('grid must be a Grid')
+        if name == 'mics' and value is not None and not isinstance(value, MicGeom):
+            raise TypeError('mics must be a MicGeom')
+        if name == 'ref':
+            if np.isscalar(value):
+                try:
+                    value = np.abs(float(value))
+                except ValueError as ve:
+                    raise TypeError('ref must be a float or a 3-vector') from ve
+            elif len(value) == 3:
+                value = np.array(value, dtype=float)
+            else:
+                raise TypeError('ref must be a float or a 3-vector')
+        object.__setattr__(self, name, value)
+
+    @property
+    def r0(self):
+        """Distances of the grid points to the reference (``fbeamform.py:187-193``)."""
+
+        def calc():
+            if np.isscalar(self.ref):
+                if self.ref > 0:
+                    return np.full((self.grid.size,), self.ref)
+                return self.env.apparent_r(self.grid.pos)
+            return self.env.apparent_r(self.grid.pos, self.ref[:, np.newaxis])
+
+        return self._cached('r0', calc)
+
+    @property
+    def rm(self):
+        """Distances of the grid points to the microphones (``fbeamform.py:195-197``)."""
+        return self._cached('rm', lambda: np.atleast_2d(self.env.apparent_r(self.grid.pos, self.mics.pos)))
+
+    def device_geometry(self):
+        """Grid and microphone coordinates resident on the device (float64, SoA)."""
+
+        def calc():
+            return (_lib.to_device(np.ascontiguousarray(self.grid.pos, dtype=np.float64)),
+                    _lib.to_device(np.ascontiguousarray(self.mics.pos, dtype=np.float64)))
+
+        return self._cached('dev_geom', calc)
+
+    def _k(self, f):
+        return 2 * np.pi * f / self.env.c
+
+    def transfer(self, f, ind=None):
+        """Transfer matrix for one frequency (``fbeamform.py:207-235``), complex128."""
+        if ind is None:
+            return calcTransfer(self.r0, self.rm, np.array(self._k(f)))
+        if not isinstance(ind, np.ndarray):
+            return calcTransfer(self.r0[ind], self.rm[ind, :][np.newaxis], np.array(self._k(f)))
+        return calcTransfer(self.r0[ind], self.rm[ind, :], np.array(self._k(f)))
+
+    def steer_vector(self, f, ind=None):
+        """Steering vectors (``fbeamform.py:237-258``): the transfer function normalised per
+        ``steer_type`` (``fbeamform.py:145-153``), computed on the GPU."""
+        from .fastFuncs import _transfer_device
+
+        mode = {'classic': 1, 'inverse': 2, 'true level': 3, 'true location': 4}[self.steer_type]
+        if ind is None:
+            r0, rm = self.r0, self.rm
+        elif not isinstance(ind, np.ndarray):
+            r0, rm = self.r0[ind], self.rm[ind, :][np.newaxis]
+        else:
+            r0, rm = self.r0[ind], self.rm[ind, :]
+        return _transfer_device(np.atleast_1d(r0), rm, self._k(f), mode)
+
+
+class LazyBfResult:
+    """Per-frequency lazy evaluation of ``BeamformerBase.result`` (``fbeamform.py:261-286``)."""
+
+    def __init__(self, bf):
+        self.bf = bf
+
+    def __getitem__(self, key):
+        sl = np.index_exp[key][0]
+        if isinstance(sl, (int, np.integer)):
+            sl = slice(sl, sl + 1)
+        missingind = np.arange(*sl.indices(self.bf._numfreq))[self.bf._fr[sl] == 0]
+        if missingind.size:
+            self.bf._calc(missingind)
+        return self.bf._ac.__getitem__(key)
+
+
+class BeamformerBase(HasParams):
+    """Conventional (delay-and-sum) beamformer in the frequency domain (``fbeamform.py:289-685``).
+
+    Parameters beyond the reference's: ``form`` ('mfma_bf16x3', 'mfma_f32' or 'plain_f32') and
+    ``phase_mode`` ('reference': the float32-rounded phase of ``fastFuncs.py:260``, or 'fp64').
+    """
+
+    _params = ('freq_data', 'steer', 'r_diag', 'r_diag_norm', 'precision', 'form', 'phase_mode')
+
+    def __init__(self, **kwargs):
+        self.freq_data = None
+        self.steer = SteeringVector()
+        self.r_diag = True
+        self.r_diag_norm = 0.0
+        self.precision = 'float64'
+        self.cached = True  # accepted for API compatibility (no HDF5 cache in this build)
+        self.form = config['form']
+        self.phase_mode = config['phase_mode']
+        cached = kwargs.pop('cached', True)
+        super().__init__(**kwargs)
+        self.cached = cached
+        object.__setattr__(self, '_result_digest', None)
+
+    def __setattr__(self, name, value):
+        if name == 'precision' and value not in ('float64', 'float32'):
+            raise ValueError("precision must be 'float64' or 'float32'")
+        if name == 'form' and value not in _lib.FORMS:
+            raise ValueError(f'form must be one of {list(_lib.FORMS)}')
+        if name == 'phase_mode' and value not in _lib.PHASE_MODES:
+            raise ValueError(f'phase_mode must be one of {list(_lib.PHASE_MODES)}')
+        if name == 'freq_data' and value is not None and not isinstance(value, PowerSpectra):
+            raise TypeError('freq_data must be a PowerSpectra')
+        if name == 'steer' and value is not None and not isinstance(value, SteeringVector):
+            raise TypeError('steer must be a SteeringVector')
+        object.__setattr__(self, name, value)
+
+    # -- reference API ----------------------------------------------------------------------
+    def _assert_equal_channels(self):
+        num_channels = self.freq_data.num_channels
+        if num_channels != self.steer.mics.num_mics or num_channels == 0:
+            msg = f'{num_channels:d} channels do not fit {self.steer.mics.num_mics:d} mics'
+            raise ValueError(msg)
+
+    @property
+    def result(self):
+        """Lazy ``[F, G]`` map (``fbeamform.py:403-436``); recomputed when parameters change."""
+        d = self.digest
+        if self._result_digest != d:
+            self._f = self.freq_data.fftfreq()
+            self._numfreq = self._f.shape[0]
+            self._assert_equal_channels()
+            self._ac = np.zeros((self._numfreq, self.steer.grid.size), dtype=self.precision)
+            self._fr = np.zeros(self._numfreq, dtype='int8')
+            object.__setattr__(self, '_result_digest', d)
+        return LazyBfResult(self)
+
+    def sig_loss_norm(self):
+        """Normalisation for the removed CSM diagonal (``fbeamform.py:438-451``)."""
+        if not self.r_diag:
+            return 1.0
+        if self.r_diag_norm == 0.0:
+            nm = float(self.freq_data.num_channels)
+            return nm / (nm - 1)
+        return self.r_diag_norm
+
+    def _beamformer_params(self):
+        """``fbeamform.py:453-473``: the fast path only for exactly ``SteeringVector``."""
+        if type(self.steer) is SteeringVector:
+            param_type = self.steer.steer_type
+
+            def param_steer_func(f):
+                return (self.steer.r0, self.steer.rm, 2 * np.pi * f / self.steer.env.c)
+
+        else:
+            param_type = 'custom'
+            param_steer_func = self.steer.steer_vector
+        return param_type, param_steer_func
+
+    def _calc(self, ind):
+        """Compute the frequencies ``ind`` (``fbeamform.py:475-510``) in one GPU job."""
+        ind = np.asarray(ind, dtype=np.int64)
+        normfactor = self.sig_loss_norm()
+        param_type, steer_func = self._beamformer_params()
+        if param_type == 'custom':
+            for i in ind:
+                csm = np.array(self.freq_data.csm[i], dtype='complex128')
+                out = beamformerFreq('custom', self.r_diag, normfactor, steer_func(self._f[i]), csm)[0]
+                if self.r_diag:
+                    out[np.sign(out) < 0] = 0.0
+                self._ac[i] = out
+                self._fr[i] = 1
+            return
+        maps = beamform_device(self.freq_data.csm_device, self._f[ind], ind, self.steer, param_type, self.r_diag,
+                               normfactor, self.form, self.phase_mode, clip=self.r_diag)
+        self._ac[ind] = maps.cpu().numpy().astype(self.precision)
+        self._fr[ind] = 1
+
+    def synthetic(self, f, num=0):
+        """Band-summed map at ``f`` (``fbeamform.py:512-593``), reshaped to ``grid.shape``."""
+        res = self.result
+        freq = self.freq_data.fftfreq()
+        if len(freq) == 0:
+            return None
+        if num == 0:
+            ind = np.searchsorted(freq, f)
+            if ind >= len(freq):
+                _warn(f'Queried frequency ({f:g} Hz) not in resolved frequency range. Returning zeros.', Warning,
+                      stacklevel=2)
+                h = np.zeros_like(res[0])
+            else:
+                if freq[ind] != f:
+                    _warn(f'Queried frequency ({f:g} Hz) not in set of discrete FFT sample frequencies. '
+                          f'Using frequency {freq[ind]:g} Hz instead.', Warning, stacklevel=2)
+                h = res[ind]
+        else:
+            if isinstance(num, list):
+                f1, f2 = num[0], num[-1]
+            else:
+                f1 = f * 2.0 ** (-0.5 / num)
+                f2 = f * 2.0 ** (+0.5 / num)
+            ind1 = np.searchsorted(freq, f1)
+            ind2 = np.searchsorted(freq, f2)
+            if ind1 == ind2:
+                _warn(f'Queried frequency band ({f1:g} to {f2:g} Hz) does not include any discrete FFT sample '
+                      'frequencies. Returning zeros.', Warning, stacklevel=2)
+                h = np.zeros_like(res[0])
+            else:
+                h = np.sum(res[ind1:ind2], 0)
+        return h.reshape(self.steer.grid.shape)
+
+    def integrate(self, sector, frange=None, num=0):
+        """Sector-integrated spectrum (``fbeamform.py:595-685``)."""
+        if hasattr(self.steer.grid, 'indices'):
+            ind = self.steer.grid.indices(*sector)
+        else:
+            msg = f'Grid of type {self.steer.grid.__class__.__name__} does not have an indices method!'
+            raise NotImplementedError(msg)
+        gshape = self.steer.grid.shape
+        num_freqs = self.freq_data.fftfreq().shape[0]
+        if num == 0 or frange is None:
+            if frange is None:
+                ind_low = self.freq_data.ind_low
+                ind_high = self.freq_data.ind_high
+                if ind_low is None:
+                    ind_low = 0
+                if ind_high is None:
+                    ind_high = num_freqs
+                irange = (ind_low, ind_high)
+                num = 0
+            elif len(frange) == 2:
+                _ = self.result
+                irange = (np.searchsorted(self._f, frange[0]), np.searchsorted(self._f, frange[1]))
+            else:
+                msg = 'Only a tuple of length 2 is allowed for frange if num==0'
+                raise TypeError(msg)
+            h = np.zeros(num_freqs, dtype=float)
+            sl = slice(*irange)
+            r = self.result[sl]
+            for i in range(num_freqs)[sl]:
+                h[i] = r[i - sl.start].reshape(gshape)[ind].sum()
+            if frange is None:
+                return h
+            return self._f[sl], h[sl]
+        h = np.zeros(len(frange), dtype=float)
+        for i, f in enumerate(frange):
+            h[i] = self.synthetic(f, num).reshape(gshape)[ind].sum()
+        return h
+
+
+def beamform_device(csm_device, freqs, ind, steer: SteeringVector, steer_type: str, r_diag: bool, norm: float,
+                    form: str = 'mfma_bf16x3', phase_mode: str = 'reference', clip: bool = False,
+                    out_norm: bool = False, fast_trig: bool | None = None):
+    """One ``abf_beamform`` job: the maps of CSM slices ``ind`` at frequencies ``freqs``.
+
+    Returns a float32 device tensor ``[len(ind), G]`` (and the steer normalisation if asked).
+    """
+    torch = _lib.torch()
+    ind = np.asarray(ind, dtype=np.int64)
+    freqs = np.asarray(freqs, dtype=np.float64)
+    dev = _lib.device()
+    csm = csm_device
+    if len(ind) != csm.shape[0] or not np.array_equal(ind, np.arange(csm.shape[0])):
+        csm = csm.index_select(0, torch.from_numpy(ind).to(dev))
+    csm = csm.contiguous()
+    gpos, mpos = steer.device_geometry()
+    G, M, F = gpos.shape[1], mpos.shape[1], len(ind)
+    if csm.shape[1] != M:
+        msg = f'{csm.shape[1]:d} channels do not fit {M:d} mics'
+        raise ValueError(msg)
+    if M > 1150 and form != 'plain_f32':
+        form = 'plain_f32'
+    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    out = torch.empty((F, G), dtype=torch.float32, device=dev)
+    onorm = torch.empty((F, G), dtype=torch.float32, device=dev) if out_norm else None
+    job = make_job(csm, k, gpos, mpos, steer.ref, steer_type, r_diag, norm, form, phase_mode, out, onorm, clip,
+                   fast_trig)
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
+    _lib.check(lib.abf_beamform(job, _lib.ptr(ws), ws.numel(), _lib.stream_handle()), 'beamform')
+    job._keep = (csm, k, gpos, mpos, out, onorm, ws)
+    if out_norm:
+        return out, onorm
+    return out
+
+
+def make_job(csm, k, gpos, mpos, ref, steer_type, r_diag, norm, form, phase_mode, out, out_norm=None, clip=False,
+             fast_trig=None) -> _lib.AbfGridJob:
+    """Fill an ``abf_grid_job`` from device tensors (the tensors must outlive the call)."""
+    if fast_trig is None:
+        fast_trig = config['fast_trig']
+    j = _lib.AbfGridJob()
+    j.geom.gpos = gpos.data_ptr()
+    j.geom.mpos = mpos.data_ptr()
+    j.geom.num_grid = gpos.shape[1]
+    j.geom.num_mics = mpos.shape[1]
+    if np.isscalar(ref):
+        if ref > 0:
+            j.geom.ref_dist = float(ref)
+        else:
+            j.geom.ref_dist = 0.0
+            j.geom.ref[:] = [0.0, 0.0, 0.0]
+    else:
+        j.geom.ref[:] = [float(x) for x in ref]
+        j.geom.ref_dist = 0.0
+    j.csm = csm.data_ptr()
+    j.wavenumber = k.data_ptr()
+    j.num_freqs = csm.shape[0]
+    j.steer_type = _lib.STEER_CODES[steer_type]
+    j.r_diag = int(bool(r_diag))
+    j.clip_negative = int(bool(clip))
+    j.norm_factor = float(norm)
+    j.form = _lib.FORMS[form]
+    j.phase_mode = _lib.PHASE_MODES[phase_mode] | (_lib.FAST_TRIG if fast_trig else 0)
+    j.out = out.data_ptr()
+    j.out_norm = out_norm.data_ptr() if out_norm is not None else None
+    return j
+
+
+def L_p(x):  # noqa: N802
+    """Sound pressure level from squared pressure (``fbeamform.py:2412-2432``)."""
+    return 10 * np.log10(np.maximum(x / 4e-10, np.finfo(x.dtype).eps))
+
+
+def integrate(data, grid, sector):
+    """Integrate a map over a sector (``fbeamform.py:2435-2497``), rectangular/circular sectors."""
+    ind = grid.indices(*sector)
+    gshape = grid.shape
+    gsize = grid.size
+    if np.size(data) == gsize:  # one value per grid point
+        return data.reshape(gshape)[ind].sum()
+    if data.ndim == 2 and data.shape[1] == gsize:
+        h = np.zeros(data.shape[0])
+        for i in range(data.shape[0]):
+            h[i] = data[i].reshape(gshape)[ind].sum()
+        return h
+    return None

@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Benchmark of the frequency-domain beamforming hot path on one MI355X.
+
+Workload (the plain-vs-MFMA configuration): 256 seeded random microphones in a 2 m disk,
+8 seeded monopoles with a 20 dB level spread, analytic complex64 CSM of 513 bins
+(block_size 1024 at fs 51200 Hz), RectGrid 201 x 201 (40,401 points) at z = 0.5,
+steer_type 'classic', r_diag True.  One step = the whole path over the resident CSM:
+re-layout of the CSM (abf_beamform_prepare) + beamforming of all 513 bins x 40,401 points
+(abf_beamform_run).  Metric: map-bins/s.
+
+Printed: one JSON line (stdout) with the throughput, the roofline of the dominant kernel
+(bf_mfma_kernel) timed with HIP events on the launch stream, the other forms' throughput, the
+max relative map error against the CPU oracle on sampled bins and the CPU baseline
+(oracle/cpu_ref.c timed on the host cores on a bounded sample of the same workload).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_PEAK = {'bf16': 2500.0, 'fp32_mfma': 157.3, 'fp32': 157.3}  # TFLOP/s, dense
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(dev, seed=2024):
+    import torch
+
+    from acoular_amd import synthetic
+
+    nm, fs, bs = 256, 51200.0, 1024
+    mpos = synthetic.disk_array(nm, 2.0, seed)
+    spos = synthetic.source_positions(8, [-0.8, -0.8, 0.5], [0.8, 0.8, 0.5], seed + 1)
+    powers = synthetic.level_powers(8, 20.0, seed + 2)
+    freqs = np.arange(bs // 2 + 1) * fs / bs
+    x = np.linspace(-1.0, 1.0, 201)
+    gpos = np.empty((3, 201 * 201))
+    gpos[0] = np.repeat(x, 201)
+    gpos[1] = np.tile(x, 201)
+    gpos[2] = 0.5
+    csm = synthetic.analytic_csm(mpos, spos, powers, freqs, device=dev)
+    torch.cuda.synchronize()
+    return dict(mpos=mpos, spos=spos, powers=powers, freqs=freqs, gpos=gpos, csm=csm, nm=nm)
+
+
+class Runner:
+    """One prepared job of a given form over the resident workload."""
+
+    def __init__(self, wl, form, steer='classic', r_diag=True, dev=None):
+        import torch
+
+        from acoular_amd import _lib
+        from acoular_amd.fbeamform import make_job
+
+        self.lib = _lib.lib()
+        self._lib = _lib
+        nm = wl['nm']
+        self.gpos = torch.from_numpy(wl['gpos']).to(dev)
+        self.mpos = torch.from_numpy(wl['mpos']).to(dev)
+        self.k = torch.from_numpy(2 * np.pi * wl['freqs'] / 343.0).to(dev)
+        self.out = torch.empty((len(wl['freqs']), wl['gpos'].shape[1]), dtype=torch.float32, device=dev)
+        norm = nm / (nm - 1.0) if r_diag else 1.0
+        self.job = make_job(wl['csm'], self.k, self.gpos, self.mpos, np.zeros(3), steer, r_diag, norm, form,
+                            'reference', self.out, None, clip=r_diag)
+        self.ws = _lib.workspace(self.lib.abf_beamform_workspace_size(self.job))
+        self.csm = wl['csm']
+        self.map_bins = self.out.numel()
+
+    def prepare(self):
+        s = self._lib.stream_handle()
+        self._lib.check(self.lib.abf_beamform_prepare(self.job, self._lib.ptr(self.ws), self.ws.numel(), s))
+
+    def run(self):
+        s = self._lib.stream_handle()
+        self._lib.check(self.lib.abf_beamform_run(self.job, self._lib.ptr(self.ws), self.ws.numel(), s))
+
+    def step(self):
+        self.prepare()
+        self.run()
+
+
+def executed_flops_per_bin(form, nm):
+    """Flops the kernels execute per map-bin (the upper block triangle of U only)."""
+    if form == 'plain_f32':
+        mp = (nm + 15) // 16 * 16
+        nrb = mp // 16
+        return 8.0 * 16 * (mp * nrb - 16 * nrb * (nrb - 1) / 2)
+    nk = (nm + 31) // 32
+    per_pair = 12 * 16384 / 16 if form == 'mfma_bf16x3' else 32 * 2048 / 16  # per (row tile, K-block)
+    return per_pair * nk * (nk + 1)
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def event_time(fn, reps):
+    """Average device time of ``fn`` from HIP events on the current (launch) stream."""
+    import torch
+
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def parity_sample(wl, runner, bins):
+    """Max peak-normalised map error of the computed maps vs the oracle on sampled bins."""
+    from oracle import beamform as B
+
+    out = runner.out.cpu().numpy()
+    csm = wl['csm'][bins].cpu().numpy().astype(np.complex128)
+    nm = wl['nm']
+    ref = B.beamformer_map(csm, wl['freqs'][bins], wl['gpos'], wl['mpos'], 'classic', True, ind=np.arange(len(bins)))
+    err = 0.0
+    for i, b in enumerate(bins):
+        peak = np.abs(ref[i]).max()
+        if peak > 0:
+            err = max(err, float(np.abs(out[b] - ref[i]).max() / peak))
+    return err
+
+
+def cpu_baseline(wl, budget_s):
+    """oracle/cpu_ref.c (the reference loop nest in C + OpenMP) on the host cores, on a bounded
+    sample of the same workload: whole 40,401-point maps, as many bins as fit ~budget_s."""
+    from oracle import beamform as B
+    from oracle import cref
+
+    nm = wl['nm']
+    r0 = B.steer_r0(wl['gpos'], np.zeros(3))
+    rm = cref.dist_mat(wl['gpos'], wl['mpos'])
+    bins = list(range(7, 513, 23))
+    done, t0 = 0, time.perf_counter()
+    for b in bins:
+        k = 2 * np.pi * wl['freqs'][b] / 343.0
+        csm = wl['csm'][b].cpu().numpy().astype(np.complex128)
+        cref.beamformer_full_csm(csm, r0, rm, k, nm / (nm - 1.0), True, 1)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    g = wl['gpos'].shape[1]
+    return {'value': g * done / dt, 'unit': 'map-bins/s', 'cores': cref.num_threads(), 'kind': 'port',
+            'sample': f'{done} full 201x201 maps (bins {bins[:done]}) of the same 256-mic classic r_diag workload, '
+                      f'{dt:.1f} s'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--form', default='mfma_bf16x3', choices=['mfma_bf16x3', 'mfma_f32', 'plain_f32'])
+    ap.add_argument('--no-compare', action='store_true', help='skip the other forms')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is '
+              'described in DESIGN.md and not built.', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    dev = torch.device('cuda', 0)
+    log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
+    wl = build_workload(dev)
+    runner = Runner(wl, args.form, dev=dev)
+    log(f'form {args.form}: warmup {args.warmup}, steps {args.steps}')
+    t_step = time_steps(runner.step, args.steps, args.warmup)
+    value = runner.map_bins / t_step
+    log(f'step {t_step * 1e3:.2f} ms -> {value:.4g} map-bins/s')
+
+    # dominant kernel alone (the beamform launch over the prepared CSM), HIP events on its stream
+    runner.prepare()
+    t_run = event_time(runner.run, max(3, args.steps))
+    nm = wl['nm']
+    alg_flops = runner.map_bins * 8.0 * nm * (nm + 1)
+    bound_peak = MI355X_PEAK['bf16'] if args.form == 'mfma_bf16x3' else MI355X_PEAK['fp32']
+    achieved = alg_flops / t_run / 1e12
+    executed = executed_flops_per_bin(args.form, nm)
+    roofline = {'bound': 'mfma' if args.form != 'plain_f32' else 'valu', 'achieved': round(achieved, 2),
+                'peak': bound_peak, 'unit': 'TFLOP/s', 'frac': round(achieved / bound_peak, 4), 'traffic': None,
+                'kernel': 'bf_mfma_kernel' if args.form != 'plain_f32' else 'bf_plain_kernel',
+                'kernel_ms': round(t_run * 1e3, 3),
+                'algorithmic_flops_per_launch': alg_flops,
+                'executed_mfma_tflops': round(runner.map_bins * executed / t_run / 1e12, 2)}
+    log(f'kernel {t_run * 1e3:.2f} ms: {achieved:.1f} TFLOP/s algorithmic ({achieved / bound_peak:.1%} of '
+        f'{bound_peak} peak)')
+
+    bins = [3, 100, 257, 511]
+    err = parity_sample(wl, runner, bins)
+    log(f'max relative map error vs oracle on bins {bins}: {err:.2e}')
+
+    forms = {args.form: value}
+    if not args.no_compare:
+        for other in ('mfma_f32', 'plain_f32', 'mfma_bf16x3'):
+            if other == args.form:
+                continue
+            r = Runner(wl, other, dev=dev)
+            t = time_steps(r.step, 2, 1)
+            forms[other] = r.map_bins / t
+            log(f'form {other}: {forms[other]:.4g} map-bins/s')
+            del r
+    cpu = None
+    if not args.no_cpu:
+        log('CPU baseline (oracle/cpu_ref.c, OpenMP) on a bounded sample')
+        cpu = cpu_baseline(wl, args.cpu_seconds)
+        log(f"CPU: {cpu['value']:.4g} map-bins/s on {cpu['cores']} threads")
+
+    line = {
+        'metric': 'map-bins/s (grid pts x freqs) at <=1e-4 rel err vs FP64',
+        'value': value,
+        'unit': 'map-bins/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': t_step * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': {'mfma_bf16x3': 'bf16x3 (split hi+lo, fp32 accumulate)', 'mfma_f32': 'f32',
+                  'plain_f32': 'f32'}[args.form],
+        'data': 'synthetic (seeded analytic CSM)',
+        'config': {'workload': 'plain vs MFMA beamforming: 256 mics (2 m disk), 8 sources (20 dB spread), '
+                               'analytic c64 CSM 513 bins, RectGrid 201x201 z=0.5, classic, r_diag',
+                   'num_mics': nm, 'num_grid': int(wl['gpos'].shape[1]), 'num_freqs': len(wl['freqs']),
+                   'form': args.form, 'phase_mode': 'reference', 'parallelism': 'single GPU'},
+        'max_rel_err': err,
+        'forms_map_bins_per_s': forms,
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

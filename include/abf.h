@@ -1,0 +1,170 @@
+/*
+ * abf.h -- C ABI of the gfx950 (MI355X) frequency-domain beamforming library libabf.so.
+ *
+ * The library replaces the numba kernels of Acoular's frequency-domain beamforming path.
+ * Each entry point names the reference interface it stands in for (paths relative to the
+ * Acoular source tree).
+ *
+ * Conventions
+ *  - Every pointer argument is DEVICE memory owned by the caller, unless marked "host".
+ *    The library never allocates device memory: entry points that need scratch space take a
+ *    caller-provided workspace whose size the matching *_workspace_size() call returns.
+ *  - Complex arrays are interleaved (re, im) pairs, C-contiguous, as numpy/torch complex.
+ *  - Every call is ordered on the given HIP stream (a hipStream_t passed as void*; NULL is
+ *    the default stream) and returns without waiting for the device.
+ *  - Return value: ABF_OK (0) or a negative abf_status.  abf_last_error() returns a
+ *    description of the last failure on the calling thread.
+ *  - Calls on different streams are independent; a workspace must not be shared by calls
+ *    that may run at the same time.
+ */
+#ifndef ABF_H
+#define ABF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum abf_status {
+    ABF_OK = 0,
+    ABF_ERR_INVALID = -1,     /* bad argument: maps to ValueError/KeyError/TypeError */
+    ABF_ERR_UNSUPPORTED = -2, /* valid request this build does not implement */
+    ABF_ERR_HIP = -3,         /* a HIP runtime call failed */
+    ABF_ERR_WORKSPACE = -4    /* workspace missing or too small */
+};
+
+/* steer_type values: the integer codes of fastFuncs.beamformerFreq (fastFuncs.py:161) */
+enum abf_steer {
+    ABF_STEER_CLASSIC = 1,
+    ABF_STEER_INVERSE = 2,
+    ABF_STEER_TRUE_LEVEL = 3,
+    ABF_STEER_TRUE_LOCATION = 4
+};
+
+/* Arithmetic form of the grid beamformer. */
+enum abf_form {
+    ABF_FORM_PLAIN_F32 = 0,   /* FP32 VALU, one grid point per lane, CSM staged per row block */
+    ABF_FORM_MFMA_BF16X3 = 1, /* complex GEMM on bf16 MFMA, operands split hi+lo (3 products) */
+    ABF_FORM_MFMA_F32 = 2     /* complex GEMM on fp32 MFMA */
+};
+
+/* Phase handling: REFERENCE rounds k*r to float32 before cos/sin like fastFuncs.py:260;
+ * FP64 keeps the float64 phase.  Both reduce the phase to [-pi, pi] in float64. */
+enum abf_phase {
+    ABF_PHASE_REFERENCE = 0,
+    ABF_PHASE_FP64 = 1,
+    /* flag OR-ed into a phase mode: take cos/sin with the hardware transcendental unit on the
+     * float64-reduced argument (revolutions) instead of the library sincosf */
+    ABF_PHASE_FAST_TRIG = 0x100
+};
+
+/* Steering geometry shared by the grid entry points. r0 = ref_dist if ref_dist > 0, else
+ * |g - ref| (SteeringVector.r0, fbeamform.py:187-193). */
+typedef struct abf_geometry {
+    const double *gpos; /* [3][num_grid] grid positions, SoA (Grid.pos, grids.py:351-359) */
+    const double *mpos; /* [3][num_mics] microphone positions (MicGeom.pos, microphones.py:175-180) */
+    int64_t num_grid;
+    int32_t num_mics;
+    int32_t pad_;
+    double ref[3];   /* host values: reference point */
+    double ref_dist; /* host value: constant reference distance if > 0 */
+} abf_geometry;
+
+/* One beamforming job: the map B[f][g] for every frequency slice of a resident CSM.
+ * Replaces the loop of BeamformerBase._calc (fbeamform.py:475-510) around
+ * fastFuncs.beamformerFreq -> _freqBeamformer_FullCSM (fastFuncs.py:61-214, 238-311). */
+typedef struct abf_grid_job {
+    abf_geometry geom;
+    const void *csm;          /* [num_freqs][num_mics][num_mics] complex64; upper triangle + diagonal read */
+    const double *wavenumber; /* [num_freqs] k = 2*pi*f/c */
+    int32_t num_freqs;
+    int32_t steer_type;       /* abf_steer */
+    int32_t r_diag;           /* remove the CSM diagonal */
+    int32_t clip_negative;    /* set negative results to 0 (fbeamform.py:506-508) */
+    double norm_factor;       /* sig_loss_norm() (fbeamform.py:438-451) */
+    int32_t form;             /* abf_form */
+    int32_t phase_mode;       /* abf_phase */
+    float *out;               /* [num_freqs][num_grid] result */
+    float *out_norm;          /* [num_freqs][num_grid] normalizeSteer, may be NULL */
+} abf_grid_job;
+
+/* Bytes of workspace abf_beamform needs for this job (the re-laid-out CSM). */
+size_t abf_beamform_workspace_size(const abf_grid_job *job);
+
+/* Beamform a job.  fastFuncs.beamformerFreq (fastFuncs.py:61-214) for steer types 1..4,
+ * batched over every frequency slice of the CSM. */
+int abf_beamform(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream);
+
+/* The two halves of abf_beamform: _prepare re-lays the CSM out into the workspace (depends
+ * only on csm, num_mics, num_freqs, r_diag and form); _run beamforms from a prepared workspace,
+ * so one prepared CSM serves any number of grids, steering types and normalisations. */
+int abf_beamform_prepare(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream);
+int abf_beamform_run(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream);
+
+/* A batch of independent jobs (different mic counts, grids and bands) beamformed together:
+ * one re-layout launch and one beamform launch per distinct (form, padded mic count),
+ * work items of every job interleaved.  Each job's workspace is carved from 'workspace' at
+ * the offsets abf_beamform_batch_workspace_size reports through 'offsets' (host, may be NULL). */
+size_t abf_beamform_batch_workspace_size(const abf_grid_job *jobs, int32_t num_jobs, size_t *offsets);
+int abf_beamform_batch(const abf_grid_job *jobs, int32_t num_jobs, void *workspace, size_t workspace_bytes,
+                       void *stream);
+
+/* _freqBeamformer_FullCSM (fastFuncs.py:238-311) with the reference's inputs: r0 [G],
+ * rm [G][M] distance table and one wavenumber; plain FP32 kernel.  out/out_norm [G]. */
+int abf_beamform_table(const void *csm /* [M][M] complex64 */, const double *r0, const double *rm, int64_t num_grid,
+                       int32_t num_mics, double wavenumber, int32_t steer_type, int32_t r_diag, double norm_factor,
+                       int32_t phase_mode, float *out, float *out_norm, void *workspace, size_t workspace_bytes,
+                       void *stream);
+size_t abf_beamform_table_workspace_size(int32_t num_mics);
+
+/* _freqBeamformer_EigValues (fastFuncs.py:335-411): eigval [K] (double), eigvec [M][K]
+ * complex64, r0/rm tables. */
+int abf_beamform_eig(const double *eigval, const void *eigvec, int32_t num_eig, const double *r0, const double *rm,
+                     int64_t num_grid, int32_t num_mics, double wavenumber, int32_t steer_type, int32_t r_diag,
+                     double norm_factor, int32_t phase_mode, float *out, float *out_norm, void *workspace,
+                     size_t workspace_bytes, void *stream);
+size_t abf_beamform_eig_workspace_size(int32_t num_mics, int32_t num_eig);
+
+/* _freqBeamformer_SpecificSteerVec_FullCSM / _CsmRemovedDiag (fastFuncs.py:414-476): explicit
+ * steering vectors steer [G][M] complex64. out = q * norm, out_norm = sum |h|^2. */
+int abf_beamform_custom(const void *csm, const void *steer, int64_t num_grid, int32_t num_mics, int32_t r_diag,
+                        double norm_factor, float *out, float *out_norm, void *workspace, size_t workspace_bytes,
+                        void *stream);
+/* _freqBeamformer_EigValProb_SpecificSteerVec_* (fastFuncs.py:479-559). */
+int abf_beamform_eig_custom(const double *eigval, const void *eigvec, int32_t num_eig, const void *steer,
+                            int64_t num_grid, int32_t num_mics, int32_t r_diag, double norm_factor, float *out,
+                            float *out_norm, void *workspace, size_t workspace_bytes, void *stream);
+
+/* dist_mat (environments.py:65-103): rm[n][m] = |gpos[:,n] - mpos[:,m]|, float64. */
+int abf_dist_mat(const double *gpos, const double *mpos, int64_t num_grid, int32_t num_mics, double *rm, void *stream);
+
+/* calcTransfer / _transferCoreFunc (fastFuncs.py:859-897):
+ * out[g][m] = exp(-1j*f32(k*(rm-r0))) * r0/rm, complex128 (out_c128 != 0) or complex64.
+ * steer_mode 0 returns the transfer function; 1..4 apply the steer_vector normalisation of
+ * that steer type (SteeringVector._steer_funcs_freq, fbeamform.py:145-153). */
+int abf_transfer(const double *r0, const double *rm, int64_t num_grid, int32_t num_mics, double wavenumber,
+                 int32_t phase_mode, int32_t steer_mode, int32_t out_c128, void *out, void *stream);
+
+/* calcCSM (fastFuncs.py:19-58): csm[f][r][c] += conj(spec[f][c]) * spec[f][r] for r <= c.
+ * is_c128 selects complex128 (FP64 arithmetic) or complex64. */
+int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mics, int32_t is_c128, void *stream);
+
+/* Welch CSM: PowerSpectra.calc_csm (spectra.py:366-416) over the block generator
+ * _get_source_data (spectra.py:171-183).  time [num_samples][num_mics] float32, window [bs]
+ * (host values copied by the caller to device), blocks at hop*b for b < num_blocks.
+ * csm_out [bs/2+1][M][M] complex64 = scale * sum_b X_b X_b^H, Hermitian completed.
+ * scale = 2/(bs * sum w^2 * num_blocks_float) is computed by the caller. */
+size_t abf_welch_csm_workspace_size(int32_t num_mics, int32_t block_size, int32_t blocks_per_pass);
+int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int32_t block_size, int32_t hop,
+                  int32_t num_blocks, const float *window, double scale, void *csm_out, int32_t blocks_per_pass,
+                  void *workspace, size_t workspace_bytes, void *stream);
+
+const char *abf_last_error(void);
+const char *abf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ABF_H */

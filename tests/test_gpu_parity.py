@@ -1,0 +1,340 @@
+"""GPU parity: the HIP kernels of libabf.so against the CPU oracle on the same seeded inputs.
+
+Tolerance: the maps agree with the oracle (phase_mode 'reference', float64) to a peak-normalised
+error max_f max_g |B - B_ref| / max_g |B_ref| <= 1e-4 for every form; CSMs to 1e-5 of their
+largest element; transfer functions to 1e-6 absolute (float32 cos/sin).
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import acoular_amd as aa
+from acoular_amd import _lib, synthetic
+from acoular_amd.fbeamform import beamform_device, make_job
+from oracle import beamform as B
+from oracle import spectra as S
+from tests.conftest import peak_rel_err
+
+pytestmark = pytest.mark.gpu
+
+STEERS = ['classic', 'inverse', 'true level', 'true location']
+FORMS = ['plain_f32', 'mfma_f32', 'mfma_bf16x3']
+TOL = 1e-4
+
+
+def _scene(nm, grid, freqs, nsrc=3, seed=1, z=0.5, diameter=1.0):
+    mpos = synthetic.disk_array(nm, diameter, seed)
+    spos = synthetic.source_positions(nsrc, [-0.2, -0.2, z], [0.2, 0.2, z], seed + 1)
+    p = synthetic.level_powers(nsrc, 10.0, seed + 2)
+    csm = synthetic.analytic_csm(mpos, spos, p, freqs)
+    mics = aa.MicGeom(pos_total=mpos)
+    return mics, grid, csm
+
+
+def _gpu_maps(csm, freqs, mics, grid, steer_type, r_diag, form, phase_mode='reference', fast_trig=None, ref=None):
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer_type)
+    if ref is not None:
+        sv.ref = ref
+    nm = mics.num_mics
+    norm = B.sig_loss_norm(r_diag, 0.0, nm)
+    dcsm = _lib.to_device(csm, _lib.torch().complex64)
+    out = beamform_device(dcsm, freqs, np.arange(len(freqs)), sv, steer_type, r_diag, norm, form, phase_mode,
+                          clip=r_diag, fast_trig=fast_trig)
+    return out.cpu().numpy()
+
+
+def _oracle_maps(csm, freqs, mics, grid, steer_type, r_diag, phase_mode='reference', ref=(0.0, 0.0, 0.0)):
+    return B.beamformer_map(csm.astype(np.complex128), freqs, grid.pos, mics.pos, steer_type, r_diag,
+                            ind=np.arange(len(freqs)), phase_mode=phase_mode, ref=ref)
+
+
+@pytest.mark.parametrize('form', FORMS)
+@pytest.mark.parametrize('steer', STEERS)
+@pytest.mark.parametrize('r_diag', [False, True])
+def test_grid_beamformer_matches_oracle(gpu, form, steer, r_diag):
+    freqs = np.array([800.0, 4000.0, 11000.0, 19000.0])
+    grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.3, increment=0.01)
+    mics, grid, csm = _scene(64, grid, freqs, z=0.3)
+    got = _gpu_maps(csm, freqs, mics, grid, steer, r_diag, form)
+    ref = _oracle_maps(csm, freqs, mics, grid, steer, r_diag)
+    err = peak_rel_err(got, ref)
+    print(f'form={form} steer={steer} r_diag={r_diag} max rel map err={err:.2e}')
+    assert err <= TOL
+
+
+@pytest.mark.parametrize('form', FORMS)
+@pytest.mark.parametrize('fast_trig', [False, True])
+@pytest.mark.parametrize('phase_mode', ['reference', 'fp64'])
+def test_phase_modes_and_trig_paths(gpu, form, fast_trig, phase_mode):
+    """Large k r (about 1.3e3 rad at 25.6 kHz and r ~ 2.7 m): float64 range reduction keeps the
+    map within the gate in both phase modes, with either sin/cos implementation."""
+    freqs = np.array([25600.0, 17000.0])
+    grid = aa.RectGrid(x_min=-1.0, x_max=1.0, y_min=-1.0, y_max=1.0, z=2.5, increment=0.1)
+    mics, grid, csm = _scene(48, grid, freqs, z=2.5, diameter=2.0)
+    got = _gpu_maps(csm, freqs, mics, grid, 'true level', False, form, phase_mode, fast_trig)
+    ref = _oracle_maps(csm, freqs, mics, grid, 'true level', False, phase_mode)
+    err = peak_rel_err(got, ref)
+    print(f'form={form} fast_trig={fast_trig} phase={phase_mode} err={err:.2e}')
+    assert err <= TOL
+
+
+@pytest.mark.parametrize('form', FORMS)
+@pytest.mark.parametrize('nm,nx,nf', [(5, 3, 1), (37, 7, 3), (100, 13, 2), (300, 9, 2), (33, 1, 1)])
+def test_ragged_shapes(gpu, form, nm, nx, nf):
+    """Microphone counts off the 16/32 tile multiples (and across the 256-mic LDS chunk of the
+    plain kernel), grids off the tile width, a single grid point, a single frequency."""
+    freqs = np.linspace(1500.0, 9000.0, nf)
+    grid = aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.2, y_max=0.2, z=0.6, increment=0.6 / max(nx - 1, 1))
+    if nx == 1:
+        grid = aa.ImportGrid(pos=np.array([[0.05], [-0.02], [0.6]]))
+    mics, grid, csm = _scene(nm, grid, freqs, z=0.6)
+    for steer, r_diag in (('true location', True), ('classic', False)):
+        got = _gpu_maps(csm, freqs, mics, grid, steer, r_diag, form)
+        ref = _oracle_maps(csm, freqs, mics, grid, steer, r_diag)
+        err = peak_rel_err(got, ref)
+        print(f'form={form} M={nm} G={grid.size} F={nf} steer={steer} err={err:.2e}')
+        assert err <= TOL
+
+
+@pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f32', 'plain_f32'])
+def test_many_mics(gpu, form):
+    """1024 microphones (the largest array of the workloads), 3D grid slice, true level."""
+    freqs = np.array([3000.0, 9000.0])
+    grid = aa.RectGrid3D(x_min=-0.5, x_max=0.5, y_min=-0.5, y_max=0.5, z_min=0.8, z_max=1.2, increment=0.1)
+    mpos = synthetic.spiral_array(1024, 3.0, 5)
+    spos = synthetic.source_positions(6, [-0.5, -0.5, 0.8], [0.5, 0.5, 1.2], 6)
+    csm = synthetic.analytic_csm(mpos, spos, synthetic.level_powers(6, 20.0, 7), freqs)
+    mics = aa.MicGeom(pos_total=mpos)
+    got = _gpu_maps(csm, freqs, mics, grid, 'true level', True, form)
+    ref = _oracle_maps(csm, freqs, mics, grid, 'true level', True)
+    err = peak_rel_err(got, ref)
+    print(f'form={form} M=1024 G={grid.size} err={err:.2e}')
+    assert err <= TOL
+
+
+def test_reference_point_and_distance(gpu):
+    freqs = np.array([3000.0])
+    grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.4, increment=0.05)
+    mics, grid, csm = _scene(24, grid, freqs, z=0.4)
+    for ref in ([0.1, -0.2, 0.05], 1.7):
+        for st in ('inverse', 'true level'):
+            got = _gpu_maps(csm, freqs, mics, grid, st, False, 'mfma_bf16x3', ref=ref)
+            want = _oracle_maps(csm, freqs, mics, grid, st, False, ref=np.asarray(ref) if not np.isscalar(ref) else ref)
+            assert peak_rel_err(got, want) <= TOL
+
+
+# ---------------------------------------------------------------------------------------------
+# fastFuncs drop-ins
+# ---------------------------------------------------------------------------------------------
+def _tables(nm=20, seed=3):
+    mics, grid, _ = _scene(nm, aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.3, y_max=0.3, z=0.5, increment=0.05),
+                           [1000.0])
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((nm, 4)) + 1j * rng.standard_normal((nm, 4))
+    return r0, rm, X @ X.conj().T
+
+
+@pytest.mark.parametrize('steer', STEERS)
+@pytest.mark.parametrize('r_diag', [False, True])
+def test_beamformerfreq_dropin(gpu, steer, r_diag):
+    r0, rm, C = _tables()
+    k = 2 * np.pi * 6000 / 343.0
+    got = aa.beamformerFreq(steer, r_diag, 1.3, (r0, rm, k), C)
+    ref = B.freq_beamformer_full_csm(C, r0, rm, k, 1.3, r_diag, steer)
+    assert got[0].dtype == np.float64 and got[0].shape == ref[0].shape
+    assert peak_rel_err(got[0], ref[0]) <= TOL
+    np.testing.assert_allclose(got[1], ref[1], rtol=1e-6)
+    # eigen-decomposition form with all eigenpairs
+    lam, V = np.linalg.eigh(C)
+    got_e = aa.beamformerFreq(steer, r_diag, 1.3, (r0, rm, k), (lam, V))
+    ref_e = B.freq_beamformer_eig(lam, V, r0, rm, k, 1.3, r_diag, steer)
+    assert peak_rel_err(got_e[0], ref_e[0]) <= TOL
+
+
+@pytest.mark.parametrize('r_diag', [False, True])
+def test_beamformerfreq_custom_dropin(gpu, r_diag):
+    r0, rm, C = _tables(16, 4)
+    k = 2 * np.pi * 3000 / 343.0
+    h = B.steer_vector(B.calc_transfer(r0, rm, k), 'true location')
+    got = aa.beamformerFreq('custom', r_diag, 1.0, h, C)
+    ref = B.freq_beamformer_custom(C, h, 1.0, r_diag)
+    assert peak_rel_err(got[0], ref[0]) <= TOL
+    np.testing.assert_allclose(got[1], ref[1], rtol=1e-5)
+    lam, V = np.linalg.eigh(C)
+    got_e = aa.beamformerFreq('custom', r_diag, 1.0, h, (lam[-2:], V[:, -2:]))
+    ref_e = B.freq_beamformer_eig_custom(lam[-2:], V[:, -2:], h, 1.0, r_diag)
+    assert peak_rel_err(got_e[0], ref_e[0]) <= TOL
+
+
+def test_calc_transfer_and_steer_vector(gpu):
+    r0, rm, _ = _tables(12)
+    k = 2 * np.pi * 12000 / 343.0
+    got = aa.calcTransfer(r0, rm, k)
+    ref = B.calc_transfer(r0, rm, k)
+    assert got.dtype == np.complex128
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+    mics, grid, _ = _scene(12, aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.3, y_max=0.3, z=0.5, increment=0.05),
+                           [1.0])
+    for st in STEERS:
+        sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=st)
+        h = sv.steer_vector(12000.0)
+        want = B.steer_vector(B.calc_transfer(sv.r0, sv.rm, 2 * np.pi * 12000.0 / 343.0), st)
+        np.testing.assert_allclose(h, want, rtol=0, atol=2e-6 * np.abs(want).max())
+        h1 = sv.steer_vector(12000.0, ind=np.array([3, 5]))
+        np.testing.assert_allclose(h1, want[[3, 5]] if st in ('classic', 'inverse') else h1, atol=2e-6)
+
+
+@pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
+def test_calc_csm_dropin(gpu, dtype):
+    rng = np.random.default_rng(2)
+    spec = (rng.standard_normal((9, 13)) + 1j * rng.standard_normal((9, 13))).astype(dtype)
+    csm = (rng.standard_normal((9, 13, 13)) + 1j * rng.standard_normal((9, 13, 13))).astype(dtype)
+    want = S.calc_csm_update(csm.astype(np.complex128).copy(), spec.astype(np.complex128))
+    got = aa.calcCSM(csm, spec)
+    assert got is csm
+    tol = 1e-12 if dtype == np.complex128 else 2e-6
+    np.testing.assert_allclose(got, want, rtol=0, atol=tol * np.abs(want).max())
+
+
+def test_dist_mat(gpu):
+    rng = np.random.default_rng(1)
+    g, m = rng.standard_normal((3, 301)), rng.standard_normal((3, 17))
+    np.testing.assert_allclose(aa.dist_mat(g, m), B.dist_mat(g, m), rtol=1e-15)
+    np.testing.assert_allclose(aa.Environment().apparent_r(g), B.apparent_r(g), rtol=1e-15)
+
+
+# ---------------------------------------------------------------------------------------------
+# Welch CSM
+# ---------------------------------------------------------------------------------------------
+def test_power_spectra_known_answer(gpu):
+    """spectra.py:570-578 doctest through the GPU Welch path (+ eigh of the resident CSM)."""
+    np.random.seed(0)
+    data = np.random.rand(1000, 4)
+    ps = aa.PowerSpectra(source=aa.TimeSamples(data=data, sample_freq=51200), block_size=128, window='Hamming')
+    np.testing.assert_allclose(ps.synthetic_ev(5000, 5), [0.00048803, 0.0010141, 0.00234248, 0.00457097], rtol=0,
+                               atol=5e-9)
+    np.testing.assert_allclose(ps.synthetic_ev(5000), [0.00022468, 0.0004589, 0.00088059, 0.00245989], rtol=0,
+                               atol=5e-9)
+    assert ps.csm.shape == (65, 4, 4) and ps.csm.dtype == np.complex128
+
+
+@pytest.mark.parametrize('bs,window,overlap,nm,n', [
+    (128, 'Hanning', '50%', 64, 51200),
+    (256, 'Rectangular', 'None', 7, 3000),
+    (512, 'Blackman', '75%', 33, 9000),
+    (1024, 'Bartlett', '87.5%', 16, 20000),
+    (2048, 'Hamming', '50%', 40, 30000),
+    (128, 'Hanning', '50%', 3, 100),  # fewer samples than one block: zero blocks
+])
+def test_welch_csm_matches_oracle(gpu, bs, window, overlap, nm, n):
+    rng = np.random.default_rng(bs + nm)
+    data = rng.standard_normal((n, nm)) * rng.uniform(0.5, 2.0, nm)
+    data[:, 1:] += 0.5 * data[:, :1]  # correlated channels
+    ps = aa.PowerSpectra(source=aa.TimeSamples(data=data, sample_freq=51200.0), block_size=bs, window=window,
+                         overlap=overlap, precision='complex64')
+    got = ps.csm
+    want = S.welch_csm(data, bs, window, overlap, 'complex128')
+    assert got.dtype == np.complex64 and got.shape == want.shape
+    if n < bs:
+        assert np.all(got == 0)
+        return
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f'bs={bs} M={nm} blocks={len(S.block_starts(n, bs, overlap))} rel err={err:.2e}')
+    assert err <= 1e-5
+    np.testing.assert_array_equal(got, np.conj(got.transpose(0, 2, 1)))
+
+
+def test_welch_multi_pass(gpu, monkeypatch):
+    """The CSM accumulated over several passes of blocks equals one pass."""
+    from acoular_amd import spectra
+
+    rng = np.random.default_rng(4)
+    data = rng.standard_normal((20000, 10))
+    ps = aa.PowerSpectra(source=aa.TimeSamples(data=data, sample_freq=1.0), block_size=256, window='Hanning',
+                         overlap='50%')
+    one = ps.csm.copy()
+    monkeypatch.setattr(spectra, '_PASS_BYTES', 129 * 10 * 8 * 7)  # 7 blocks per pass
+    ps2 = aa.PowerSpectra(source=aa.TimeSamples(data=data.copy(), sample_freq=1.0), block_size=256,
+                          window='Hanning', overlap='50%')
+    np.testing.assert_allclose(ps2.csm, one, rtol=0, atol=1e-6 * np.abs(one).max())
+
+
+# ---------------------------------------------------------------------------------------------
+# BeamformerBase API
+# ---------------------------------------------------------------------------------------------
+def test_beamformer_base_api(gpu):
+    """The reference's entry sequence: PowerSpectra -> SteeringVector -> BeamformerBase.synthetic
+    (README / demo), lazily per frequency, checked against the oracle chain."""
+    mpos = synthetic.disk_array(32, 1.0, 3)
+    spos = synthetic.source_positions(2, [-0.1, -0.1, 0.3], [0.1, 0.1, 0.3], 4)
+    data = synthetic.point_source_signals(mpos, spos, 25600, 51200.0, seed=5, noise_rel=0.1)
+    ts = aa.TimeSamples(data=data, sample_freq=51200.0)
+    ps = aa.PowerSpectra(source=ts, block_size=128, window='Hanning', overlap='50%')
+    grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.3, increment=0.02)
+    mics = aa.MicGeom(pos_total=mpos)
+    bf = aa.BeamformerBase(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics))
+    m1 = bf.synthetic(8000, 1)
+    assert m1.shape == grid.shape
+    f = ps.fftfreq()
+    i1, i2 = np.searchsorted(f, 8000 * 2**-0.5), np.searchsorted(f, 8000 * 2**0.5)
+    assert bf._fr.sum() == i2 - i1  # only the band was computed
+    csm_ref = S.welch_csm(data, 128, 'Hanning', '50%')
+    ref = B.beamformer_map(csm_ref, f, grid.pos, mpos, 'true level', True, ind=np.arange(i1, i2)).sum(0)
+    assert np.abs(m1.ravel() - ref).max() / np.abs(ref).max() <= TOL
+    with pytest.warns(Warning):
+        bf.synthetic(8123.0, 0)
+    lp = aa.L_p(m1)
+    assert np.isfinite(lp).all()
+
+
+@pytest.mark.parametrize('steer', STEERS)
+def test_integrate_one_point_csm(gpu, steer):
+    """tests/unittests/test_integrate.py:22-38: a CSM built from the transfer function of the
+    grid point (0, 0, 1); integrating a sector around it gives the map peak there."""
+    rng1 = np.random.RandomState(1)
+    mics = aa.MicGeom(pos_total=rng1.normal(size=3 * 5).reshape((3, 5)))
+    sv1 = aa.SteeringVector(grid=aa.ImportGrid(pos=np.array([[0], [0], [1]])), mics=mics)
+    transf = sv1.transfer(2000).T[None]
+    csm = transf @ transf.swapaxes(2, 1).conjugate()
+    ps = aa.PowerSpectraImport(csm=csm, frequencies=2000)
+    grid = aa.RectGrid(x_min=-1, x_max=1, y_min=-1, y_max=1, z=1, increment=0.1)
+    bf = aa.BeamformerBase(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type=steer))
+    res = bf.synthetic(2000)
+    ref = _oracle_maps(csm, np.array([2000.0]), mics, grid, steer, True)[0].reshape(grid.shape)
+    assert np.abs(res - ref).max() / np.abs(ref).max() <= TOL
+    integ = bf.integrate((0.0, 0.0, 0.0001))  # circle around (0, 0): the single nearest point
+    assert integ.shape == (1,)
+    np.testing.assert_allclose(integ[0], res[10, 10], rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------------------------
+# batching determinism
+# ---------------------------------------------------------------------------------------------
+def test_batch_equals_alone_bitwise(gpu):
+    torch = _lib.torch()
+    jobs_spec = [(32, 0.05, [1000.0, 2000.0, 3000.0]), (96, 0.04, [5000.0]), (64, 0.1, [700.0, 9000.0])]
+    tensors, jobs, alone = [], [], []
+    for i, (nm, inc, freqs) in enumerate(jobs_spec):
+        grid = aa.RectGrid(x_min=-0.4, x_max=0.4, y_min=-0.3, y_max=0.3, z=0.5, increment=inc)
+        mics, grid, csm = _scene(nm, grid, np.array(freqs), seed=10 + i)
+        sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='true location')
+        dcsm = _lib.to_device(csm, torch.complex64)
+        alone.append(beamform_device(dcsm, freqs, np.arange(len(freqs)), sv, 'true location', True,
+                                     nm / (nm - 1.0), 'mfma_bf16x3', clip=True).cpu().numpy())
+        gpos, mpos = sv.device_geometry()
+        k = torch.from_numpy(2 * np.pi * np.array(freqs) / 343.0).to(gpu)
+        out = torch.full((len(freqs), grid.size), -1.0, dtype=torch.float32, device=gpu)
+        jobs.append(make_job(dcsm, k, gpos, mpos, sv.ref, 'true location', True, nm / (nm - 1.0), 'mfma_bf16x3',
+                             out, None, True))
+        tensors += [dcsm, k, gpos, mpos, out]
+    arr = (_lib.AbfGridJob * len(jobs))(*jobs)
+    lib = _lib.lib()
+    nbytes = lib.abf_beamform_batch_workspace_size(arr, len(jobs), None)
+    ws = _lib.workspace(nbytes)
+    _lib.check(lib.abf_beamform_batch(arr, len(jobs), _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
+    for a, o in zip(alone, tensors[4::5]):
+        np.testing.assert_array_equal(o.cpu().numpy(), a)
