@@ -55,6 +55,8 @@ class AbfGridJob(ctypes.Structure):
         ('phase_mode', ctypes.c_int32),
         ('out', _c_dp),
         ('out_norm', _c_dp),
+        ('tile_cols', ctypes.c_int32),
+        ('tile_freqs', ctypes.c_int32),
     ]
 
 

@@ -85,7 +85,7 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     int ct_cap = 16;
     const int64_t tiles16 = cdiv(j->geom.num_grid, 16);
     while (ct_cap > 1 && ct_cap / 2 >= tiles16) ct_cap /= 2;
-    const int ct_env = env_int("ABF_MFMA_CT", 0);
+    const int ct_env = j->tile_cols > 0 ? std::max(1, j->tile_cols / 16) : env_int("ABF_MFMA_CT", 0);
     p.CT = 0;
     for (int ct = 16; ct >= 1; ct /= 2) {
         if (ct > ct_cap) continue;
@@ -100,7 +100,7 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     if (p.CT == 0)
         return fail(ABF_ERR_UNSUPPORTED, "MFMA form supports at most ~1150 microphones; use the plain form");
     p.CW = std::min(p.CT, 4);
-    const int fb = env_int("ABF_MFMA_FB", 8);
+    const int fb = j->tile_freqs > 0 ? j->tile_freqs : env_int("ABF_MFMA_FB", 8);
     p.FB = std::max(1, std::min(fb, j->num_freqs));
     p.ntiles = cdiv(j->geom.num_grid, 16 * p.CT);
     p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);

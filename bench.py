@@ -155,7 +155,7 @@ def cpu_baseline(wl, budget_s):
     nm = wl['nm']
     r0 = B.steer_r0(wl['gpos'], np.zeros(3))
     rm = cref.dist_mat(wl['gpos'], wl['mpos'])
-    bins = list(range(7, 513, 23))
+    bins = [7 + (23 * i) % 506 for i in range(512)]
     done, t0 = 0, time.perf_counter()
     for b in bins:
         k = 2 * np.pi * wl['freqs'][b] / 343.0

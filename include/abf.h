@@ -88,6 +88,8 @@ typedef struct abf_grid_job {
     int32_t phase_mode;       /* abf_phase */
     float *out;               /* [num_freqs][num_grid] result */
     float *out_norm;          /* [num_freqs][num_grid] normalizeSteer, may be NULL */
+    int32_t tile_cols;        /* MFMA forms: grid points per workgroup (16..256, power of two); 0 = auto */
+    int32_t tile_freqs;       /* MFMA forms: frequencies per workgroup; 0 = auto */
 } abf_grid_job;
 
 /* Bytes of workspace abf_beamform needs for this job (the re-laid-out CSM). */

@@ -329,7 +329,7 @@ def test_batch_equals_alone_bitwise(gpu):
         k = torch.from_numpy(2 * np.pi * np.array(freqs) / 343.0).to(gpu)
         out = torch.full((len(freqs), grid.size), -1.0, dtype=torch.float32, device=gpu)
         jobs.append(make_job(dcsm, k, gpos, mpos, sv.ref, 'true location', True, nm / (nm - 1.0), 'mfma_bf16x3',
-                             out, None, True))
+                             'reference', out, None, True))
         tensors += [dcsm, k, gpos, mpos, out]
     arr = (_lib.AbfGridJob * len(jobs))(*jobs)
     lib = _lib.lib()
