@@ -64,6 +64,7 @@ struct MfPlan {
 
 size_t mfma_lds(bool bf16, int BN, int Mp)
 {
+    // X planes of the tile, r0 + normalisation sums, per-wave partial results
     const size_t KS = bf16 ? Mp + 8 : Mp + 4;
     const size_t planes = bf16 ? 4 * (size_t)BN * KS * 2 : 2 * (size_t)BN * KS * 4;
     return planes + 16 * (size_t)BN + 4 * 8 * (size_t)BN;
@@ -81,15 +82,17 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     p.Mp = round_up(j->geom.num_mics, 32);
     p.nK = p.Mp / 32;
     p.nUp = p.nK * (p.nK + 1) / 2;
-    // columns needed by this grid, as a power of two tile count
+    const int fb = j->tile_freqs > 0 ? j->tile_freqs : env_int("ABF_MFMA_FB", 8);
+    p.FB = std::max(1, std::min(fb, j->num_freqs));
+    p.prep_bytes = (size_t)j->num_freqs * p.nUp * 8192;
+    // grid points needed by this job, as a power-of-two count of 16-point tiles
     int ct_cap = 16;
     const int64_t tiles16 = cdiv(j->geom.num_grid, 16);
     while (ct_cap > 1 && ct_cap / 2 >= tiles16) ct_cap /= 2;
-    const int ct_env = j->tile_cols > 0 ? std::max(1, j->tile_cols / 16) : env_int("ABF_MFMA_CT", 0);
+    const int ct_req = j->tile_cols > 0 ? std::max(1, j->tile_cols / 16) : env_int("ABF_MFMA_CT", 0);
     p.CT = 0;
     for (int ct = 16; ct >= 1; ct /= 2) {
-        if (ct > ct_cap) continue;
-        if (ct_env > 0 && ct > ct_env) continue;
+        if (ct > ct_cap || (ct_req > 0 && ct > ct_req)) continue;
         const int BN = 16 * ct;
         const int TPC = 512 / BN;
         if (p.Mp / 8 > TPC * 5) continue; // steering prologue covers at most 5 groups of 8 per thread
@@ -100,11 +103,8 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     if (p.CT == 0)
         return fail(ABF_ERR_UNSUPPORTED, "MFMA form supports at most ~1150 microphones; use the plain form");
     p.CW = std::min(p.CT, 4);
-    const int fb = j->tile_freqs > 0 ? j->tile_freqs : env_int("ABF_MFMA_FB", 8);
-    p.FB = std::max(1, std::min(fb, j->num_freqs));
     p.ntiles = cdiv(j->geom.num_grid, 16 * p.CT);
     p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
-    p.prep_bytes = (size_t)j->num_freqs * p.nUp * 8192;
     p.lds = mfma_lds(p.bf16, 16 * p.CT, p.Mp);
     return ABF_OK;
 }

@@ -16,14 +16,9 @@
 //    split hi+lo (3 products per real product) or on 16x16x4 fp32 MFMA; X (the steering
 //    matrix of a grid tile) lives in LDS, U is pre-laid-out in MFMA fragment order by
 //    prep_mfma_kernel and streamed from L2; q = Re sum_i conj(x_i) Y_i in the epilogue.
-#include "abf_common.h"
-#include "abf_kernels.h"
+#include "abf_mfma.h"
 
 namespace abf {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 // -------------------------------------------------------------------------------------------
 // steering element: v = w * (cos phi, -sin phi), phi = k r
@@ -199,7 +194,6 @@ __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
 //   f32:    parts {Ur, Ui}, each [64 lanes][8 f32]
 // Lane l of a tile holds row (l & 15) and columns 8*(l >> 4) + 0..7 of the block, the same
 // k-set the B operand (X) uses, so each MFMA pairs matching k.
-__device__ __forceinline__ int upper_index(int I, int J, int nK) { return I * nK - (I * (I - 1)) / 2 + (J - I); }
 
 __global__ __launch_bounds__(64) void prep_mfma_kernel(const float2 *__restrict__ csm, int M, int Mp, int r_diag,
                                                        int bf16x3, char *__restrict__ prep)
@@ -210,11 +204,10 @@ __global__ __launch_bounds__(64) void prep_mfma_kernel(const float2 *__restrict_
     const int t = blockIdx.y, f = blockIdx.z;
     const int l = threadIdx.x;
     const int i = I * 32 + t * 16 + (l & 15);
-    const int jb = J * 32 + (l >> 4) * 8;
     float ur[8], ui[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const int j = jb + e;
+        const int j = J * 32 + kslot_mic(l >> 4, e);
         float x = 0.f, y = 0.f;
         if (i < M && j < M) {
             const float2 c = csm[((int64_t)f * M + i) * M + j];
@@ -231,24 +224,14 @@ __global__ __launch_bounds__(64) void prep_mfma_kernel(const float2 *__restrict_
     const int nUp = nK * (nK + 1) / 2;
     char *dst = prep + (((int64_t)f * nUp + upper_index(I, J, nK)) * 2 + t) * 4096;
     if (bf16x3) {
-        uint32_t rh[4], rl[4], ih[4], il[4];
-#pragma unroll
-        for (int e = 0; e < 8; e += 2) {
-            uint32_t h0, l0, h1, l1;
-            split_bf16(ur[e], h0, l0);
-            split_bf16(ur[e + 1], h1, l1);
-            rh[e / 2] = h0 | (h1 << 16);
-            rl[e / 2] = l0 | (l1 << 16);
-            split_bf16(ui[e], h0, l0);
-            split_bf16(ui[e + 1], h1, l1);
-            ih[e / 2] = h0 | (h1 << 16);
-            il[e / 2] = l0 | (l1 << 16);
-        }
+        u4 rh, rl, ih, il;
+        split8(ur, rh, rl);
+        split8(ui, ih, il);
         u4 *d = reinterpret_cast<u4 *>(dst);
-        d[0 * 64 + l] = u4{rh[0], rh[1], rh[2], rh[3]};
-        d[1 * 64 + l] = u4{rl[0], rl[1], rl[2], rl[3]};
-        d[2 * 64 + l] = u4{ih[0], ih[1], ih[2], ih[3]};
-        d[3 * 64 + l] = u4{il[0], il[1], il[2], il[3]};
+        d[0 * 64 + l] = rh;
+        d[1 * 64 + l] = rl;
+        d[2 * 64 + l] = ih;
+        d[3 * 64 + l] = il;
     } else {
         f4 *d = reinterpret_cast<f4 *>(dst);
         d[(0 * 64 + l) * 2 + 0] = f4{ur[0], ur[1], ur[2], ur[3]};
@@ -260,21 +243,17 @@ __global__ __launch_bounds__(64) void prep_mfma_kernel(const float2 *__restrict_
 
 constexpr int kMfWaves = 8;
 constexpr int kMfThreads = kMfWaves * 64;
-constexpr int kMfNS = 5; // max 8-mic groups per thread in the steering prologue
+constexpr int kMfNS = 5; // max 8-slot groups per thread in the steering generation
 
-__device__ __forceinline__ int find_job(const MfJob *__restrict__ jobs, int njobs, int64_t wg)
-{
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].wg_begin <= wg)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
+// One workgroup = 8 waves computes a tile of BN = 16 CT grid points against all Mp microphones
+// for FB frequencies.  Prologue: float64 distances of the workgroup's (point, microphone) pairs
+// into registers, steering-normalisation sums and r0.  Per frequency: the steering matrix X of
+// the tile (BN x Mp, split hi+lo bf16 or fp32, K-slot order of kslot_mic) is generated into LDS;
+// then each wave takes (row tile, column group) units in a boustrophedon order that balances
+// the block triangle, streams its U fragments from L2 (prefetched one block ahead, each reused
+// across CW column tiles) and accumulates Yr, Yi in MFMA registers; q = Re sum conj(x) Y is
+// reduced per grid point in a fixed order (bit-identical results alone and batched), then
+// normalised per steer type.
 template <bool BF16X3, int CW>
 __global__ __launch_bounds__(kMfThreads, 1) void bf_mfma_kernel(const MfJob *__restrict__ jobs, int njobs)
 {
@@ -283,32 +262,30 @@ __global__ __launch_bounds__(kMfThreads, 1) void bf_mfma_kernel(const MfJob *__r
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t wg = blockIdx.x;
-    const int jidx = find_job(jobs, njobs, wg);
-    const MfJob &jb = jobs[jidx];
+    const MfJob &jb = jobs[find_job(jobs, njobs, wg)];
 
     const int M = jb.M, Mp = jb.Mp, CT = jb.CT, F = jb.F, FB = jb.FB;
     const int64_t G = jb.G;
     const int BN = 16 * CT;
     const int64_t local = wg - jb.wg_begin;
     const int tile = (int)(local % jb.ntiles);
-    const int fgrp = (int)(local / jb.ntiles);
-    const int f0 = fgrp * FB;
+    const int f0 = (int)(local / jb.ntiles) * FB;
     const int f1 = min(f0 + FB, F);
     const int KS = BF16X3 ? Mp + 8 : Mp + 4; // LDS row stride in elements (16-byte skew)
     const int nK = Mp / 32;
     const int nUp = nK * (nK + 1) / 2;
 
-    // LDS carve-up
-    uint16_t *xh16 = reinterpret_cast<uint16_t *>(smem);     // bf16x3: 4 planes [BN][KS]
-    float *x32 = reinterpret_cast<float *>(smem);            // f32: 2 planes [BN][KS]
-    const int64_t plane_bytes = BF16X3 ? (int64_t)BN * KS * 2 : (int64_t)BN * KS * 4;
-    const int nplanes = BF16X3 ? 4 : 2;
-    double *s_r0 = reinterpret_cast<double *>(smem + plane_bytes * nplanes);
+    // LDS: X planes [BN][KS] (bf16x3: re hi, re lo, im hi, im lo; fp32: re, im), then r0, help, q
+    uint16_t *xh16 = reinterpret_cast<uint16_t *>(smem);
+    float *x32 = reinterpret_cast<float *>(smem);
+    const int64_t pe = (int64_t)BN * KS; // plane stride in elements
+    const int64_t plane_bytes = BF16X3 ? pe * 2 : pe * 4;
+    double *s_r0 = reinterpret_cast<double *>(smem + plane_bytes * (BF16X3 ? 4 : 2));
     double *s_help = s_r0 + BN;
     float *s_q = reinterpret_cast<float *>(s_help + BN); // [kMfWaves][BN]
 
-    // ---- frequency-independent prologue: distances kept in registers ----
-    const int TPC = kMfThreads / BN; // threads per grid column (power of two)
+    // ---- prologue: distances of this thread's (point, 8-slot groups) into registers ----
+    const int TPC = kMfThreads / BN; // threads per grid point (power of two)
     const int col = tid / TPC, part = tid % TPC;
     const int64_t g = (int64_t)tile * BN + col;
     const bool valid = g < G;
@@ -318,22 +295,21 @@ __global__ __launch_bounds__(kMfThreads, 1) void bf_mfma_kernel(const MfJob *__r
         gy = jb.gpos[G + g];
         gz = jb.gpos[2 * G + g];
     }
-    const int ngroups = Mp / 8;
+    const int ngroups = Mp / 8; // group gi = 8 K-slots: block gi / 4, lane group gi % 4
     double rr[kMfNS][8];
+    uint32_t live = 0; // bit s*8+e: slot holds a real microphone of a real grid point
     double help = 0.0;
 #pragma unroll
     for (int s = 0; s < kMfNS; ++s) {
         const int gi = part + TPC * s;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int m = gi * 8 + e;
+            const int m = (gi >> 2) * 32 + kslot_mic(gi & 3, e);
             double r = 1.0;
             if (gi < ngroups && valid && m < M) {
                 r = dist3(gx, gy, gz, jb.mpos[m], jb.mpos[M + m], jb.mpos[2 * M + m]);
-                float w;
-                double h;
-                steer_weight(jb.steer_type, r, w, h);
-                help += h;
+                help += help_term(jb.steer_type, r);
+                live |= 1u << (s * 8 + e);
             }
             rr[s][e] = r;
         }
@@ -347,15 +323,19 @@ __global__ __launch_bounds__(kMfThreads, 1) void bf_mfma_kernel(const MfJob *__r
     }
     for (int idx = tid; idx < kMfWaves * BN; idx += kMfThreads) s_q[idx] = 0.f;
 
+    const int wk = weight_kind(jb.steer_type);
+    const bool refph = jb.phase_mode == ABF_PHASE_REFERENCE;
+    const bool fast = jb.fast_trig != 0;
     const int RT = Mp / 16;
     const int NCG = CT / CW;
     const int nunits = RT * NCG;
     const int rounds = (nunits + kMfWaves - 1) / kMfWaves;
     const char *__restrict__ prep_base = reinterpret_cast<const char *>(jb.prep);
+    const u4 sg = u4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
 
     for (int f = f0; f < f1; ++f) {
         const double k = jb.wavenumber[f];
-        // ---- steering matrix X of this frequency into LDS ----
+        // ---- steering matrix X of this frequency into LDS (wave-uniform selects only) ----
 #pragma unroll
         for (int s = 0; s < kMfNS; ++s) {
             const int gi = part + TPC * s;
@@ -363,37 +343,37 @@ __global__ __launch_bounds__(kMfThreads, 1) void bf_mfma_kernel(const MfJob *__r
                 float xr[8], xi[8];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const int m = gi * 8 + e;
-                    float c, sn, w;
-                    double h;
-                    phasor_rev(k * rr[s][e], jb.phase_mode, jb.fast_trig, c, sn);
-                    steer_weight(jb.steer_type, rr[s][e], w, h);
-                    const bool live = valid && m < M;
-                    xr[e] = live ? w * c : 0.f;
-                    xi[e] = live ? -w * sn : 0.f;
+                    const double r = rr[s][e];
+                    double phi = k * r;
+                    phi = refph ? (double)(float)phi : phi;
+                    float sn, c;
+                    if (fast) {
+                        const double u = phi * kInvTwoPi;
+                        const float t = (float)(u - rint(u));
+                        sn = __builtin_amdgcn_sinf(t);
+                        c = __builtin_amdgcn_cosf(t);
+                    } else {
+                        const double n = rint(phi * kInvTwoPi);
+                        double t = fma(-n, kTwoPiHi, phi);
+                        t = fma(-n, kTwoPiLo, t);
+                        sincosf((float)t, &sn, &c);
+                    }
+                    const float rf = (float)r;
+                    float w = wk == 0 ? 1.f : (wk == 1 ? rf : __builtin_amdgcn_rcpf(rf));
+                    w = (live >> (s * 8 + e)) & 1u ? w : 0.f;
+                    xr[e] = w * c;
+                    xi[e] = -w * sn;
                 }
                 const int64_t off = (int64_t)col * KS + gi * 8;
                 if constexpr (BF16X3) {
-                    uint32_t rh[4], rl[4], ih[4], il[4];
-#pragma unroll
-                    for (int e = 0; e < 8; e += 2) {
-                        uint32_t h0, l0, h1, l1;
-                        split_bf16(xr[e], h0, l0);
-                        split_bf16(xr[e + 1], h1, l1);
-                        rh[e / 2] = h0 | (h1 << 16);
-                        rl[e / 2] = l0 | (l1 << 16);
-                        split_bf16(xi[e], h0, l0);
-                        split_bf16(xi[e + 1], h1, l1);
-                        ih[e / 2] = h0 | (h1 << 16);
-                        il[e / 2] = l0 | (l1 << 16);
-                    }
-                    const int64_t pe = (int64_t)BN * KS; // plane stride in elements
-                    *reinterpret_cast<u4 *>(xh16 + off) = u4{rh[0], rh[1], rh[2], rh[3]};
-                    *reinterpret_cast<u4 *>(xh16 + pe + off) = u4{rl[0], rl[1], rl[2], rl[3]};
-                    *reinterpret_cast<u4 *>(xh16 + 2 * pe + off) = u4{ih[0], ih[1], ih[2], ih[3]};
-                    *reinterpret_cast<u4 *>(xh16 + 3 * pe + off) = u4{il[0], il[1], il[2], il[3]};
+                    u4 rh, rl, ih, il;
+                    split8(xr, rh, rl);
+                    split8(xi, ih, il);
+                    *reinterpret_cast<u4 *>(xh16 + off) = rh;
+                    *reinterpret_cast<u4 *>(xh16 + pe + off) = rl;
+                    *reinterpret_cast<u4 *>(xh16 + 2 * pe + off) = ih;
+                    *reinterpret_cast<u4 *>(xh16 + 3 * pe + off) = il;
                 } else {
-                    const int64_t pe = (int64_t)BN * KS;
                     *reinterpret_cast<f4 *>(x32 + off) = f4{xr[0], xr[1], xr[2], xr[3]};
                     *reinterpret_cast<f4 *>(x32 + off + 4) = f4{xr[4], xr[5], xr[6], xr[7]};
                     *reinterpret_cast<f4 *>(x32 + pe + off) = f4{xi[0], xi[1], xi[2], xi[3]};
@@ -419,45 +399,64 @@ __global__ __launch_bounds__(kMfThreads, 1) void bf_mfma_kernel(const MfJob *__r
             const char *ablk = prep_f + ((int64_t)upper_index(I, I, nK) * 2 + t) * 4096;
             const int colbase = cg * CW * 16 + (lane & 15);
             const int kq = (lane >> 4) * 8;
-            for (int J = I; J < nK; ++J, ablk += 8192) {
-                const int koff = J * 32 + kq;
-                if constexpr (BF16X3) {
-                    const u4 *ap = reinterpret_cast<const u4 *>(ablk);
-                    const u4 a_rh = ap[0 * 64 + lane], a_rl = ap[1 * 64 + lane];
-                    const u4 a_ih = ap[2 * 64 + lane], a_il = ap[3 * 64 + lane];
-                    const u4 sg = u4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-                    const bf16x8 Arh = __builtin_bit_cast(bf16x8, a_rh), Arl = __builtin_bit_cast(bf16x8, a_rl);
-                    const bf16x8 Aih = __builtin_bit_cast(bf16x8, a_ih), Ail = __builtin_bit_cast(bf16x8, a_il);
-                    const bf16x8 Nih = __builtin_bit_cast(bf16x8, a_ih ^ sg), Nil = __builtin_bit_cast(bf16x8, a_il ^ sg);
-                    const int64_t pe = (int64_t)BN * KS;
+            if constexpr (BF16X3) {
+                const u4 *ap = reinterpret_cast<const u4 *>(ablk);
+                u4 a_rh = ap[lane], a_rl = ap[64 + lane], a_ih = ap[128 + lane], a_il = ap[192 + lane];
+                for (int J = I; J < nK; ++J) {
+                    // prefetch the next block's fragments (blocks of a row are 8 KiB apart)
+                    u4 n_rh = a_rh, n_rl = a_rl, n_ih = a_ih, n_il = a_il;
+                    if (J + 1 < nK) {
+                        const u4 *np = reinterpret_cast<const u4 *>(ablk + (int64_t)(J + 1 - I) * 8192);
+                        n_rh = np[lane];
+                        n_rl = np[64 + lane];
+                        n_ih = np[128 + lane];
+                        n_il = np[192 + lane];
+                    }
+                    const u4 a_nh = a_ih ^ sg, a_nl = a_il ^ sg;
+                    const int koff = J * 32 + kq;
+#define MF(A, B, C) C = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, A), B, C, 0, 0, 0)
+                    bf16x8 Xrh[CW], Xrl[CW], Xih[CW], Xil[CW];
 #pragma unroll
                     for (int c = 0; c < CW; ++c) {
                         const int64_t xo = (int64_t)(colbase + c * 16) * KS + koff;
-                        const bf16x8 Xrh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u4 *>(xh16 + xo));
-                        const bf16x8 Xrl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u4 *>(xh16 + pe + xo));
-                        const bf16x8 Xih = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u4 *>(xh16 + 2 * pe + xo));
-                        const bf16x8 Xil = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u4 *>(xh16 + 3 * pe + xo));
-                        // Yr = Ur Xr - Ui Xi ; Yi = Ui Xr + Ur Xi  (each real product as hi*hi + hi*lo + lo*hi)
-                        accr[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Arh, Xrh, accr[c], 0, 0, 0);
-                        acci[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Aih, Xrh, acci[c], 0, 0, 0);
-                        accr[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Arh, Xrl, accr[c], 0, 0, 0);
-                        acci[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Aih, Xrl, acci[c], 0, 0, 0);
-                        accr[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Arl, Xrh, accr[c], 0, 0, 0);
-                        acci[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ail, Xrh, acci[c], 0, 0, 0);
-                        accr[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Nih, Xih, accr[c], 0, 0, 0);
-                        acci[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Arh, Xih, acci[c], 0, 0, 0);
-                        accr[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Nih, Xil, accr[c], 0, 0, 0);
-                        acci[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Arh, Xil, acci[c], 0, 0, 0);
-                        accr[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Nil, Xih, accr[c], 0, 0, 0);
-                        acci[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Arl, Xih, acci[c], 0, 0, 0);
+                        Xrh[c] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u4 *>(xh16 + xo));
+                        Xrl[c] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u4 *>(xh16 + pe + xo));
+                        Xih[c] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u4 *>(xh16 + 2 * pe + xo));
+                        Xil[c] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u4 *>(xh16 + 3 * pe + xo));
                     }
-                } else {
-                    const f4 *ap = reinterpret_cast<const f4 *>(ablk);
-                    const f4 ur0 = ap[(0 * 64 + lane) * 2], ur1 = ap[(0 * 64 + lane) * 2 + 1];
-                    const f4 ui0 = ap[(1 * 64 + lane) * 2], ui1 = ap[(1 * 64 + lane) * 2 + 1];
-                    const float ur[8] = {ur0[0], ur0[1], ur0[2], ur0[3], ur1[0], ur1[1], ur1[2], ur1[3]};
-                    const float ui[8] = {ui0[0], ui0[1], ui0[2], ui0[3], ui1[0], ui1[1], ui1[2], ui1[3]};
-                    const int64_t pe = (int64_t)BN * KS;
+                    // Yr = Ur Xr - Ui Xi ; Yi = Ui Xr + Ur Xi, each product hi*hi + hi*lo + lo*hi;
+                    // 2 CW independent accumulators between dependent MFMAs
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MF(a_rh, Xrh[c], accr[c]); MF(a_ih, Xrh[c], acci[c]); }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MF(a_rh, Xrl[c], accr[c]); MF(a_ih, Xrl[c], acci[c]); }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MF(a_rl, Xrh[c], accr[c]); MF(a_il, Xrh[c], acci[c]); }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MF(a_nh, Xih[c], accr[c]); MF(a_rh, Xih[c], acci[c]); }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MF(a_nh, Xil[c], accr[c]); MF(a_rh, Xil[c], acci[c]); }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MF(a_nl, Xih[c], accr[c]); MF(a_rl, Xih[c], acci[c]); }
+#undef MF
+                    a_rh = n_rh;
+                    a_rl = n_rl;
+                    a_ih = n_ih;
+                    a_il = n_il;
+                }
+            } else {
+                const f4 *ap = reinterpret_cast<const f4 *>(ablk);
+                f4 u_a = ap[2 * lane], u_b = ap[2 * lane + 1], v_a = ap[128 + 2 * lane], v_b = ap[128 + 2 * lane + 1];
+                for (int J = I; J < nK; ++J) {
+                    f4 nu_a = u_a, nu_b = u_b, nv_a = v_a, nv_b = v_b;
+                    if (J + 1 < nK) {
+                        const f4 *np = reinterpret_cast<const f4 *>(ablk + (int64_t)(J + 1 - I) * 8192);
+                        nu_a = np[2 * lane];
+                        nu_b = np[2 * lane + 1];
+                        nv_a = np[128 + 2 * lane];
+                        nv_b = np[128 + 2 * lane + 1];
+                    }
+                    const int koff = J * 32 + kq;
 #pragma unroll
                     for (int c = 0; c < CW; ++c) {
                         const int64_t xo = (int64_t)(colbase + c * 16) * KS + koff;
@@ -465,34 +464,36 @@ __global__ __launch_bounds__(kMfThreads, 1) void bf_mfma_kernel(const MfJob *__r
                         const f4 xr1 = *reinterpret_cast<const f4 *>(x32 + xo + 4);
                         const f4 xi0 = *reinterpret_cast<const f4 *>(x32 + pe + xo);
                         const f4 xi1 = *reinterpret_cast<const f4 *>(x32 + pe + xo + 4);
-                        const float xr[8] = {xr0[0], xr0[1], xr0[2], xr0[3], xr1[0], xr1[1], xr1[2], xr1[3]};
-                        const float xi[8] = {xi0[0], xi0[1], xi0[2], xi0[3], xi1[0], xi1[1], xi1[2], xi1[3]};
 #pragma unroll
                         for (int e = 0; e < 8; ++e) {
-                            accr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[e], xr[e], accr[c], 0, 0, 0);
-                            acci[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[e], xr[e], acci[c], 0, 0, 0);
-                            accr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[e], xi[e], accr[c], 0, 0, 0);
-                            acci[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[e], xi[e], acci[c], 0, 0, 0);
+                            const float ur = e < 4 ? u_a[e & 3] : u_b[e & 3], ui = e < 4 ? v_a[e & 3] : v_b[e & 3];
+                            const float xr = e < 4 ? xr0[e & 3] : xr1[e & 3], xi = e < 4 ? xi0[e & 3] : xi1[e & 3];
+                            accr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur, xr, accr[c], 0, 0, 0);
+                            acci[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui, xr, acci[c], 0, 0, 0);
+                            accr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui, xi, accr[c], 0, 0, 0);
+                            acci[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur, xi, acci[c], 0, 0, 0);
                         }
                     }
+                    u_a = nu_a;
+                    u_b = nu_b;
+                    v_a = nv_a;
+                    v_b = nv_b;
                 }
             }
-            // epilogue: lane holds rows rt*16 + 4*(lane>>4) + 0..3 of column colbase + c*16
-            const int row0 = rt * 16 + (lane >> 4) * 4;
+            // epilogue: result rows 4 g + i of row tile t sit at K-slot 8 g + 4 t + i of block I
+            const int slot0 = I * 32 + (lane >> 4) * 8 + t * 4;
 #pragma unroll
             for (int c = 0; c < CW; ++c) {
                 const int colc = colbase + c * 16;
-                const int64_t xo = (int64_t)colc * KS + row0;
+                const int64_t xo = (int64_t)colc * KS + slot0;
                 float sacc = 0.f;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     float xr, xi;
                     if constexpr (BF16X3) {
-                        const int64_t pe = (int64_t)BN * KS;
                         xr = bf16_to_float(xh16[xo + e]) + bf16_to_float(xh16[pe + xo + e]);
                         xi = bf16_to_float(xh16[2 * pe + xo + e]) + bf16_to_float(xh16[3 * pe + xo + e]);
                     } else {
-                        const int64_t pe = (int64_t)BN * KS;
                         xr = x32[xo + e];
                         xi = x32[pe + xo + e];
                     }

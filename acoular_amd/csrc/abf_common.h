@@ -97,4 +97,16 @@ __device__ __forceinline__ void normalise(int steer_type, double q, double r0, d
     }
 }
 
+// K-slot order inside a 32-microphone block of the MFMA forms: slot (g, e) -- lane group g =
+// lane >> 4, element e of the lane's 8-element fragment -- holds microphone
+// 4 g + (e & 3) + 16 (e >> 2).  With it, the B fragment of lane (column, g) holds exactly the
+// microphones 16 t + 4 g + i (t = e >> 2, i = e & 3) that the 16x16 MFMA result rows 4 g + i of
+// row tile t of the same block hold, so the epilogue q = Re sum conj(x) Y needs no lane exchange.
+__device__ __forceinline__ int kslot_mic(int g, int e) { return 4 * g + (e & 3) + 16 * (e >> 2); }
+
+__device__ __forceinline__ int weight_kind(int steer_type)
+{
+    return steer_type == ABF_STEER_CLASSIC ? 0 : (steer_type == ABF_STEER_INVERSE ? 1 : 2);
+}
+
 } // namespace abf

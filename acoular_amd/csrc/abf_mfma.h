@@ -1,0 +1,48 @@
+// Helpers of the MFMA beamformer kernels (abf_bf.hip).
+#pragma once
+#include "abf_common.h"
+#include "abf_kernels.h"
+
+namespace abf {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int upper_index(int I, int J, int nK) { return I * nK - (I * (I - 1)) / 2 + (J - I); }
+
+__device__ __forceinline__ int find_job(const MfJob *__restrict__ jobs, int njobs, int64_t wg)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].wg_begin <= wg)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// x = hi + lo with hi, lo bf16 (round to nearest even on the hardware converter), 8 values
+__device__ __forceinline__ void split8(const float *x, u4 &hi, u4 &lo)
+{
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const bf16x2 h = {(__bf16)x[2 * d], (__bf16)x[2 * d + 1]};
+        const bf16x2 l = {(__bf16)(x[2 * d] - (float)h[0]), (__bf16)(x[2 * d + 1] - (float)h[1])};
+        hi[d] = __builtin_bit_cast(uint32_t, h);
+        lo[d] = __builtin_bit_cast(uint32_t, l);
+    }
+}
+
+// frequency-independent steering normalisation sum of one microphone (fastFuncs.py:262-273)
+__device__ __forceinline__ double help_term(int steer_type, double r)
+{
+    if (steer_type == ABF_STEER_INVERSE) return r * r;
+    if (steer_type == ABF_STEER_TRUE_LEVEL || steer_type == ABF_STEER_TRUE_LOCATION) return 1.0 / (r * r);
+    return 0.0;
+}
+
+} // namespace abf

@@ -68,13 +68,16 @@ def analytic_csm(mpos, spos, powers, freqs, c: float = 343.0, dtype=np.complex64
     tdt = {np.complex64: torch.complex64, np.complex128: torch.complex128}[np.dtype(dtype).type]
     rsm_t = torch.from_numpy(rsm).to(device)
     rs0_t = torch.from_numpy(rs0).to(device)
-    sp = torch.from_numpy(np.sqrt(p)).to(device)
+    amp = (rs0_t[:, None] / rsm_t) * torch.from_numpy(np.sqrt(p)).to(device)[:, None]  # [S, M]
+    dr = rsm_t - rs0_t[:, None]
     out = torch.empty((len(freqs), mpos.shape[1], mpos.shape[1]), dtype=tdt, device=device)
-    for i, f in enumerate(freqs):
-        k = 2 * np.pi * f / c
-        ph = -k * (rsm_t - rs0_t[:, None])
-        a = torch.polar((rs0_t[:, None] / rsm_t) * sp[:, None], ph)  # [S, M] complex128
-        out[i] = (a.transpose(0, 1) @ a.conj()).to(tdt)
+    kk = torch.from_numpy(2 * np.pi * freqs / c).to(device)
+    chunk = 64
+    for i0 in range(0, len(freqs), chunk):
+        k = kk[i0 : i0 + chunk]
+        a = torch.polar(amp[None].expand(len(k), -1, -1), -k[:, None, None] * dr[None])  # [Fc, S, M]
+        at = a.transpose(1, 2).contiguous()
+        out[i0 : i0 + chunk] = torch.bmm(at, a.conj().contiguous()).to(tdt)
     return out
 
 

@@ -180,6 +180,8 @@ def main():
     ap.add_argument('--no-compare', action='store_true', help='skip the other forms')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--mode', default='headline', choices=['headline', 'copy'],
+                    help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is '
@@ -189,6 +191,15 @@ def main():
     import torch
 
     dev = torch.device('cuda', 0)
+    if args.mode == 'copy':
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'mode': 'copy', 'bytes_read_per_dispatch': src.numel() * 4,
+                          'bytes_written_per_dispatch': dst.numel() * 4, 'dispatches': 3}))
+        return
     log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
     wl = build_workload(dev)
     runner = Runner(wl, args.form, dev=dev)
