@@ -57,6 +57,8 @@ class AbfGridJob(ctypes.Structure):
         ('out_norm', _c_dp),
         ('tile_cols', ctypes.c_int32),
         ('tile_freqs', ctypes.c_int32),
+        ('tile_kernel', ctypes.c_int32),
+        ('pad1_', ctypes.c_int32),
     ]
 
 

@@ -57,6 +57,7 @@ int check_phase(int pm)
 // ---------------------------------------------------------------------------------------------
 struct MfPlan {
     int Mp, CT, CW, FB, nK, nUp;
+    bool ws; // warp-specialised kernel (double-buffered steering matrix)
     int64_t ntiles, wgs;
     size_t prep_bytes, lds;
     bool bf16;
@@ -68,6 +69,14 @@ size_t mfma_lds(bool bf16, int BN, int Mp)
     const size_t KS = bf16 ? Mp + 8 : Mp + 4;
     const size_t planes = bf16 ? 4 * (size_t)BN * KS * 2 : 2 * (size_t)BN * KS * 4;
     return planes + 16 * (size_t)BN + 4 * 8 * (size_t)BN;
+}
+
+size_t mfma_ws_lds(bool bf16, int BN, int Mp)
+{
+    // two X buffers (8 bytes per element in both forms), q partials [2][8][BN], r0, steering-sum
+    // partials [256 / BN][BN]
+    const size_t KS = bf16 ? Mp + 8 : Mp + 4;
+    return 16 * (size_t)BN * KS + 2 * 8 * 4 * (size_t)BN + 8 * (size_t)BN + 8 * 256;
 }
 
 int env_int(const char *name, int dflt)
@@ -90,22 +99,43 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     const int64_t tiles16 = cdiv(j->geom.num_grid, 16);
     while (ct_cap > 1 && ct_cap / 2 >= tiles16) ct_cap /= 2;
     const int ct_req = j->tile_cols > 0 ? std::max(1, j->tile_cols / 16) : env_int("ABF_MFMA_CT", 0);
+    // The kernel that alternates steering generation and MFMA work over one LDS buffer by
+    // default (the faster one where both apply, measured on the 256-mic workload); the
+    // warp-specialised kernel on request (tile_kernel, or ABF_MFMA_KERNEL=2) when its two
+    // steering buffers fit LDS with 32 or 64 grid points per tile.
+    const int kreq = j->tile_kernel > 0 ? j->tile_kernel : env_int("ABF_MFMA_KERNEL", ABF_MFMA_KERNEL_AUTO);
     p.CT = 0;
-    for (int ct = 16; ct >= 1; ct /= 2) {
-        if (ct > ct_cap || (ct_req > 0 && ct > ct_req)) continue;
-        const int BN = 16 * ct;
-        const int TPC = 512 / BN;
-        if (p.Mp / 8 > TPC * 5) continue; // steering prologue covers at most 5 groups of 8 per thread
-        if (mfma_lds(p.bf16, BN, p.Mp) > kLdsMax) continue;
-        p.CT = ct;
-        break;
+    p.ws = false;
+    if (kreq == ABF_MFMA_KERNEL_SPECIALISED) {
+        for (int ct = 4; ct >= 2; ct /= 2) { // tiles of 2 column groups of 32 grid points
+            if (ct > ct_cap || (ct_req > 0 && ct > ct_req)) continue;
+            const int BN = 16 * ct;
+            if (p.Mp / 8 > (256 / BN) * 5) continue; // 4 steering waves, at most 5 groups of 8 per thread
+            if (mfma_ws_lds(p.bf16, BN, p.Mp) > kLdsMax) continue;
+            // the per-wave step table of one frequency lives in the 64 lanes of a register
+            if (cdiv(2 * p.nK * (ct / 2), 4) * p.nK > 64) continue;
+            p.CT = ct;
+            p.ws = true;
+            break;
+        }
+    }
+    if (p.CT == 0) {
+        for (int ct = 16; ct >= 1; ct /= 2) {
+            if (ct > ct_cap || (ct_req > 0 && ct > ct_req)) continue;
+            const int BN = 16 * ct;
+            const int TPC = 512 / BN;
+            if (p.Mp / 8 > TPC * 5) continue; // steering prologue covers at most 5 groups of 8 per thread
+            if (mfma_lds(p.bf16, BN, p.Mp) > kLdsMax) continue;
+            p.CT = ct;
+            break;
+        }
     }
     if (p.CT == 0)
         return fail(ABF_ERR_UNSUPPORTED, "MFMA form supports at most ~1150 microphones; use the plain form");
-    p.CW = std::min(p.CT, 4);
+    p.CW = p.ws ? 2 : std::min(p.CT, 4);
     p.ntiles = cdiv(j->geom.num_grid, 16 * p.CT);
     p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
-    p.lds = mfma_lds(p.bf16, 16 * p.CT, p.Mp);
+    p.lds = p.ws ? mfma_ws_lds(p.bf16, 16 * p.CT, p.Mp) : mfma_lds(p.bf16, 16 * p.CT, p.Mp);
     return ABF_OK;
 }
 
@@ -120,6 +150,9 @@ int validate_job(const abf_grid_job *j)
     if (int r = check_steer(j->steer_type)) return r;
     if (int r = check_phase(j->phase_mode)) return r;
     if (j->form < ABF_FORM_PLAIN_F32 || j->form > ABF_FORM_MFMA_F32) return fail(ABF_ERR_INVALID, "unknown form");
+    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_SPECIALISED)
+        return fail(ABF_ERR_INVALID, "unknown MFMA kernel variant");
+    if (j->tile_cols < 0 || j->tile_freqs < 0) return fail(ABF_ERR_INVALID, "negative tile size");
     return ABF_OK;
 }
 
@@ -184,9 +217,23 @@ template <bool B, int CW> void launch_mfma(const MfJob *dev, int n, int64_t wgs,
     hipLaunchKernelGGL((bf_mfma_kernel<B, CW>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
 }
 
-void launch_mfma_group(bool bf16, int CW, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+template <bool B> void launch_mfma_ws(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
-    if (bf16) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)bf_mfma_ws_kernel<B>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLdsMax);
+        attr = true;
+    }
+    hipLaunchKernelGGL((bf_mfma_ws_kernel<B>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
+}
+
+void launch_mfma_group(bool bf16, int CW, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+{
+    if (ws) {
+        if (bf16) launch_mfma_ws<true>(dev, n, wgs, lds, s);
+        else launch_mfma_ws<false>(dev, n, wgs, lds, s);
+    } else if (bf16) {
         if (CW == 1) launch_mfma<true, 1>(dev, n, wgs, lds, s);
         else if (CW == 2) launch_mfma<true, 2>(dev, n, wgs, lds, s);
         else launch_mfma<true, 4>(dev, n, wgs, lds, s);
@@ -318,10 +365,11 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
     char *ws = reinterpret_cast<char *>(workspace);
     MfJob *dev_jobs = reinterpret_cast<MfJob *>(ws);
 
-    // plain jobs run one after the other; MFMA jobs are grouped per (form, CW) into one launch
+    // plain jobs run one after the other; MFMA jobs are grouped per (form, kernel, CW) into one launch
     struct Group {
         bool bf16;
         int CW;
+        bool ws;
         std::vector<int> idx;
     };
     std::vector<Group> groups;
@@ -340,9 +388,11 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         if (mode & kPrep) launch_prep_mfma(j, plans[i], ws + offs[i], s);
         if (!(mode & kRun)) continue;
         auto it = std::find_if(groups.begin(), groups.end(),
-                               [&](const Group &g) { return g.bf16 == plans[i].bf16 && g.CW == plans[i].CW; });
+                               [&](const Group &g) {
+                                   return g.bf16 == plans[i].bf16 && g.CW == plans[i].CW && g.ws == plans[i].ws;
+                               });
         if (it == groups.end()) {
-            groups.push_back({plans[i].bf16, plans[i].CW, {}});
+            groups.push_back({plans[i].bf16, plans[i].CW, plans[i].ws, {}});
             it = groups.end() - 1;
         }
         it->idx.push_back(i);
@@ -364,7 +414,7 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
             for (int q = 0; q < pk.n; ++q) pk.jobs[q] = descs[b + q];
             hipLaunchKernelGGL(write_jobs_kernel, dim3(1), dim3(64), 0, s, pk, dev_jobs + desc_base + b);
         }
-        launch_mfma_group(gr.bf16, gr.CW, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+        launch_mfma_group(gr.bf16, gr.CW, gr.ws, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
         if (int r = hip_status("MFMA beamformer launch")) return r;
         desc_base += (int)descs.size();
     }

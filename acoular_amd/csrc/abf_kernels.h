@@ -65,6 +65,7 @@ __global__ void prep_plain_kernel(const float2 *csm, int M, int Mp, int r_diag, 
 __global__ void bf_plain_kernel(PlainArgs a);
 __global__ void prep_mfma_kernel(const float2 *csm, int M, int Mp, int r_diag, int bf16x3, char *prep);
 template <bool BF16X3, int CW> __global__ void bf_mfma_kernel(const MfJob *jobs, int njobs);
+template <bool BF16X3> __global__ void bf_mfma_ws_kernel(const MfJob *jobs, int njobs);
 __global__ void bf_eig_kernel(EigArgs a);
 __global__ void bf_custom_kernel(CustomArgs a);
 

@@ -10,6 +10,13 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
+// Pointers read from job descriptors in memory are generic (flat) to the compiler; flat
+// loads count against the LDS wait counter too, so an LDS wait would also wait for every
+// prefetch in flight.  These casts make such accesses global loads/stores.
+#define ABF_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ const ABF_GLOBAL T *gp(const T *p) { return (const ABF_GLOBAL T *)p; }
+template <typename T> __device__ __forceinline__ ABF_GLOBAL T *gpw(T *p) { return (ABF_GLOBAL T *)p; }
+
 __device__ __forceinline__ int upper_index(int I, int J, int nK) { return I * nK - (I * (I - 1)) / 2 + (J - I); }
 
 __device__ __forceinline__ int find_job(const MfJob *__restrict__ jobs, int njobs, int64_t wg)

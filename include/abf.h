@@ -60,6 +60,13 @@ enum abf_phase {
     ABF_PHASE_FAST_TRIG = 0x100
 };
 
+/* MFMA kernel variants (tile_kernel of abf_grid_job). */
+enum abf_mfma_kernel {
+    ABF_MFMA_KERNEL_AUTO = 0,
+    ABF_MFMA_KERNEL_ALTERNATING = 1, /* all waves generate the steering matrix, then all run MFMA */
+    ABF_MFMA_KERNEL_SPECIALISED = 2  /* steering waves beside MFMA waves, double-buffered */
+};
+
 /* Steering geometry shared by the grid entry points. r0 = ref_dist if ref_dist > 0, else
  * |g - ref| (SteeringVector.r0, fbeamform.py:187-193). */
 typedef struct abf_geometry {
@@ -90,6 +97,8 @@ typedef struct abf_grid_job {
     float *out_norm;          /* [num_freqs][num_grid] normalizeSteer, may be NULL */
     int32_t tile_cols;        /* MFMA forms: grid points per workgroup (16..256, power of two); 0 = auto */
     int32_t tile_freqs;       /* MFMA forms: frequencies per workgroup; 0 = auto */
+    int32_t tile_kernel;      /* MFMA forms: enum abf_mfma_kernel; 0 = auto */
+    int32_t pad1_;
 } abf_grid_job;
 
 /* Bytes of workspace abf_beamform needs for this job (the re-laid-out CSM). */

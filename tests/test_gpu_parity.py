@@ -338,3 +338,47 @@ def test_batch_equals_alone_bitwise(gpu):
     _lib.check(lib.abf_beamform_batch(arr, len(jobs), _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
     for a, o in zip(alone, tensors[4::5]):
         np.testing.assert_array_equal(o.cpu().numpy(), a)
+
+
+@pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f32'])
+@pytest.mark.parametrize('kernel,tile_cols,tile_freqs', [(1, 0, 0), (1, 16, 1), (1, 64, 3), (2, 0, 0), (2, 64, 2),
+                                                         (2, 32, 5)])
+@pytest.mark.parametrize('nm,steer', [(64, 'true level'), (100, 'classic'), (256, 'inverse'), (33, 'true location')])
+def test_mfma_kernel_variants_and_tiles(gpu, form, kernel, tile_cols, tile_freqs, nm, steer):
+    """Both MFMA kernels (alternating, warp-specialised) and explicit tile shapes, including
+    ragged microphone counts and bands that do not divide into the frequency tile."""
+    torch = _lib.torch()
+    freqs = np.array([600.0, 2500.0, 7000.0, 12000.0, 15000.0, 21000.0, 3100.0])
+    grid = aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.25, y_max=0.25, z=0.4, increment=0.02)
+    mics, grid, csm = _scene(nm, grid, freqs, z=0.4, seed=nm)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer)
+    gpos, mpos = sv.device_geometry()
+    dcsm = _lib.to_device(csm, torch.complex64)
+    k = torch.from_numpy(2 * np.pi * freqs / 343.0).to(gpu)
+    out = torch.full((len(freqs), grid.size), -7.0, dtype=torch.float32, device=gpu)
+    norm = nm / (nm - 1.0)
+    job = make_job(dcsm, k, gpos, mpos, sv.ref, steer, True, norm, form, 'reference', out, None, True)
+    job.tile_kernel, job.tile_cols, job.tile_freqs = kernel, tile_cols, tile_freqs
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
+    _lib.check(lib.abf_beamform(job, _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
+    ref = _oracle_maps(csm, freqs, mics, grid, steer, True)
+    err = peak_rel_err(out.cpu().numpy(), ref)
+    print(f'form={form} kernel={kernel} tile=({tile_cols},{tile_freqs}) nm={nm} steer={steer} err={err:.2e}')
+    assert err <= TOL
+
+
+def test_mfma_kernel_variant_rejected_when_unknown(gpu):
+    torch = _lib.torch()
+    freqs = np.array([1000.0])
+    grid = aa.RectGrid(x_min=-0.1, x_max=0.1, y_min=-0.1, y_max=0.1, z=0.4, increment=0.05)
+    mics, grid, csm = _scene(16, grid, freqs)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='classic')
+    gpos, mpos = sv.device_geometry()
+    dcsm = _lib.to_device(csm, torch.complex64)
+    k = torch.from_numpy(2 * np.pi * freqs / 343.0).to(gpu)
+    out = torch.empty((1, grid.size), dtype=torch.float32, device=gpu)
+    job = make_job(dcsm, k, gpos, mpos, sv.ref, 'classic', False, 1.0, 'mfma_bf16x3', 'reference', out)
+    job.tile_kernel = 7
+    with pytest.raises(ValueError):
+        _lib.check(_lib.lib().abf_beamform(job, None, 0, _lib.stream_handle()))
