@@ -65,6 +65,9 @@ class AbfGridJob(ctypes.Structure):
 # function name -> (restype, argtypes)
 _SIGS = {
     'abf_beamform_workspace_size': (ctypes.c_size_t, [ctypes.POINTER(AbfGridJob)]),
+    'abf_beamform_plan': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_int64)]),
     'abf_beamform': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), _c_dp, ctypes.c_size_t, _c_dp]),
     'abf_beamform_prepare': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), _c_dp, ctypes.c_size_t, _c_dp]),
     'abf_beamform_run': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), _c_dp, ctypes.c_size_t, _c_dp]),

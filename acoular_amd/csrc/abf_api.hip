@@ -336,6 +336,29 @@ size_t abf_beamform_workspace_size(const abf_grid_job *job)
     return abf_beamform_batch_workspace_size(job, 1, nullptr);
 }
 
+int abf_beamform_plan(const abf_grid_job *job, int32_t *tile_kernel, int32_t *tile_cols, int32_t *tile_freqs,
+                      int64_t *workgroups)
+{
+    if (int r = validate_job(job)) return r;
+    int32_t k = 0, c = 64, f = 1;
+    int64_t w = 0;
+    if (job->form == ABF_FORM_PLAIN_F32) {
+        w = cdiv(job->geom.num_grid, 64) * std::max(job->num_freqs, 0);
+    } else {
+        MfPlan p;
+        if (int r = plan_mfma(job, p)) return r;
+        k = p.ws ? ABF_MFMA_KERNEL_SPECIALISED : ABF_MFMA_KERNEL_ALTERNATING;
+        c = 16 * p.CT;
+        f = p.FB;
+        w = p.wgs;
+    }
+    if (tile_kernel) *tile_kernel = k;
+    if (tile_cols) *tile_cols = c;
+    if (tile_freqs) *tile_freqs = f;
+    if (workgroups) *workgroups = w;
+    return ABF_OK;
+}
+
 size_t abf_beamform_batch_workspace_size(const abf_grid_job *jobs, int32_t num_jobs, size_t *offsets)
 {
     size_t off = align256((size_t)std::max(num_jobs, 1) * sizeof(MfJob));

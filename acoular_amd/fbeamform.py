@@ -23,7 +23,9 @@ from .spectra import PowerSpectra
 STEER_TYPES = ('true level', 'true location', 'classic', 'inverse')
 
 #: defaults of the GPU path (not reference traits)
-config = {'form': 'mfma_bf16x3', 'phase_mode': 'reference', 'fast_trig': True}
+# autotuned: MFMA jobs take their launch shape from the autotuner's cache when it holds their
+# shape class (acoular_amd.autotune)
+config = {'form': 'mfma_bf16x3', 'phase_mode': 'reference', 'fast_trig': True, 'autotuned': True}
 
 
 class SteeringVector(HasParams):
@@ -362,6 +364,10 @@ def make_job(csm, k, gpos, mpos, ref, steer_type, r_diag, norm, form, phase_mode
     j.phase_mode = _lib.PHASE_MODES[phase_mode] | (_lib.FAST_TRIG if fast_trig else 0)
     j.out = out.data_ptr()
     j.out_norm = out_norm.data_ptr() if out_norm is not None else None
+    if config['autotuned']:
+        from . import autotune
+
+        autotune.apply(j)
     return j
 
 

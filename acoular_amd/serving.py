@@ -101,13 +101,20 @@ class BeamformServer:
     """
 
     def __init__(self, hbm_budget: int = 32 << 30, max_batch: int = 64, num_streams: int = 2,
-                 form: str = 'mfma_bf16x3', batched: bool = True, phase_mode: str = 'reference'):
+                 form: str = 'mfma_bf16x3', batched: bool = True, phase_mode: str = 'reference',
+                 keep_results: bool = True, staging_bytes: int = 0):
+        """``keep_results=False`` drops each map once it has reached host memory (long benchmark
+        runs); ``staging_bytes`` > 0 gives every stream a reusable pinned host buffer of that size
+        for the device-to-host copies instead of a pinned allocation per request."""
         torch = _lib.torch()
         self.dev = _lib.device()
         self.budget = int(hbm_budget)
         self.max_batch = max_batch if batched else 1
         # default-priority streams only
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(num_streams)]
+        self.keep_results = keep_results
+        self.staging = {id(s): torch.empty(int(staging_bytes), dtype=torch.uint8, pin_memory=True)
+                        for s in self.streams} if staging_bytes > 0 else {}
         self.form = form
         self.phase_mode = phase_mode
         self.inflight: list[_Launch] = []
@@ -134,7 +141,8 @@ class BeamformServer:
             if la.event.query():
                 t = now_fn()
                 for r, h in zip(la.reqs, la.host_out):
-                    r.result = h.numpy()
+                    if self.keep_results:
+                        r.result = h.numpy().copy() if self.staging else h.numpy()
                     r.t_done = t
                     if r.grid_fn is not None:
                         r.gpos = None  # regenerated grids are not kept after completion
@@ -177,8 +185,15 @@ class BeamformServer:
             ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=self.dev)
             _lib.check(lib.abf_beamform_batch(arr, len(jobs), _lib.ptr(ws), ws.numel(),
                                               _lib.ctypes.c_void_p(stream.cuda_stream)), 'serve batch')
+            stage = self.staging.get(id(stream))
+            off = 0
             for o in tensors[0::2]:
-                h = torch.empty(o.shape, dtype=o.dtype, pin_memory=True)
+                nbytes = o.numel() * o.element_size()
+                if stage is not None and off + nbytes <= stage.numel():
+                    h = stage[off:off + nbytes].view(o.dtype).view(o.shape)
+                    off += (nbytes + 255) // 256 * 256
+                else:
+                    h = torch.empty(o.shape, dtype=o.dtype, pin_memory=True)
                 h.copy_(o, non_blocking=True)
                 host_out.append(h)
             ev = torch.cuda.Event()

@@ -171,6 +171,61 @@ def cpu_baseline(wl, budget_s):
                       f'{dt:.1f} s'}
 
 
+def _serve(server_kw, reqs, realtime):
+    from acoular_amd import serving
+
+    srv = serving.BeamformServer(**server_kw)
+    t = srv.run(reqs, realtime=realtime, progress=log)
+    st = serving.latency_stats(reqs)
+    bins = sum(r.map_bins for r in reqs)
+    st.update({'requests': len(reqs), 'wall_s': t, 'req_per_s': len(reqs) / t, 'map_bins_per_s': bins / t,
+               'launches': srv.launches, 'mean_batch': float(np.mean(srv.batch_sizes)) if srv.batch_sizes else 0.0})
+    return st
+
+
+def serving_bench(args):
+    """Mixed-size serving: 10,000 requests (mic count in {32..512}, grid 1k..250k points 2D or 3D,
+    1..64 contiguous bins, analytic CSMs of 1..16 sources), Poisson arrivals at a fixed fraction of
+    the measured saturation rate of the batched MFMA engine; latency = arrival -> map in host
+    memory.  Baseline: the unbatched plain kernel (one request per launch, one launch at a time)
+    on the same request stream."""
+    from acoular_amd import serving
+
+    log('serving: resident CSM pools (analytic, 513 bins, 32..512 mics)')
+    fac = serving.RequestFactory(seed=7, pool=2)
+    mfma = dict(hbm_budget=24 << 30, max_batch=64, num_streams=2, keep_results=False, staging_bytes=1 << 30)
+    plain = dict(hbm_budget=24 << 30, max_batch=1, num_streams=1, form='plain_f32', batched=False,
+                 keep_results=False, staging_bytes=1 << 30)
+    _serve(mfma, fac.make(40, None, seed_offset=9), False)  # warm-up: first launches of every shape
+    _serve(plain, fac.make(4, None, seed_offset=9), False)
+    sat = _serve(mfma, fac.make(args.serve_sat, None, seed_offset=1), False)
+    log(f"saturation (all {args.serve_sat} requests queued at once): {sat['req_per_s']:.1f} req/s, "
+        f"{sat['map_bins_per_s']:.3g} map-bins/s")
+    qps = args.serve_load * sat['req_per_s']
+    log(f'timed run: {args.serve_requests} requests, Poisson at {qps:.1f} req/s')
+    reqs = fac.make(args.serve_requests, qps, seed_offset=2)
+    run = _serve(mfma, reqs, True)
+    log(f"p50 {run['p50_ms']:.1f} ms, p99 {run['p99_ms']:.1f} ms, {run['map_bins_per_s']:.3g} map-bins/s")
+    log('baseline: unbatched plain kernel')
+    psat = _serve(plain, fac.make(args.serve_plain, None, seed_offset=1), False)
+    preqs = fac.make(args.serve_plain, qps, seed_offset=2)  # the first requests of the same stream
+    prun = _serve(plain, preqs, True)
+    log(f"plain: saturation {psat['req_per_s']:.1f} req/s; at {qps:.1f} req/s offered p50 {prun['p50_ms']:.1f} "
+        f"ms, p99 {prun['p99_ms']:.1f} ms")
+    return {
+        'metric': 'serving p99 latency (mixed-size beamform requests, continuous batching)',
+        'value': run['p99_ms'], 'unit': 'ms', 'n_gpus': 1, 'higher_is_better': False, 'vs_baseline': None,
+        'dtype': 'bf16x3 (split hi+lo, fp32 accumulate)', 'data': 'synthetic (seeded analytic CSMs)',
+        'config': {'workload': 'serving loop: mic count in {32,64,128,256,512}, grid log-uniform 1k..250k points '
+                               '(2D/3D), 1..64 contiguous bins, 1..16 sources, r_diag, all four steer types',
+                   'requests': args.serve_requests, 'offered_req_per_s': qps, 'load_fraction': args.serve_load,
+                   'hbm_budget_bytes': mfma['hbm_budget'], 'max_batch': mfma['max_batch'],
+                   'streams': mfma['num_streams'], 'parallelism': 'single GPU'},
+        'serving': run, 'saturation': sat,
+        'plain_unbatched': {'saturation': psat, 'same_offered_load_first_requests': prun},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -179,9 +234,16 @@ def main():
     ap.add_argument('--form', default='mfma_bf16x3', choices=['mfma_bf16x3', 'mfma_f32', 'plain_f32'])
     ap.add_argument('--no-compare', action='store_true', help='skip the other forms')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner (default shapes)')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
-    ap.add_argument('--mode', default='headline', choices=['headline', 'copy'],
-                    help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters")
+    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving'],
+                    help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
+                         "'serving': the mixed-size request loop (Poisson arrivals) against the unbatched "
+                         "plain kernel")
+    ap.add_argument('--serve-requests', type=int, default=10000, help='serving: requests of the timed run')
+    ap.add_argument('--serve-load', type=float, default=0.65, help='serving: offered load / saturation rate')
+    ap.add_argument('--serve-sat', type=int, default=600, help='serving: requests of the saturation run')
+    ap.add_argument('--serve-plain', type=int, default=300, help='serving: requests of the plain baseline runs')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is '
@@ -200,9 +262,24 @@ def main():
         print(json.dumps({'mode': 'copy', 'bytes_read_per_dispatch': src.numel() * 4,
                           'bytes_written_per_dispatch': dst.numel() * 4, 'dispatches': 3}))
         return
+    if args.mode == 'serving':
+        print(json.dumps(serving_bench(args)), flush=True)
+        return
     log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
     wl = build_workload(dev)
     runner = Runner(wl, args.form, dev=dev)
+    tiles = None
+    if args.form != 'plain_f32' and not args.no_tune:
+        # one-time launch-shape tuning, outside the timed region
+        from acoular_amd import autotune
+
+        res = autotune.tune_job(runner.job, reps=2)
+        runner.job.tile_kernel, runner.job.tile_cols, runner.job.tile_freqs = (
+            res['tile_kernel'], res['tile_cols'], res['tile_freqs'])
+        tiles = {k: res[k] for k in ('tile_kernel', 'tile_cols', 'tile_freqs')}
+        tiles['candidates_timed'] = len(res['tried'])
+        log(f"autotuned launch shape: {tiles} ({res['seconds'] * 1e3:.2f} ms; slowest tried "
+            f"{res['tried'][-1]['seconds'] * 1e3:.2f} ms)")
     log(f'form {args.form}: warmup {args.warmup}, steps {args.steps}')
     t_step = time_steps(runner.step, args.steps, args.warmup)
     value = runner.map_bins / t_step
@@ -262,7 +339,8 @@ def main():
         'config': {'workload': 'plain vs MFMA beamforming: 256 mics (2 m disk), 8 sources (20 dB spread), '
                                'analytic c64 CSM 513 bins, RectGrid 201x201 z=0.5, classic, r_diag',
                    'num_mics': nm, 'num_grid': int(wl['gpos'].shape[1]), 'num_freqs': len(wl['freqs']),
-                   'form': args.form, 'phase_mode': 'reference', 'parallelism': 'single GPU'},
+                   'form': args.form, 'phase_mode': 'reference', 'launch_shape': tiles,
+                   'parallelism': 'single GPU'},
         'max_rel_err': err,
         'forms_map_bins_per_s': forms,
         'roofline': roofline,

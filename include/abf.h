@@ -104,6 +104,12 @@ typedef struct abf_grid_job {
 /* Bytes of workspace abf_beamform needs for this job (the re-laid-out CSM). */
 size_t abf_beamform_workspace_size(const abf_grid_job *job);
 
+/* The launch shape abf_beamform uses for this job after its tile requests are clamped to what
+ * the kernels support: kernel variant, grid points and frequencies per workgroup, workgroups
+ * (plain form: 0, 64, 1, blocks).  No counterpart in the reference: it serves the autotuner. */
+int abf_beamform_plan(const abf_grid_job *job, int32_t *tile_kernel, int32_t *tile_cols, int32_t *tile_freqs,
+                      int64_t *workgroups);
+
 /* Beamform a job.  fastFuncs.beamformerFreq (fastFuncs.py:61-214) for steer types 1..4,
  * batched over every frequency slice of the CSM. */
 int abf_beamform(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream);
