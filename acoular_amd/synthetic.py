@@ -110,3 +110,41 @@ def point_source_signals(mpos, spos, num_samples: int, fs: float, c: float = 343
         rms = np.sqrt(np.mean(out**2, axis=0))
         out += rng.standard_normal(out.shape) * (noise_rel * rms)[None, :]
     return out.astype(dtype)
+
+
+def point_source_signals_device(mpos, spos, num_samples: int, fs: float, c: float = 343.0, seed: int = 1,
+                                noise_rel: float = 0.0, mixing=None, chunk: int = 64):
+    """Device (float32 ``[N, M]`` torch tensor) counterpart of ``point_source_signals`` for large
+    arrays and long signals: the same propagation model (delay ``r_sm / c`` as a circular linear
+    phase, amplitude ``1 / r_sm``), computed with the GPU FFT.
+
+    Source signals are seeded unit-variance white noise; with ``mixing`` (``[S, S]``) they are
+    mixed as ``s = mixing @ w`` (correlated sources).  Microphone noise: independent white noise of
+    ``noise_rel`` times each microphone's source-signal rms (seeded torch generator on the device).
+    """
+    import torch
+
+    dev = torch.device('cuda', 0)
+    rng = np.random.default_rng(seed)
+    mpos = np.asarray(mpos, dtype=float)
+    spos = np.asarray(spos, dtype=float)
+    ns, nm = spos.shape[1], mpos.shape[1]
+    w = rng.standard_normal((ns, num_samples))
+    if mixing is not None:
+        w = np.asarray(mixing, dtype=float) @ w
+    rsm = np.sqrt(((spos[:, :, None] - mpos[:, None, :]) ** 2).sum(0))  # [S, M]
+    omega = torch.from_numpy(2 * np.pi * np.fft.rfftfreq(num_samples, 1.0 / fs)).to(dev)
+    out = torch.zeros((num_samples, nm), dtype=torch.float32, device=dev)
+    for s in range(ns):
+        spec = torch.fft.rfft(torch.from_numpy(w[s]).to(dev))
+        for m0 in range(0, nm, chunk):
+            m1 = min(nm, m0 + chunk)
+            r = torch.from_numpy(rsm[s, m0:m1]).to(dev)
+            ph = torch.exp(-1j * omega[:, None] * (r / c)[None, :]) / r[None, :]
+            out[:, m0:m1] += torch.fft.irfft(spec[:, None] * ph, n=num_samples, dim=0).to(torch.float32)
+    if noise_rel > 0:
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(seed) + 7919)
+        rms = torch.sqrt(torch.mean(out.double() ** 2, dim=0)).float()
+        out += torch.randn(out.shape, generator=gen, device=dev, dtype=torch.float32) * (noise_rel * rms)[None, :]
+    return out
