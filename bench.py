@@ -308,6 +308,86 @@ def fused_bench(args):
     }
 
 
+def large3d_bench(args):
+    """Large 3D map: 1024 mics on a 3 m spiral + seeded jitter, RectGrid3D 64 x 64 x 32 (131,072
+    points; x, y in [-1, 1], z from 0.5, increment 2/63), 256 bins (bins 2..257 of block size 1024
+    at 51.2 kHz) of an analytic complex64 CSM of 32 seeded 3D sources (20 dB spread), true level,
+    r_diag; autotuned split-bf16 MFMA against the plain FP32 kernel."""
+    import torch
+
+    import acoular_amd as aa
+    from acoular_amd import _lib, autotune, synthetic
+    from acoular_amd.fbeamform import make_job
+    from oracle import beamform as B
+
+    dev = torch.device('cuda', 0)
+    nm, seed = 1024, 4004
+    mpos = synthetic.spiral_array(nm, 3.0, seed)
+    inc = 2.0 / 63
+    grid = aa.RectGrid3D(x_min=-1.0, x_max=1.0, y_min=-1.0, y_max=1.0, z_min=0.5, z_max=0.5 + 31 * inc,
+                         increment=inc)
+    gpos = grid.pos
+    assert gpos.shape[1] == 64 * 64 * 32
+    spos = synthetic.source_positions(32, [-0.9, -0.9, 0.6], [0.9, 0.9, 1.4], seed + 1)
+    freqs = np.arange(2, 258) * 51200.0 / 1024
+    log('large3d: analytic CSM 256 x 1024 x 1024 on the GPU')
+    csm = synthetic.analytic_csm(mpos, spos, synthetic.level_powers(32, 20.0, seed + 2), freqs, device=dev)
+    k = torch.from_numpy(2 * np.pi * freqs / 343.0).to(dev)
+    gd = torch.from_numpy(gpos).to(dev)
+    md = torch.from_numpy(mpos).to(dev)
+    lib = _lib.lib()
+    res = {}
+    outs = {}
+    for form in ('mfma_bf16x3', 'plain_f32'):
+        out = torch.empty((len(freqs), gpos.shape[1]), dtype=torch.float32, device=dev)
+        job = make_job(csm, k, gd, md, np.zeros(3), 'true level', True, nm / (nm - 1.0), form, 'reference', out,
+                       None, clip=True)
+        tiles = None
+        if form != 'plain_f32' and not args.no_tune:
+            tr = autotune.tune_job(job, reps=1)
+            job.tile_kernel, job.tile_cols, job.tile_freqs = tr['tile_kernel'], tr['tile_cols'], tr['tile_freqs']
+            tiles = {kk: tr[kk] for kk in ('tile_kernel', 'tile_cols', 'tile_freqs')}
+            log(f'autotuned launch shape: {tiles}')
+        ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
+
+        def step(job=job, ws=ws):
+            _lib.check(lib.abf_beamform(job, _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
+
+        n = max(1, args.steps // 3) if form != 'plain_f32' else 1
+        t = time_steps(step, n, 1)
+        res[form] = {'ms_per_step': t * 1e3, 'map_bins_per_s': out.numel() / t, 'launch_shape': tiles,
+                     'alg_tflops': out.numel() * 8.0 * nm * (nm + 1) / t / 1e12}
+        log(f"{form}: {t * 1e3:.1f} ms -> {res[form]['map_bins_per_s']:.4g} map-bins/s")
+        outs[form] = out
+        del ws
+    err = None
+    bins = [40, 200]
+    if not args.no_compare:
+        # oracle on 8192 seeded grid points (plus the map maximum) of two bins
+        rng = np.random.default_rng(seed + 3)
+        errs = []
+        for b in bins:
+            got = outs['mfma_bf16x3'][b].double().cpu().numpy()
+            sub = np.unique(np.concatenate([rng.choice(gpos.shape[1], 8192, replace=False), [int(np.argmax(got))]]))
+            ref = B.beamformer_map(csm[b:b + 1].cpu().numpy(), freqs[b:b + 1], gpos[:, sub], mpos, 'true level',
+                                   True)[0]
+            errs.append(float(np.max(np.abs(got[sub] - ref)) / np.max(np.abs(ref))))
+        err = max(errs)
+        log(f'max relative map error on bins {bins} (8192 sampled points): {err:.2e}')
+    m = res['mfma_bf16x3']
+    return {
+        'metric': 'map-bins/s (grid pts x freqs) at <=1e-4 rel err vs FP64', 'value': m['map_bins_per_s'],
+        'unit': 'map-bins/s', 'n_gpus': 1, 'ms_per_step': m['ms_per_step'], 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'bf16x3 (split hi+lo, fp32 accumulate)',
+        'data': 'synthetic (seeded analytic CSM)',
+        'config': {'workload': 'large 3D map: 1024 mics (3 m spiral + jitter), RectGrid3D 64x64x32, 256 bins, '
+                               '32 sources, true level, r_diag', 'num_mics': nm, 'num_grid': int(gpos.shape[1]),
+                   'num_freqs': len(freqs), 'parallelism': 'single GPU'},
+        'forms': res, 'mfma_over_plain': m['map_bins_per_s'] / res['plain_f32']['map_bins_per_s'],
+        'max_rel_err': err, 'max_rel_err_bins': bins,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -318,11 +398,12 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner (default shapes)')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
-    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused'],
+    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
                          "'serving': the mixed-size request loop (Poisson arrivals) against the unbatched "
                          "plain kernel; 'fused': Welch CSM + beamforming of 512 mics, 10 s at 51.2 kHz, "
-                         "1025 bins, 128x128 grid, replayed as one hipGraph")
+                         "1025 bins, 128x128 grid, replayed as one hipGraph; 'large3d': 1024 mics, "
+                         "64x64x32 grid, 256 bins, autotuned MFMA vs the plain kernel")
     ap.add_argument('--serve-requests', type=int, default=10000, help='serving: requests of the timed run')
     ap.add_argument('--serve-load', type=float, default=0.65, help='serving: offered load / saturation rate')
     ap.add_argument('--serve-sat', type=int, default=600, help='serving: requests of the saturation run')
@@ -350,6 +431,9 @@ def main():
         return
     if args.mode == 'fused':
         print(json.dumps(fused_bench(args)), flush=True)
+        return
+    if args.mode == 'large3d':
+        print(json.dumps(large3d_bench(args)), flush=True)
         return
     log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
     wl = build_workload(dev)
