@@ -577,13 +577,9 @@ __global__ __launch_bounds__(kMfWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     const int NCG = CT / CW;
     const ABF_GLOBAL char *prep_base = gp(reinterpret_cast<const char *>(jb.prep));
 
-    // Workgroups running side by side start their band at different frequencies, so that they
-    // stream different U blocks at any moment (blockIdx goes round-robin over the 8 XCDs, so the
-    // workgroups sharing one L2 are tiles 8 apart).
-    const int nf = f1 - f0;
-    const int rot = (tile >> 3) % nf;
-    for (int i = 0; i < nf; ++i) {
-        const int f = f0 + (i + rot < nf ? i + rot : i + rot - nf);
+    // Frequencies in band order: workgroups running side by side on one XCD then read the U
+    // blocks of the same one or two frequencies at any moment, which keeps them in that XCD's L2.
+    for (int f = f0; f < f1; ++f) {
         steer_generate_any<BF16X3>(fast, rr, live, gp(jb.wavenumber)[f], refph, wk, col, part, TPC, ngroups, KS, pe,
                                    smem);
         __syncthreads();
@@ -640,12 +636,8 @@ __global__ __launch_bounds__((kWsMf + kGenWaves) * 64) void bf_mfma_ws_kernel(co
     const int KS = BF16X3 ? Mp + 8 : Mp + 4;
     const int nK = Mp / 32;
     const int nUp = nK * (nK + 1) / 2;
-    // band order rotated per tile (see bf_mfma_kernel): logical frequency f -> stored frequency
-    const int rot = (tile >> 3) % nf;
-    auto phys = [&](int f) __attribute__((always_inline)) {
-        const int i = f - f0 + rot;
-        return f0 + (i < nf ? i : i - nf);
-    };
+    // frequencies in band order (see bf_mfma_kernel)
+    auto phys = [&](int f) __attribute__((always_inline)) { return f; };
 
     // LDS: two X buffers [planes][BN][KS] (8 bytes per element), q partials [2][kWsMf][BN],
     // r0 [BN], steering-sum partials [TPC][BN]
