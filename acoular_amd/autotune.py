@@ -10,7 +10,10 @@ the number of bins (the frequency tile amortises the float64 distance prologue).
 ``tune_job`` times every distinct launch shape of one prepared job with HIP events on the launch
 stream and returns the fastest; ``TileCache`` keeps the winners per shape class in a JSON file
 (``$ABF_TUNE_CACHE`` or ``~/.cache/acoular_amd/tune.json``) so that later jobs of the same class
-(``make_job`` consults the cache) start from the measured best without tuning again.
+(``make_job`` consults the cache) start from the measured best without tuning again.  Classes
+missing there fall back to the table shipped with the package (``tiles_gfx950.json``), which
+``python -m acoular_amd.autotune --out acoular_amd/tiles_gfx950.json`` regenerates on an MI355X
+for the standard workload classes (``STANDARD_CLASSES``).
 """
 
 from __future__ import annotations
@@ -24,8 +27,9 @@ import threading
 from . import _lib
 
 KERNELS = (1, 2)  # abf_mfma_kernel: alternating, warp-specialised
+SHIPPED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tiles_gfx950.json')
 TILE_COLS = (16, 32, 64, 128)
-TILE_FREQS = (2, 4, 8, 16, 32)
+TILE_FREQS = (2, 4, 8, 16, 32, 64)
 
 
 def _bucket(n: int) -> int:
@@ -47,20 +51,26 @@ def default_cache_path() -> str:
 class TileCache:
     """Measured best launch shapes per shape class, persisted as JSON."""
 
-    def __init__(self, path: str | None = None):
+    def __init__(self, path: str | None = None, fallback: str | None = None):
         self.path = path or default_cache_path()
         self._lock = threading.Lock()
-        self.entries: dict[str, dict] = {}
+        self.entries = self._load(self.path)
+        self.fallback = self._load(fallback) if fallback else {}
+
+    @staticmethod
+    def _load(path: str) -> dict[str, dict]:
         try:
-            with open(self.path) as fh:
+            with open(path) as fh:
                 data = json.load(fh)
-            if isinstance(data, dict):
-                self.entries = {str(k): dict(v) for k, v in data.items() if isinstance(v, dict)}
         except (OSError, ValueError):
-            self.entries = {}
+            return {}
+        if not isinstance(data, dict):
+            return {}
+        return {str(k): dict(v) for k, v in data.items() if isinstance(v, dict)}
 
     def get(self, key: str) -> dict | None:
-        return self.entries.get(key)
+        e = self.entries.get(key)
+        return e if e is not None else self.fallback.get(key)
 
     def put(self, key: str, entry: dict, save: bool = True) -> None:
         with self._lock:
@@ -87,7 +97,7 @@ _cache: TileCache | None = None
 def cache() -> TileCache:
     global _cache
     if _cache is None:
-        _cache = TileCache()
+        _cache = TileCache(fallback=SHIPPED_PATH)
     return _cache
 
 
@@ -177,3 +187,57 @@ def tune_job(job: _lib.AbfGridJob, reps: int = 3, kernels=KERNELS, tile_cols=TIL
         (tiles or cache()).put(job_key(job), {k: result[k] for k in ('tile_kernel', 'tile_cols', 'tile_freqs',
                                                                      'seconds')})
     return result
+
+
+# (form, mics, grid points, bins) of the classes the shipped table covers: the 256-mic 201 x 201
+# plain-vs-MFMA map, the fused 512-mic 128 x 128 map of 1025 bins, the 1024-mic 64 x 64 x 32 map
+# and the serving mix (mic counts 32..512, small and large grids, narrow and wide bands)
+STANDARD_CLASSES = (
+    [('mfma_bf16x3', 256, 40401, 513), ('mfma_f32', 256, 40401, 513), ('mfma_bf16x3', 512, 16384, 1025),
+     ('mfma_bf16x3', 1024, 131072, 256)]
+    + [('mfma_bf16x3', m, g, f) for m in (32, 64, 128, 256, 512) for g in (4096, 65536) for f in (4, 32)]
+)
+
+
+def tune_class(form: str, num_mics: int, num_grid: int, num_freqs: int, reps: int = 2, tiles=None) -> dict:
+    """Tune one shape class on a synthetic job of that shape (seeded random Hermitian CSM, grid
+    points in a 2 m box at z = 1, microphones in a 2 m disk; timing does not depend on values)."""
+    import numpy as np
+
+    from .fbeamform import make_job
+
+    torch = _lib.torch()
+    dev = _lib.device()
+    rng = np.random.default_rng(num_mics * 7 + num_grid + num_freqs)
+    a = rng.standard_normal((num_freqs, num_mics, 4)) + 1j * rng.standard_normal((num_freqs, num_mics, 4))
+    csm = torch.from_numpy((a @ np.conj(a.transpose(0, 2, 1))).astype(np.complex64)).to(dev)
+    gpos = torch.from_numpy(np.stack([rng.uniform(-1, 1, num_grid), rng.uniform(-1, 1, num_grid),
+                                      np.ones(num_grid)])).to(dev)
+    r = np.sqrt(rng.uniform(0, 1, num_mics))
+    ph = rng.uniform(0, 2 * np.pi, num_mics)
+    mpos = torch.from_numpy(np.stack([r * np.cos(ph), r * np.sin(ph), np.zeros(num_mics)])).to(dev)
+    k = torch.from_numpy(2 * np.pi * np.linspace(500.0, 20000.0, num_freqs) / 343.0).to(dev)
+    out = torch.empty((num_freqs, num_grid), dtype=torch.float32, device=dev)
+    job = make_job(csm, k, gpos, mpos, np.zeros(3), 'true level', True, 1.0, form, 'reference', out, None, True)
+    job.tile_kernel = job.tile_cols = job.tile_freqs = 0
+    return tune_job(job, reps=reps, tiles=tiles)
+
+
+def main(argv=None):
+    import argparse
+
+    ap = argparse.ArgumentParser(description='Tune the MFMA launch shapes of the standard workload classes.')
+    ap.add_argument('--out', default=None, help='JSON file to write (default: the user cache)')
+    ap.add_argument('--reps', type=int, default=2)
+    args = ap.parse_args(argv)
+    tiles = TileCache(args.out) if args.out else cache()
+    for form, m, g, f in STANDARD_CLASSES:
+        res = tune_class(form, m, g, f, reps=args.reps, tiles=tiles)
+        print(json.dumps({'class': shape_key(form, m, g, f),
+                          **{k: res[k] for k in ('tile_kernel', 'tile_cols', 'tile_freqs', 'seconds')},
+                          'tried': len(res['tried'])}), flush=True)
+    tiles.save()
+
+
+if __name__ == '__main__':
+    main()

@@ -57,11 +57,11 @@ def build_workload(dev, seed=2024):
 class Runner:
     """One prepared job of a given form over the resident workload."""
 
-    def __init__(self, wl, form, steer='classic', r_diag=True, dev=None):
+    def __init__(self, wl, form, steer='classic', r_diag=True, dev=None, tuned=True):
         import torch
 
         from acoular_amd import _lib
-        from acoular_amd.fbeamform import make_job
+        from acoular_amd.fbeamform import config, make_job
 
         self.lib = _lib.lib()
         self._lib = _lib
@@ -71,8 +71,12 @@ class Runner:
         self.k = torch.from_numpy(2 * np.pi * wl['freqs'] / 343.0).to(dev)
         self.out = torch.empty((len(wl['freqs']), wl['gpos'].shape[1]), dtype=torch.float32, device=dev)
         norm = nm / (nm - 1.0) if r_diag else 1.0
-        self.job = make_job(wl['csm'], self.k, self.gpos, self.mpos, np.zeros(3), steer, r_diag, norm, form,
-                            'reference', self.out, None, clip=r_diag)
+        saved, config['autotuned'] = config['autotuned'], tuned
+        try:
+            self.job = make_job(wl['csm'], self.k, self.gpos, self.mpos, np.zeros(3), steer, r_diag, norm, form,
+                                'reference', self.out, None, clip=r_diag)
+        finally:
+            config['autotuned'] = saved
         self.ws = _lib.workspace(self.lib.abf_beamform_workspace_size(self.job))
         self.csm = wl['csm']
         self.map_bins = self.out.numel()
@@ -268,11 +272,18 @@ def fused_bench(args):
                          ind_high=bs // 2 + 1)
     p.run_csm()
     tiles = None
-    if not args.no_tune:
-        res = autotune.tune_job(p.job, reps=1)
-        p.job.tile_kernel, p.job.tile_cols, p.job.tile_freqs = res['tile_kernel'], res['tile_cols'], res['tile_freqs']
-        tiles = {k: res[k] for k in ('tile_kernel', 'tile_cols', 'tile_freqs')}
-        log(f'autotuned launch shape: {tiles}')
+    if args.no_tune:
+        p.job.tile_kernel = p.job.tile_cols = p.job.tile_freqs = 0
+    else:
+        src = 'autotuner table'
+        if args.tune or p.job.tile_cols == 0:
+            res = autotune.tune_job(p.job, reps=1)
+            p.job.tile_kernel, p.job.tile_cols, p.job.tile_freqs = (res['tile_kernel'], res['tile_cols'],
+                                                                    res['tile_freqs'])
+            src = 'tuned now'
+        tiles = {'tile_kernel': p.job.tile_kernel, 'tile_cols': p.job.tile_cols, 'tile_freqs': p.job.tile_freqs,
+                 'source': src}
+        log(f'launch shape: {tiles}')
     t_csm = event_time(p.run_csm, 2)
     t_bf = event_time(p.run_beamform, 2)
     t_eager = time_steps(p.run, 2, 1)
@@ -343,11 +354,17 @@ def large3d_bench(args):
         job = make_job(csm, k, gd, md, np.zeros(3), 'true level', True, nm / (nm - 1.0), form, 'reference', out,
                        None, clip=True)
         tiles = None
-        if form != 'plain_f32' and not args.no_tune:
-            tr = autotune.tune_job(job, reps=1)
-            job.tile_kernel, job.tile_cols, job.tile_freqs = tr['tile_kernel'], tr['tile_cols'], tr['tile_freqs']
-            tiles = {kk: tr[kk] for kk in ('tile_kernel', 'tile_cols', 'tile_freqs')}
-            log(f'autotuned launch shape: {tiles}')
+        if args.no_tune:
+            job.tile_kernel = job.tile_cols = job.tile_freqs = 0
+        elif form != 'plain_f32':
+            src = 'autotuner table'
+            if args.tune or job.tile_cols == 0:
+                tr = autotune.tune_job(job, reps=1)
+                job.tile_kernel, job.tile_cols, job.tile_freqs = tr['tile_kernel'], tr['tile_cols'], tr['tile_freqs']
+                src = 'tuned now'
+            tiles = {'tile_kernel': job.tile_kernel, 'tile_cols': job.tile_cols, 'tile_freqs': job.tile_freqs,
+                     'source': src}
+            log(f'launch shape: {tiles}')
         ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
 
         def step(job=job, ws=ws):
@@ -396,7 +413,8 @@ def main():
     ap.add_argument('--form', default='mfma_bf16x3', choices=['mfma_bf16x3', 'mfma_f32', 'plain_f32'])
     ap.add_argument('--no-compare', action='store_true', help='skip the other forms')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
-    ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner (default shapes)')
+    ap.add_argument('--no-tune', action='store_true', help='default launch shapes (no autotuner table, no tuning)')
+    ap.add_argument('--tune', action='store_true', help='tune the launch shape now even if the table has it')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
@@ -437,19 +455,22 @@ def main():
         return
     log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
     wl = build_workload(dev)
-    runner = Runner(wl, args.form, dev=dev)
+    runner = Runner(wl, args.form, dev=dev, tuned=not args.no_tune)
     tiles = None
     if args.form != 'plain_f32' and not args.no_tune:
-        # one-time launch-shape tuning, outside the timed region
+        # launch shape from the autotuner's table (shipped or user cache); tuned now, outside the
+        # timed region, when the table lacks this shape class or --tune asks for it
         from acoular_amd import autotune
 
-        res = autotune.tune_job(runner.job, reps=2)
-        runner.job.tile_kernel, runner.job.tile_cols, runner.job.tile_freqs = (
-            res['tile_kernel'], res['tile_cols'], res['tile_freqs'])
-        tiles = {k: res[k] for k in ('tile_kernel', 'tile_cols', 'tile_freqs')}
-        tiles['candidates_timed'] = len(res['tried'])
-        log(f"autotuned launch shape: {tiles} ({res['seconds'] * 1e3:.2f} ms; slowest tried "
-            f"{res['tried'][-1]['seconds'] * 1e3:.2f} ms)")
+        src = 'autotuner table'
+        if args.tune or runner.job.tile_cols == 0:
+            res = autotune.tune_job(runner.job, reps=2)
+            runner.job.tile_kernel, runner.job.tile_cols, runner.job.tile_freqs = (
+                res['tile_kernel'], res['tile_cols'], res['tile_freqs'])
+            src = f"tuned now ({len(res['tried'])} shapes timed)"
+        tiles = {'tile_kernel': runner.job.tile_kernel, 'tile_cols': runner.job.tile_cols,
+                 'tile_freqs': runner.job.tile_freqs, 'source': src}
+        log(f'launch shape: {tiles}')
     log(f'form {args.form}: warmup {args.warmup}, steps {args.steps}')
     t_step = time_steps(runner.step, args.steps, args.warmup)
     value = runner.map_bins / t_step
