@@ -94,6 +94,26 @@ class Runner:
         self.run()
 
 
+def measured_traffic(form, runner):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs (profiles/hbm_traffic.json, calibrated against a 1 GiB copy) when they were
+    taken with this form and launch shape; None otherwise."""
+    from acoular_amd import autotune
+
+    try:
+        with open(os.path.join(ROOT, 'profiles', 'hbm_traffic.json')) as fh:
+            table = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if form == 'plain_f32':
+        key = 'bf_plain_kernel|plain_f32|256mic_201x201_513bins|0,64,1'
+    else:
+        k, c, f, _ = autotune.plan(runner.job)
+        key = f'bf_mfma_kernel|{form}|256mic_201x201_513bins|{k},{c},{f}'
+    e = table.get(key)
+    return None if e is None else int(e['bytes_per_launch'])
+
+
 def executed_flops_per_bin(form, nm):
     """Flops the kernels execute per map-bin (the upper block triangle of U only)."""
     if form == 'plain_f32':
@@ -485,7 +505,8 @@ def main():
     achieved = alg_flops / t_run / 1e12
     executed = executed_flops_per_bin(args.form, nm)
     roofline = {'bound': 'mfma' if args.form != 'plain_f32' else 'valu', 'achieved': round(achieved, 2),
-                'peak': bound_peak, 'unit': 'TFLOP/s', 'frac': round(achieved / bound_peak, 4), 'traffic': None,
+                'peak': bound_peak, 'unit': 'TFLOP/s', 'frac': round(achieved / bound_peak, 4),
+                'traffic': measured_traffic(args.form, runner),
                 'kernel': 'bf_mfma_kernel' if args.form != 'plain_f32' else 'bf_plain_kernel',
                 'kernel_ms': round(t_run * 1e3, 3),
                 'algorithmic_flops_per_launch': alg_flops,
