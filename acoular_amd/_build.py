@@ -9,7 +9,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libabf.so')
-SOURCES = ['abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip', 'abf_api.hip']
+SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f32.hip', 'abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip',
+           'abf_api.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 # packed fp32 instructions are disabled on purpose (build flags)
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-Xclang', '-target-feature', '-Xclang',
@@ -28,7 +29,7 @@ def _compile(src: str, build_dir: str) -> str:
 def build(verbose: bool = False) -> str:
     build_dir = os.path.join(HERE, '_build')
     os.makedirs(build_dir, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         objs = list(ex.map(lambda s: _compile(s, build_dir), SOURCES))
     tmp = LIB + '.tmp'
     cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp, *objs]

@@ -57,6 +57,7 @@ int check_phase(int pm)
 // ---------------------------------------------------------------------------------------------
 struct MfPlan {
     int Mp, CT, CW, FB, nK, nUp;
+    int TT;  // row tiles per work unit (alternating kernel)
     bool ws; // warp-specialised kernel (double-buffered steering matrix)
     int64_t ntiles, wgs;
     size_t prep_bytes, lds;
@@ -132,7 +133,8 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     }
     if (p.CT == 0)
         return fail(ABF_ERR_UNSUPPORTED, "MFMA form supports at most ~1150 microphones; use the plain form");
-    p.CW = p.ws ? 2 : std::min(p.CT, 4);
+    p.TT = kreq == ABF_MFMA_KERNEL_ROW_PAIRS ? 2 : 1;
+    p.CW = p.ws ? 2 : std::min(p.CT, p.TT == 2 ? 2 : 4);
     p.ntiles = cdiv(j->geom.num_grid, 16 * p.CT);
     p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
     p.lds = p.ws ? mfma_ws_lds(p.bf16, 16 * p.CT, p.Mp) : mfma_lds(p.bf16, 16 * p.CT, p.Mp);
@@ -150,7 +152,7 @@ int validate_job(const abf_grid_job *j)
     if (int r = check_steer(j->steer_type)) return r;
     if (int r = check_phase(j->phase_mode)) return r;
     if (j->form < ABF_FORM_PLAIN_F32 || j->form > ABF_FORM_MFMA_F32) return fail(ABF_ERR_INVALID, "unknown form");
-    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_SPECIALISED)
+    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_ROW_PAIRS)
         return fail(ABF_ERR_INVALID, "unknown MFMA kernel variant");
     if (j->tile_cols < 0 || j->tile_freqs < 0) return fail(ABF_ERR_INVALID, "negative tile size");
     return ABF_OK;
@@ -206,42 +208,65 @@ __global__ void twiddle_kernel(float2 *tw, int bs)
     tw[j] = make_float2((float)c, (float)-s);
 }
 
-template <bool B, int CW> void launch_mfma(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+template <bool B, int CW, int V, int TT>
+void launch_mfma(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)bf_mfma_kernel<B, CW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)kLdsMax);
+        (void)hipFuncSetAttribute((const void *)bf_mfma_kernel<B, CW, V, TT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
         attr = true;
     }
-    hipLaunchKernelGGL((bf_mfma_kernel<B, CW>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
+    hipLaunchKernelGGL((bf_mfma_kernel<B, CW, V, TT>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
 }
 
-template <bool B> void launch_mfma_ws(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+template <bool B, int V> void launch_mfma_ws(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)bf_mfma_ws_kernel<B>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void *)bf_mfma_ws_kernel<B, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)kLdsMax);
         attr = true;
     }
-    hipLaunchKernelGGL((bf_mfma_ws_kernel<B>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
+    hipLaunchKernelGGL((bf_mfma_ws_kernel<B, V>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
 }
 
-void launch_mfma_group(bool bf16, int CW, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+template <bool B, int V>
+void launch_mfma_form(int CW, int TT, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
-    if (ws) {
-        if (bf16) launch_mfma_ws<true>(dev, n, wgs, lds, s);
-        else launch_mfma_ws<false>(dev, n, wgs, lds, s);
-    } else if (bf16) {
-        if (CW == 1) launch_mfma<true, 1>(dev, n, wgs, lds, s);
-        else if (CW == 2) launch_mfma<true, 2>(dev, n, wgs, lds, s);
-        else launch_mfma<true, 4>(dev, n, wgs, lds, s);
-    } else {
-        if (CW == 1) launch_mfma<false, 1>(dev, n, wgs, lds, s);
-        else if (CW == 2) launch_mfma<false, 2>(dev, n, wgs, lds, s);
-        else launch_mfma<false, 4>(dev, n, wgs, lds, s);
+    if (ws) launch_mfma_ws<B, V>(dev, n, wgs, lds, s);
+    else if (TT == 2 && CW == 1) launch_mfma<B, 1, V, 2>(dev, n, wgs, lds, s);
+    else if (TT == 2) launch_mfma<B, 2, V, 2>(dev, n, wgs, lds, s);
+    else if (CW == 1) launch_mfma<B, 1, V, 1>(dev, n, wgs, lds, s);
+    else if (CW == 2) launch_mfma<B, 2, V, 1>(dev, n, wgs, lds, s);
+    else launch_mfma<B, 4, V, 1>(dev, n, wgs, lds, s);
+}
+
+// steering-generation variant of a job: fast trig (bit 0) | weight kind << 1
+int mfma_variant(const abf_grid_job *j)
+{
+    return ((j->phase_mode & kFastTrigFlag) ? 1 : 0) | (weight_kind(j->steer_type) << 1);
+}
+
+template <bool B>
+void launch_mfma_var(int var, int CW, int TT, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds,
+                     hipStream_t s)
+{
+    switch (var) {
+    case 0: launch_mfma_form<B, 0>(CW, TT, ws, dev, n, wgs, lds, s); break;
+    case 1: launch_mfma_form<B, 1>(CW, TT, ws, dev, n, wgs, lds, s); break;
+    case 2: launch_mfma_form<B, 2>(CW, TT, ws, dev, n, wgs, lds, s); break;
+    case 3: launch_mfma_form<B, 3>(CW, TT, ws, dev, n, wgs, lds, s); break;
+    case 4: launch_mfma_form<B, 4>(CW, TT, ws, dev, n, wgs, lds, s); break;
+    default: launch_mfma_form<B, 5>(CW, TT, ws, dev, n, wgs, lds, s); break;
     }
+}
+
+void launch_mfma_group(bool bf16, int CW, int TT, bool ws, int var, const MfJob *dev, int n, int64_t wgs, size_t lds,
+                       hipStream_t s)
+{
+    if (bf16) launch_mfma_var<true>(var, CW, TT, ws, dev, n, wgs, lds, s);
+    else launch_mfma_var<false>(var, CW, TT, ws, dev, n, wgs, lds, s);
 }
 
 void prep_plain(const abf_grid_job *j, void *ws, hipStream_t s)
@@ -347,7 +372,8 @@ int abf_beamform_plan(const abf_grid_job *job, int32_t *tile_kernel, int32_t *ti
     } else {
         MfPlan p;
         if (int r = plan_mfma(job, p)) return r;
-        k = p.ws ? ABF_MFMA_KERNEL_SPECIALISED : ABF_MFMA_KERNEL_ALTERNATING;
+        k = p.ws ? ABF_MFMA_KERNEL_SPECIALISED
+                 : (p.TT == 2 ? ABF_MFMA_KERNEL_ROW_PAIRS : ABF_MFMA_KERNEL_ALTERNATING);
         c = 16 * p.CT;
         f = p.FB;
         w = p.wgs;
@@ -388,11 +414,13 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
     char *ws = reinterpret_cast<char *>(workspace);
     MfJob *dev_jobs = reinterpret_cast<MfJob *>(ws);
 
-    // plain jobs run one after the other; MFMA jobs are grouped per (form, kernel, CW) into one launch
+    // plain jobs run one after the other; MFMA jobs are grouped per (form, kernel, CW, steering
+    // variant) into one launch
     struct Group {
         bool bf16;
-        int CW;
+        int CW, TT;
         bool ws;
+        int var;
         std::vector<int> idx;
     };
     std::vector<Group> groups;
@@ -412,10 +440,11 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         if (!(mode & kRun)) continue;
         auto it = std::find_if(groups.begin(), groups.end(),
                                [&](const Group &g) {
-                                   return g.bf16 == plans[i].bf16 && g.CW == plans[i].CW && g.ws == plans[i].ws;
+                                   return g.bf16 == plans[i].bf16 && g.CW == plans[i].CW && g.TT == plans[i].TT &&
+                                          g.ws == plans[i].ws && g.var == mfma_variant(&jobs[i]);
                                });
         if (it == groups.end()) {
-            groups.push_back({plans[i].bf16, plans[i].CW, plans[i].ws, {}});
+            groups.push_back({plans[i].bf16, plans[i].CW, plans[i].TT, plans[i].ws, mfma_variant(&jobs[i]), {}});
             it = groups.end() - 1;
         }
         it->idx.push_back(i);
@@ -437,7 +466,7 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
             for (int q = 0; q < pk.n; ++q) pk.jobs[q] = descs[b + q];
             hipLaunchKernelGGL(write_jobs_kernel, dim3(1), dim3(64), 0, s, pk, dev_jobs + desc_base + b);
         }
-        launch_mfma_group(gr.bf16, gr.CW, gr.ws, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+        launch_mfma_group(gr.bf16, gr.CW, gr.TT, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
         if (int r = hip_status("MFMA beamformer launch")) return r;
         desc_base += (int)descs.size();
     }

@@ -104,7 +104,7 @@ __device__ __forceinline__ void normalise(int steer_type, double q, double r0, d
 // row tile t of the same block hold, so the epilogue q = Re sum conj(x) Y needs no lane exchange.
 __device__ __forceinline__ int kslot_mic(int g, int e) { return 4 * g + (e & 3) + 16 * (e >> 2); }
 
-__device__ __forceinline__ int weight_kind(int steer_type)
+__host__ __device__ __forceinline__ int weight_kind(int steer_type)
 {
     return steer_type == ABF_STEER_CLASSIC ? 0 : (steer_type == ABF_STEER_INVERSE ? 1 : 2);
 }
