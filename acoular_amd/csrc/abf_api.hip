@@ -58,6 +58,7 @@ int check_phase(int pm)
 struct MfPlan {
     int Mp, CT, CW, FB, nK, nUp;
     int TT;  // row tiles per work unit (alternating kernel)
+    bool pf; // software-pipelined fragment loads (split-bf16 alternating kernel)
     bool ws; // warp-specialised kernel (double-buffered steering matrix)
     int64_t ntiles, wgs;
     size_t prep_bytes, lds;
@@ -125,7 +126,9 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
             if (ct > ct_cap || (ct_req > 0 && ct > ct_req)) continue;
             const int BN = 16 * ct;
             const int TPC = 512 / BN;
-            if (p.Mp / 8 > TPC * 5) continue; // steering prologue covers at most 5 groups of 8 per thread
+            // steering prologue: at most 5 groups of 8 per thread (4 for the pipelined kernels)
+            const bool pf_req = kreq == ABF_MFMA_KERNEL_PIPELINED || kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS;
+            if (p.Mp / 8 > TPC * (pf_req && p.bf16 ? 4 : 5)) continue;
             if (mfma_lds(p.bf16, BN, p.Mp) > kLdsMax) continue;
             p.CT = ct;
             break;
@@ -133,8 +136,12 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     }
     if (p.CT == 0)
         return fail(ABF_ERR_UNSUPPORTED, "MFMA form supports at most ~1150 microphones; use the plain form");
-    p.TT = kreq == ABF_MFMA_KERNEL_ROW_PAIRS ? 2 : 1;
-    p.CW = p.ws ? 2 : std::min(p.CT, p.TT == 2 ? 2 : 4);
+    p.TT = (kreq == ABF_MFMA_KERNEL_ROW_PAIRS || kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS) ? 2 : 1;
+    p.pf = !p.ws && p.bf16 && (kreq == ABF_MFMA_KERNEL_PIPELINED || kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS);
+    if (p.ws) p.CW = 2;
+    else if (kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS) p.CW = 1;
+    else if (p.TT == 2 || kreq == ABF_MFMA_KERNEL_PIPELINED) p.CW = std::min(p.CT, 2);
+    else p.CW = std::min(p.CT, 4);
     p.ntiles = cdiv(j->geom.num_grid, 16 * p.CT);
     p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
     p.lds = p.ws ? mfma_ws_lds(p.bf16, 16 * p.CT, p.Mp) : mfma_lds(p.bf16, 16 * p.CT, p.Mp);
@@ -152,7 +159,7 @@ int validate_job(const abf_grid_job *j)
     if (int r = check_steer(j->steer_type)) return r;
     if (int r = check_phase(j->phase_mode)) return r;
     if (j->form < ABF_FORM_PLAIN_F32 || j->form > ABF_FORM_MFMA_F32) return fail(ABF_ERR_INVALID, "unknown form");
-    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_ROW_PAIRS)
+    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS)
         return fail(ABF_ERR_INVALID, "unknown MFMA kernel variant");
     if (j->tile_cols < 0 || j->tile_freqs < 0) return fail(ABF_ERR_INVALID, "negative tile size");
     return ABF_OK;
@@ -208,16 +215,16 @@ __global__ void twiddle_kernel(float2 *tw, int bs)
     tw[j] = make_float2((float)c, (float)-s);
 }
 
-template <bool B, int CW, int V, int TT>
+template <bool B, int CW, int V, int TT, bool PF>
 void launch_mfma(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)bf_mfma_kernel<B, CW, V, TT>,
+        (void)hipFuncSetAttribute((const void *)bf_mfma_kernel<B, CW, V, TT, PF>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
         attr = true;
     }
-    hipLaunchKernelGGL((bf_mfma_kernel<B, CW, V, TT>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
+    hipLaunchKernelGGL((bf_mfma_kernel<B, CW, V, TT, PF>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
 }
 
 template <bool B, int V> void launch_mfma_ws(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
@@ -232,14 +239,24 @@ template <bool B, int V> void launch_mfma_ws(const MfJob *dev, int n, int64_t wg
 }
 
 template <bool B, int V>
-void launch_mfma_form(int CW, int TT, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+void launch_mfma_form(int CW, int TT, bool pf, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds,
+                      hipStream_t s)
 {
     if (ws) launch_mfma_ws<B, V>(dev, n, wgs, lds, s);
-    else if (TT == 2 && CW == 1) launch_mfma<B, 1, V, 2>(dev, n, wgs, lds, s);
-    else if (TT == 2) launch_mfma<B, 2, V, 2>(dev, n, wgs, lds, s);
-    else if (CW == 1) launch_mfma<B, 1, V, 1>(dev, n, wgs, lds, s);
-    else if (CW == 2) launch_mfma<B, 2, V, 1>(dev, n, wgs, lds, s);
-    else launch_mfma<B, 4, V, 1>(dev, n, wgs, lds, s);
+    else if constexpr (B) {
+        if (pf) {
+            if (TT == 2) launch_mfma<B, 1, V, 2, true>(dev, n, wgs, lds, s);
+            else if (CW == 1) launch_mfma<B, 1, V, 1, true>(dev, n, wgs, lds, s);
+            else launch_mfma<B, 2, V, 1, true>(dev, n, wgs, lds, s);
+            return;
+        }
+    }
+    if (ws) return;
+    if (TT == 2 && CW == 1) launch_mfma<B, 1, V, 2, false>(dev, n, wgs, lds, s);
+    else if (TT == 2) launch_mfma<B, 2, V, 2, false>(dev, n, wgs, lds, s);
+    else if (CW == 1) launch_mfma<B, 1, V, 1, false>(dev, n, wgs, lds, s);
+    else if (CW == 2) launch_mfma<B, 2, V, 1, false>(dev, n, wgs, lds, s);
+    else launch_mfma<B, 4, V, 1, false>(dev, n, wgs, lds, s);
 }
 
 // steering-generation variant of a job: fast trig (bit 0) | weight kind << 1
@@ -249,24 +266,24 @@ int mfma_variant(const abf_grid_job *j)
 }
 
 template <bool B>
-void launch_mfma_var(int var, int CW, int TT, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds,
+void launch_mfma_var(int var, int CW, int TT, bool pf, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds,
                      hipStream_t s)
 {
     switch (var) {
-    case 0: launch_mfma_form<B, 0>(CW, TT, ws, dev, n, wgs, lds, s); break;
-    case 1: launch_mfma_form<B, 1>(CW, TT, ws, dev, n, wgs, lds, s); break;
-    case 2: launch_mfma_form<B, 2>(CW, TT, ws, dev, n, wgs, lds, s); break;
-    case 3: launch_mfma_form<B, 3>(CW, TT, ws, dev, n, wgs, lds, s); break;
-    case 4: launch_mfma_form<B, 4>(CW, TT, ws, dev, n, wgs, lds, s); break;
-    default: launch_mfma_form<B, 5>(CW, TT, ws, dev, n, wgs, lds, s); break;
+    case 0: launch_mfma_form<B, 0>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
+    case 1: launch_mfma_form<B, 1>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
+    case 2: launch_mfma_form<B, 2>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
+    case 3: launch_mfma_form<B, 3>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
+    case 4: launch_mfma_form<B, 4>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
+    default: launch_mfma_form<B, 5>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
     }
 }
 
-void launch_mfma_group(bool bf16, int CW, int TT, bool ws, int var, const MfJob *dev, int n, int64_t wgs, size_t lds,
-                       hipStream_t s)
+void launch_mfma_group(bool bf16, int CW, int TT, bool pf, bool ws, int var, const MfJob *dev, int n, int64_t wgs,
+                       size_t lds, hipStream_t s)
 {
-    if (bf16) launch_mfma_var<true>(var, CW, TT, ws, dev, n, wgs, lds, s);
-    else launch_mfma_var<false>(var, CW, TT, ws, dev, n, wgs, lds, s);
+    if (bf16) launch_mfma_var<true>(var, CW, TT, pf, ws, dev, n, wgs, lds, s);
+    else launch_mfma_var<false>(var, CW, TT, pf, ws, dev, n, wgs, lds, s);
 }
 
 void prep_plain(const abf_grid_job *j, void *ws, hipStream_t s)
@@ -372,8 +389,9 @@ int abf_beamform_plan(const abf_grid_job *job, int32_t *tile_kernel, int32_t *ti
     } else {
         MfPlan p;
         if (int r = plan_mfma(job, p)) return r;
-        k = p.ws ? ABF_MFMA_KERNEL_SPECIALISED
-                 : (p.TT == 2 ? ABF_MFMA_KERNEL_ROW_PAIRS : ABF_MFMA_KERNEL_ALTERNATING);
+        if (p.ws) k = ABF_MFMA_KERNEL_SPECIALISED;
+        else if (p.pf) k = p.TT == 2 ? ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS : ABF_MFMA_KERNEL_PIPELINED;
+        else k = p.TT == 2 ? ABF_MFMA_KERNEL_ROW_PAIRS : ABF_MFMA_KERNEL_ALTERNATING;
         c = 16 * p.CT;
         f = p.FB;
         w = p.wgs;
@@ -419,7 +437,7 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
     struct Group {
         bool bf16;
         int CW, TT;
-        bool ws;
+        bool pf, ws;
         int var;
         std::vector<int> idx;
     };
@@ -441,10 +459,12 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         auto it = std::find_if(groups.begin(), groups.end(),
                                [&](const Group &g) {
                                    return g.bf16 == plans[i].bf16 && g.CW == plans[i].CW && g.TT == plans[i].TT &&
-                                          g.ws == plans[i].ws && g.var == mfma_variant(&jobs[i]);
+                                          g.pf == plans[i].pf && g.ws == plans[i].ws &&
+                                          g.var == mfma_variant(&jobs[i]);
                                });
         if (it == groups.end()) {
-            groups.push_back({plans[i].bf16, plans[i].CW, plans[i].TT, plans[i].ws, mfma_variant(&jobs[i]), {}});
+            groups.push_back(
+                {plans[i].bf16, plans[i].CW, plans[i].TT, plans[i].pf, plans[i].ws, mfma_variant(&jobs[i]), {}});
             it = groups.end() - 1;
         }
         it->idx.push_back(i);
@@ -466,7 +486,7 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
             for (int q = 0; q < pk.n; ++q) pk.jobs[q] = descs[b + q];
             hipLaunchKernelGGL(write_jobs_kernel, dim3(1), dim3(64), 0, s, pk, dev_jobs + desc_base + b);
         }
-        launch_mfma_group(gr.bf16, gr.CW, gr.TT, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+        launch_mfma_group(gr.bf16, gr.CW, gr.TT, gr.pf, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
         if (int r = hip_status("MFMA beamformer launch")) return r;
         desc_base += (int)descs.size();
     }

@@ -65,8 +65,10 @@ __global__ void prep_plain_kernel(const float2 *csm, int M, int Mp, int r_diag, 
 __global__ void bf_plain_kernel(PlainArgs a);
 __global__ void prep_mfma_kernel(const float2 *csm, int M, int Mp, int r_diag, int bf16x3, char *prep);
 // VAR = fast trig (bit 0) | steering weight kind (0 none, 1 r, 2 1/r) << 1
-// TT = row tiles (of 16) per work unit: 1, or 2 (both halves of a 32-row block)
-template <bool BF16X3, int CW, int VAR, int TT> __global__ void bf_mfma_kernel(const MfJob *jobs, int njobs);
+// TT = row tiles (of 16) per work unit: 1, or 2 (both halves of a 32-row block); PF = software-
+// pipelined X/U fragment loads (split-bf16 only)
+template <bool BF16X3, int CW, int VAR, int TT, bool PF>
+__global__ void bf_mfma_kernel(const MfJob *jobs, int njobs);
 template <bool BF16X3, int VAR> __global__ void bf_mfma_ws_kernel(const MfJob *jobs, int njobs);
 __global__ void bf_eig_kernel(EigArgs a);
 __global__ void bf_custom_kernel(CustomArgs a);

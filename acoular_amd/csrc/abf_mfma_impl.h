@@ -17,14 +17,15 @@ constexpr int kGenWaves = 4; // steering waves of the warp-specialised kernel
 // groups gi = part + TPC s (group gi: block gi / 4, lane group gi % 4) into rr (1 for slots
 // without a real microphone or grid point).  Returns this thread's share of the steering
 // normalisation sum (sum r^2 or sum 1/r^2).
+template <int NS>
 __device__ __forceinline__ double steer_distances(const MfJob &jb, bool valid, double gx, double gy, double gz,
-                                                  int part, int TPC, int ngroups, double (&rr)[kMfNS][8])
+                                                  int part, int TPC, int ngroups, double (&rr)[NS][8])
 {
     const int M = jb.M;
     const ABF_GLOBAL double *mpos = gp(jb.mpos);
     double help = 0.0;
 #pragma unroll
-    for (int s = 0; s < kMfNS; ++s) {
+    for (int s = 0; s < NS; ++s) {
         const int gi = part + TPC * s;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -46,14 +47,14 @@ __device__ __forceinline__ double steer_distances(const MfJob &jb, bool valid, d
 // the 8 independent phasors of a group interleave without selects.  Slots without a real
 // microphone or grid point get finite values (r = 1): the re-laid-out CSM is zero in the rows and
 // columns of padded microphones and the maps of padded grid points are not stored.
-template <bool BF16X3, bool FAST, int WK>
-__device__ __forceinline__ void steer_generate(const double (&rr)[kMfNS][8], double k, bool refph, int col, int part,
+template <bool BF16X3, bool FAST, int WK, int NS>
+__device__ __forceinline__ void steer_generate(const double (&rr)[NS][8], double k, bool refph, int col, int part,
                                                int TPC, int ngroups, int KS, int64_t pe, char *xbuf)
 {
     uint16_t *xh16 = reinterpret_cast<uint16_t *>(xbuf);
     float *x32 = reinterpret_cast<float *>(xbuf);
 #pragma unroll
-    for (int s = 0; s < kMfNS; ++s) {
+    for (int s = 0; s < NS; ++s) {
         const int gi = part + TPC * s;
         if (gi < ngroups) {
             float xr[8], xi[8];
@@ -277,6 +278,115 @@ __device__ __forceinline__ void gemm_frequency(int wave, int lane, int nK, int N
     }
 }
 
+// Split-bf16 units with a software pipeline: the X fragments (LDS) and U fragments (L2) of block
+// J+1 are requested before the MFMAs of block J, into a second register set, so a wave does not
+// wait on LDS at the top of every block.  The J loop is unrolled by two so the two register sets
+// alternate without copies; past the last block the loads repeat block J (in bounds) instead of
+// branching, which keeps the wait counts exact.  TT row tiles (1 or 2 halves of a 32-row block)
+// per unit, CW column tiles per U fragment.
+template <int CW, int TT>
+__device__ __forceinline__ void gemm_frequency_pf(int wave, int lane, int nK, int NCG, const ABF_GLOBAL char *prep_f,
+                                                  const char *xbuf, int64_t pe, int KS, int BN, float *s_q)
+{
+    const uint16_t *xh16 = reinterpret_cast<const uint16_t *>(xbuf);
+    const u4 sg = u4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+    const int nunits = (2 / TT) * nK * NCG;
+    const int rounds = (nunits + kMfWaves - 1) / kMfWaves;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int u = rd * kMfWaves + ((rd & 1) ? (kMfWaves - 1 - wave) : wave);
+        if (u >= nunits) continue;
+        const int rt = u / NCG, cg = u % NCG;
+        const int I = TT == 2 ? rt : rt >> 1;
+        const int t0 = TT == 2 ? 0 : (rt & 1);
+        f4 accr[TT][CW], acci[TT][CW];
+#pragma unroll
+        for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                accr[tt][c] = f4{0.f, 0.f, 0.f, 0.f};
+                acci[tt][c] = f4{0.f, 0.f, 0.f, 0.f};
+            }
+        const ABF_GLOBAL char *ablk = prep_f + ((int64_t)upper_index(I, I, nK) * 2 + t0) * 4096; // (I, I), half t0
+        const int colbase = cg * CW * 16 + (lane & 15);
+        const int kq = (lane >> 4) * 8;
+        auto load_a = [&](int J, u4 (&d)[TT][4]) __attribute__((always_inline)) {
+            const ABF_GLOBAL char *b = ablk + (int64_t)(J - I) * 8192;
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) d[tt][q] = reinterpret_cast<const ABF_GLOBAL u4 *>(b + tt * 4096)[64 * q + lane];
+        };
+        auto load_x = [&](int J, u4 (&d)[CW][4]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                const int64_t xo = (int64_t)(colbase + c * 16) * KS + J * 32 + kq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) d[c][q] = *reinterpret_cast<const u4 *>(xh16 + q * pe + xo);
+            }
+        };
+        auto block = [&](const u4 (&a)[TT][4], const u4 (&x)[CW][4]) __attribute__((always_inline)) {
+#define MF(A, B, C) C = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, A), \
+                                                                __builtin_bit_cast(bf16x8, B), C, 0, 0, 0)
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt) {
+                const u4 a_rh = a[tt][0], a_rl = a[tt][1], a_ih = a[tt][2], a_il = a[tt][3];
+                const u4 a_nh = a_ih ^ sg, a_nl = a_il ^ sg;
+#pragma unroll
+                for (int c = 0; c < CW; ++c) { MF(a_rh, x[c][0], accr[tt][c]); MF(a_ih, x[c][0], acci[tt][c]); }
+#pragma unroll
+                for (int c = 0; c < CW; ++c) { MF(a_rh, x[c][1], accr[tt][c]); MF(a_ih, x[c][1], acci[tt][c]); }
+#pragma unroll
+                for (int c = 0; c < CW; ++c) { MF(a_rl, x[c][0], accr[tt][c]); MF(a_il, x[c][0], acci[tt][c]); }
+#pragma unroll
+                for (int c = 0; c < CW; ++c) { MF(a_nh, x[c][2], accr[tt][c]); MF(a_rh, x[c][2], acci[tt][c]); }
+#pragma unroll
+                for (int c = 0; c < CW; ++c) { MF(a_nh, x[c][3], accr[tt][c]); MF(a_rh, x[c][3], acci[tt][c]); }
+#pragma unroll
+                for (int c = 0; c < CW; ++c) { MF(a_nl, x[c][2], accr[tt][c]); MF(a_rl, x[c][2], acci[tt][c]); }
+            }
+#undef MF
+        };
+        u4 a0[TT][4], a1[TT][4], x0[CW][4], x1[CW][4];
+        load_a(I, a0);
+        load_x(I, x0);
+        for (int J = I;;) {
+            const int Jn = J + 1 < nK ? J + 1 : J;
+            load_a(Jn, a1);
+            load_x(Jn, x1);
+            __builtin_amdgcn_sched_barrier(0);
+            block(a0, x0);
+            if (++J >= nK) break;
+            const int Jm = J + 1 < nK ? J + 1 : J;
+            load_a(Jm, a0);
+            load_x(Jm, x0);
+            __builtin_amdgcn_sched_barrier(0);
+            block(a1, x1);
+            if (++J >= nK) break;
+        }
+        // epilogue: result rows 4 g + i of row half t sit at K-slot 8 g + 4 t + i of block I
+#pragma unroll
+        for (int tt = 0; tt < TT; ++tt) {
+            const int slot0 = I * 32 + (lane >> 4) * 8 + (t0 + tt) * 4;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                const int colc = colbase + c * 16;
+                const int64_t xo = (int64_t)colc * KS + slot0;
+                float sacc = 0.f;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float xr = bf16_to_float(xh16[xo + e]) + bf16_to_float(xh16[pe + xo + e]);
+                    const float xi = bf16_to_float(xh16[2 * pe + xo + e]) + bf16_to_float(xh16[3 * pe + xo + e]);
+                    sacc = fmaf(xr, accr[tt][c][e], sacc);
+                    sacc = fmaf(xi, acci[tt][c][e], sacc);
+                }
+                sacc += __shfl_xor(sacc, 16);
+                sacc += __shfl_xor(sacc, 32);
+                if (lane < 16) s_q[wave * BN + colc] += sacc;
+            }
+        }
+    }
+}
+
 // Normalise and store frequency f of the tile: q = sum of the NQ partials in s_q[NQ][BN]
 // (fixed order, then cleared), steering sum = sum of help_parts partials of s_help[part][BN].
 template <int NQ>
@@ -312,7 +422,7 @@ __device__ __forceinline__ void store_frequency(const MfJob &jb, int f, int tile
 // steering matrix X of the tile (BN x Mp, split hi+lo bf16 or fp32, K-slot order of kslot_mic)
 // into LDS, then run gemm_frequency; q is reduced per grid point in a fixed order
 // (bit-identical results alone and batched), then normalised per steer type.
-template <bool BF16X3, int CW, int VAR, int TT>
+template <bool BF16X3, int CW, int VAR, int TT, bool PF>
 __global__ __launch_bounds__(kMfWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void bf_mfma_kernel(
     const MfJob *__restrict__ jobs, int njobs)
 {
@@ -353,8 +463,9 @@ __global__ __launch_bounds__(kMfWaves * 64) __attribute__((amdgpu_waves_per_eu(2
         gz = gp(jb.gpos)[2 * G + g];
     }
     const int ngroups = Mp / 8;
-    double rr[kMfNS][8];
-    double help = steer_distances(jb, valid, gx, gy, gz, part, TPC, ngroups, rr);
+    constexpr int NS = PF ? 4 : kMfNS; // pipelined kernels keep registers for the X prefetch
+    double rr[NS][8];
+    double help = steer_distances<NS>(jb, valid, gx, gy, gz, part, TPC, ngroups, rr);
     for (int off = TPC >> 1; off > 0; off >>= 1) help += __shfl_xor(help, off);
     if (part == 0) {
         s_help[col] = help;
@@ -371,11 +482,14 @@ __global__ __launch_bounds__(kMfWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     // Frequencies in band order: workgroups running side by side on one XCD then read the U
     // blocks of the same one or two frequencies at any moment, which keeps them in that XCD's L2.
     for (int f = f0; f < f1; ++f) {
-        steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1)>(rr, gp(jb.wavenumber)[f], refph, col, part, TPC, ngroups,
-                                                           KS, pe, smem);
+        steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1), NS>(rr, gp(jb.wavenumber)[f], refph, col, part, TPC,
+                                                               ngroups, KS, pe, smem);
         __syncthreads();
-        gemm_frequency<BF16X3, CW, TT>(wave, lane, nK, NCG, prep_base + (int64_t)f * nUp * 8192, smem, pe, KS, BN,
-                                       s_q);
+        if constexpr (BF16X3 && PF)
+            gemm_frequency_pf<CW, TT>(wave, lane, nK, NCG, prep_base + (int64_t)f * nUp * 8192, smem, pe, KS, BN, s_q);
+        else
+            gemm_frequency<BF16X3, CW, TT>(wave, lane, nK, NCG, prep_base + (int64_t)f * nUp * 8192, smem, pe, KS,
+                                           BN, s_q);
         __syncthreads();
         store_frequency<kMfWaves>(jb, f, tile, BN, tid, NT, s_q, s_r0, s_help, 1);
         // the next frequency overwrites X only after every wave has passed this barrier
@@ -454,7 +568,7 @@ __global__ __launch_bounds__((kWsMf + kGenWaves) * 64) void bf_mfma_ws_kernel(co
         }
         const int ngroups = Mp / 8;
         double rr[kMfNS][8];
-        s_help[part * BN + col] = steer_distances(jb, valid, gx, gy, gz, part, TPC, ngroups, rr);
+        s_help[part * BN + col] = steer_distances<kMfNS>(jb, valid, gx, gy, gz, part, TPC, ngroups, rr);
         if (part == 0) {
             double r0 = jb.ref_dist;
             if (!(r0 > 0.0)) r0 = dist3(gx, gy, gz, jb.ref[0], jb.ref[1], jb.ref[2]);
@@ -462,13 +576,13 @@ __global__ __launch_bounds__((kWsMf + kGenWaves) * 64) void bf_mfma_ws_kernel(co
         }
         for (int idx = gt; idx < 2 * kWsMf * BN; idx += NG) s_q[idx] = 0.f;
         const bool refph = jb.phase_mode == ABF_PHASE_REFERENCE;
-        steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1)>(rr, gp(jb.wavenumber)[phys(f0)], refph, col, part, TPC,
+        steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1), kMfNS>(rr, gp(jb.wavenumber)[phys(f0)], refph, col, part, TPC,
                                                            ngroups, KS, pe, xb(0));
         lds_barrier();
         for (int f = f0; f < f1; ++f) {
             const int b = (f - f0) & 1;
             if (f + 1 < f1)
-                steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1)>(rr, gp(jb.wavenumber)[phys(f + 1)], refph, col, part,
+                steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1), kMfNS>(rr, gp(jb.wavenumber)[phys(f + 1)], refph, col, part,
                                                                    TPC, ngroups, KS, pe, xb(b ^ 1));
             if (f > f0)
                 store_frequency<kWsMf>(jb, phys(f - 1), tile, BN, gt, NG, s_q + (b ^ 1) * kWsMf * BN, s_r0, s_help, TPC);

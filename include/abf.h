@@ -65,8 +65,11 @@ enum abf_mfma_kernel {
     ABF_MFMA_KERNEL_AUTO = 0,
     ABF_MFMA_KERNEL_ALTERNATING = 1, /* all waves generate the steering matrix, then all run MFMA */
     ABF_MFMA_KERNEL_SPECIALISED = 2, /* steering waves beside MFMA waves, double-buffered */
-    ABF_MFMA_KERNEL_ROW_PAIRS = 3    /* alternating, each work unit covers both 16-row halves of a
+    ABF_MFMA_KERNEL_ROW_PAIRS = 3,   /* alternating, each work unit covers both 16-row halves of a
                                         32-row block (half the LDS reads per MFMA) */
+    ABF_MFMA_KERNEL_PIPELINED = 4,   /* alternating, X and U fragments of the next block loaded
+                                        before the MFMAs of the current one (split-bf16) */
+    ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS = 5 /* both of the above, one column tile per fragment */
 };
 
 /* Steering geometry shared by the grid entry points. r0 = ref_dist if ref_dist > 0, else

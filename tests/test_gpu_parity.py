@@ -342,10 +342,11 @@ def test_batch_equals_alone_bitwise(gpu):
 
 @pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f32'])
 @pytest.mark.parametrize('kernel,tile_cols,tile_freqs', [(1, 0, 0), (1, 16, 1), (1, 64, 3), (2, 0, 0), (2, 64, 2),
-                                                         (2, 32, 5), (3, 0, 0), (3, 16, 2), (3, 64, 4)])
+                                                         (2, 32, 5), (3, 0, 0), (3, 16, 2), (3, 64, 4), (4, 0, 0),
+                                                         (4, 64, 3), (5, 0, 0), (5, 32, 2)])
 @pytest.mark.parametrize('nm,steer', [(64, 'true level'), (100, 'classic'), (256, 'inverse'), (33, 'true location')])
 def test_mfma_kernel_variants_and_tiles(gpu, form, kernel, tile_cols, tile_freqs, nm, steer):
-    """The MFMA kernels (alternating, warp-specialised, alternating with row pairs) and explicit tile shapes, including
+    """Every MFMA kernel variant (alternating, warp-specialised, row pairs, pipelined) and explicit tile shapes, including
     ragged microphone counts and bands that do not divide into the frequency tile."""
     torch = _lib.torch()
     freqs = np.array([600.0, 2500.0, 7000.0, 12000.0, 15000.0, 21000.0, 3100.0])
