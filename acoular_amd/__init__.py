@@ -9,7 +9,18 @@ streams.
 
 from .environments import Environment, dist_mat
 from .fastFuncs import beamformerFreq, calcCSM, calcTransfer
-from .fbeamform import BeamformerBase, LazyBfResult, L_p, SteeringVector, beamform_device, integrate
+from .fbeamform import (
+    BeamformerBase,
+    BeamformerEig,
+    BeamformerFunctional,
+    BeamformerMusic,
+    LazyBfResult,
+    L_p,
+    SteeringVector,
+    beamform_device,
+    beamform_eig_device,
+    integrate,
+)
 from .grids import Grid, ImportGrid, RectGrid, RectGrid3D
 from .microphones import MicGeom
 from .sources import TimeSamples
@@ -18,6 +29,9 @@ from .spectra import BaseSpectra, PowerSpectra, PowerSpectraImport
 __all__ = [
     'BaseSpectra',
     'BeamformerBase',
+    'BeamformerEig',
+    'BeamformerFunctional',
+    'BeamformerMusic',
     'Environment',
     'Grid',
     'ImportGrid',
@@ -31,6 +45,7 @@ __all__ = [
     'SteeringVector',
     'TimeSamples',
     'beamform_device',
+    'beamform_eig_device',
     'beamformerFreq',
     'calcCSM',
     'calcTransfer',

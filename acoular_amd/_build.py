@@ -10,7 +10,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libabf.so')
 SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f32.hip', 'abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip',
-           'abf_api.hip']
+           'abf_eig.hip', 'abf_api.hip']
+# the batched Hermitian eigensolver of the eigenvalue path
+LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocsolver', '-lrocblas']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 # packed fp32 instructions are disabled on purpose (build flags)
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-Xclang', '-target-feature', '-Xclang',
@@ -32,7 +34,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=6) as ex:
         objs = list(ex.map(lambda s: _compile(s, build_dir), SOURCES))
     tmp = LIB + '.tmp'
-    cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp, *objs]
+    cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp, *objs, *LIBS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f'link failed:\n{r.stderr}')

@@ -8,6 +8,7 @@
 #include "abf_common.h"
 #include "abf_csm_kernels.h"
 #include "abf_kernels.h"
+#include "abf_status.h"
 #include "abf_steer_kernels.h"
 
 using namespace abf;
@@ -367,6 +368,8 @@ void launch_prep_mfma(const abf_grid_job *j, const MfPlan &p, void *prep, hipStr
 
 } // namespace
 
+int abf::set_error(int code, const char *msg) { return fail(code, msg); }
+
 extern "C" {
 
 const char *abf_last_error(void) { return g_err.c_str(); }
@@ -629,6 +632,60 @@ int abf_beamform_eig_custom(const double *eigval, const void *eigvec, int32_t nu
     a.r_diag = r_diag;
     hipLaunchKernelGGL(bf_eig_kernel, dim3((unsigned)cdiv(num_grid, 256)), dim3(256), 0, S(stream), a);
     return hip_status("eig custom beamformer launch");
+}
+
+int abf_beamform_eig_grid(const abf_geometry *geom, const double *wavenumber, int32_t num_freqs, const double *eigval,
+                          const void *eigvec, int32_t eig_first, int32_t num_eig, int32_t steer_type, int32_t r_diag,
+                          double norm_factor, int32_t phase_mode, int32_t clip_negative, float *out, float *out_norm,
+                          void *stream)
+{
+    if (!geom) return fail(ABF_ERR_INVALID, "null geometry");
+    if (int r = check_steer(steer_type)) return r;
+    if (int r = check_phase(phase_mode)) return r;
+    const int M = geom->num_mics;
+    const int64_t G = geom->num_grid;
+    if (M < 1 || G < 0 || num_freqs < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (eig_first < 0 || num_eig < 0 || (int64_t)eig_first + num_eig > M)
+        return fail(ABF_ERR_INVALID, "eigenvalue range outside [0, num_mics)");
+    if (num_freqs > 65535) return fail(ABF_ERR_INVALID, "at most 65535 frequencies per call");
+    if (G == 0 || num_freqs == 0) return ABF_OK;
+    if (!geom->gpos || !geom->mpos || !wavenumber || !out || !eigval || !eigvec)
+        return fail(ABF_ERR_INVALID, "null pointer");
+    // grid points per workgroup: the steering tile [M][P] stays within 64 KiB where it can
+    // (two or more workgroups per CU), within 144 KiB otherwise
+    int P = 64;
+    while (P > 4 && (size_t)M * P * sizeof(float2) > 65536) P /= 2;
+    const size_t lds = (size_t)M * P * sizeof(float2);
+    if (lds > 147456) return fail(ABF_ERR_UNSUPPORTED, "too many microphones for the eigenvalue form");
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)bf_eig_grid_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 147456);
+        attr = true;
+    }
+    EigGridArgs a{};
+    a.gpos = geom->gpos;
+    a.mpos = geom->mpos;
+    a.wavenumber = wavenumber;
+    a.eigval = eigval;
+    a.eigvec = reinterpret_cast<const float2 *>(eigvec);
+    a.out = out;
+    a.out_norm = out_norm;
+    for (int i = 0; i < 3; ++i) a.ref[i] = geom->ref[i];
+    a.ref_dist = geom->ref_dist;
+    a.norm = norm_factor;
+    a.G = G;
+    a.M = M;
+    a.e0 = eig_first;
+    a.K = num_eig;
+    a.P = P;
+    a.steer_type = steer_type;
+    a.r_diag = r_diag;
+    a.phase_mode = phase_mode & kPhaseMask;
+    a.fast_trig = (phase_mode & kFastTrigFlag) != 0;
+    a.clip = clip_negative;
+    hipLaunchKernelGGL(bf_eig_grid_kernel, dim3((unsigned)cdiv(G, P), (unsigned)num_freqs), dim3(256), lds, S(stream),
+                       a);
+    return hip_status("eigenvalue grid beamformer launch");
 }
 
 int abf_beamform_custom(const void *csm, const void *steer, int64_t num_grid, int32_t num_mics, int32_t r_diag,

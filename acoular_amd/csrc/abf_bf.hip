@@ -303,6 +303,87 @@ __global__ __launch_bounds__(256) void bf_eig_kernel(EigArgs a)
     if (a.out_norm) a.out_norm[g] = (float)nst;
 }
 
+// Eigenvalue form for every frequency of a job (the loop of BeamformerEig._calc,
+// fbeamform.py:852-888, around _freqBeamformer_EigValues, fastFuncs.py:335-411):
+//     q = sum_{e in [e0, e0+K)} l_e (|v_e^H h|^2 - [r_diag] sum_m |v_me|^2 |h_m|^2)
+// One workgroup per (P grid points, frequency).  The steering vectors of the P points for all
+// M microphones are generated once into LDS ([M][P] complex float32); thread t then takes grid
+// point t % P and eigenvectors e0 + t / P, e0 + t / P + 256 / P, ..., so every steering element
+// is reused by all K eigenvectors.  P (a power of two, 4..64) is chosen on the host so that the
+// LDS tile fits.
+__global__ __launch_bounds__(256) void bf_eig_grid_kernel(EigGridArgs a)
+{
+    extern __shared__ float2 hl[]; // [M][P]
+    __shared__ double s_red[256];
+    const int tid = threadIdx.x;
+    const int P = a.P;
+    const int p = tid & (P - 1);
+    const int slot = tid / P;
+    const int nslot = 256 / P;
+    const int f = blockIdx.y;
+    const int64_t g = (int64_t)blockIdx.x * P + p;
+    const bool valid = g < a.G;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (valid) {
+        gx = a.gpos[g];
+        gy = a.gpos[a.G + g];
+        gz = a.gpos[2 * a.G + g];
+    }
+    const double k = a.wavenumber[f];
+    double help = 0.0;
+    for (int m = slot; m < a.M; m += nslot) {
+        float2 v = make_float2(0.f, 0.f);
+        if (valid) {
+            const double r = dist3(gx, gy, gz, a.mpos[m], a.mpos[a.M + m], a.mpos[2 * a.M + m]);
+            float c, s, w;
+            double h;
+            phasor_rev(k * r, a.phase_mode, a.fast_trig, c, s);
+            steer_weight(a.steer_type, r, w, h);
+            help += h;
+            v = make_float2(w * c, -w * s);
+        }
+        hl[m * P + p] = v;
+    }
+    s_red[tid] = help;
+    __syncthreads();
+    double hs = 0.0;
+    if (tid < P)
+        for (int i = 0; i < nslot; ++i) hs += s_red[tid + P * i];
+    __syncthreads();
+
+    const float2 *__restrict__ V = a.eigvec + (int64_t)f * a.M * a.M;
+    const double *__restrict__ lam = a.eigval + (int64_t)f * a.M;
+    double q = 0.0;
+    for (int e = a.e0 + slot; e < a.e0 + a.K; e += nslot) {
+        float tr = 0.f, ti = 0.f, ds = 0.f;
+        for (int m = 0; m < a.M; ++m) {
+            const float2 ev = V[(int64_t)m * a.M + e]; // conj(V_me) * h_m
+            const float2 h = hl[m * P + p];
+            const float pr = ev.x * h.x + ev.y * h.y;
+            const float pi = ev.x * h.y - ev.y * h.x;
+            tr += pr;
+            ti += pi;
+            ds = fmaf(pr, pr, fmaf(pi, pi, ds));
+        }
+        double full = (double)tr * tr + (double)ti * ti;
+        if (a.r_diag) full -= ds;
+        q += full * lam[e];
+    }
+    s_red[tid] = q;
+    __syncthreads();
+    if (tid < P && valid) {
+        double qs = 0.0;
+        for (int i = 0; i < nslot; ++i) qs += s_red[tid + P * i];
+        const double r0 = a.ref_dist > 0.0 ? a.ref_dist : dist3(gx, gy, gz, a.ref[0], a.ref[1], a.ref[2]);
+        double res, nst;
+        normalise(a.steer_type, qs, r0, hs, (double)a.M, a.norm, res, nst);
+        float rf = (float)res;
+        if (a.clip && rf < 0.f) rf = 0.f;
+        a.out[(int64_t)f * a.G + g] = rf;
+        if (a.out_norm) a.out_norm[(int64_t)f * a.G + g] = (float)nst;
+    }
+}
+
 __global__ __launch_bounds__(256) void bf_custom_kernel(CustomArgs a)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

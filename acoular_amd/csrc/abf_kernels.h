@@ -52,6 +52,23 @@ struct EigArgs {
     int M, K, steer_type, r_diag, phase_mode, fast_trig;
 };
 
+// eigenvalue form over every frequency of a job, steering from coordinates
+struct EigGridArgs {
+    const double *gpos;        // [3][G]
+    const double *mpos;        // [3][M]
+    const double *wavenumber;  // [F]
+    const double *eigval;      // [F][M]
+    const float2 *eigvec;      // [F][M][M], element (m, e) = component m of eigenvector e
+    float *out;                // [F][G]
+    float *out_norm;           // [F][G] or null
+    double ref[3];
+    double ref_dist;
+    double norm;
+    int64_t G;
+    int M, e0, K, P;           // eigenvectors e0 .. e0+K-1; P grid points per workgroup
+    int steer_type, r_diag, phase_mode, fast_trig, clip;
+};
+
 struct CustomArgs {
     const float2 *csm;         // [M][M]
     const float2 *steer;       // [G][M]
@@ -71,6 +88,7 @@ template <bool BF16X3, int CW, int VAR, int TT, bool PF>
 __global__ void bf_mfma_kernel(const MfJob *jobs, int njobs);
 template <bool BF16X3, int VAR> __global__ void bf_mfma_ws_kernel(const MfJob *jobs, int njobs);
 __global__ void bf_eig_kernel(EigArgs a);
+__global__ void bf_eig_grid_kernel(EigGridArgs a);
 __global__ void bf_custom_kernel(CustomArgs a);
 
 } // namespace abf

@@ -137,12 +137,14 @@ class BeamformerBase(HasParams):
     """
 
     _params = ('freq_data', 'steer', 'r_diag', 'r_diag_norm', 'precision', 'form', 'phase_mode')
+    #: parameters a subclass pins to one value (the reference declares them ``Enum(value)``)
+    _fixed: dict = {}
 
     def __init__(self, **kwargs):
         self.freq_data = None
         self.steer = SteeringVector()
-        self.r_diag = True
-        self.r_diag_norm = 0.0
+        self.r_diag = self._fixed.get('r_diag', True)
+        self.r_diag_norm = self._fixed.get('r_diag_norm', 0.0)
         self.precision = 'float64'
         self.cached = True  # accepted for API compatibility (no HDF5 cache in this build)
         self.form = config['form']
@@ -153,6 +155,9 @@ class BeamformerBase(HasParams):
         object.__setattr__(self, '_result_digest', None)
 
     def __setattr__(self, name, value):
+        if name in self._fixed and value != self._fixed[name]:
+            msg = f'{type(self).__name__}.{name} can only be {self._fixed[name]!r}'
+            raise ValueError(msg)
         if name == 'precision' and value not in ('float64', 'float32'):
             raise ValueError("precision must be 'float64' or 'float32'")
         if name == 'form' and value not in _lib.FORMS:
@@ -298,6 +303,184 @@ class BeamformerBase(HasParams):
         return h
 
 
+class BeamformerEig(BeamformerBase):
+    """Beamforming with one eigenvalue/eigenvector pair of the CSM (``fbeamform.py:821-888``):
+    ``n`` selects the pair (negative values count from the largest eigenvalue, default -1).
+
+    All requested bins run in one ``abf_beamform_eig_grid`` launch over the resident
+    eigendecomposition (``PowerSpectra.eig_device``).
+    """
+
+    _params = BeamformerBase._params + ('n',)
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'n', -1)
+        super().__init__(**kwargs)
+
+    @property
+    def na(self):
+        """Index of the eigenvalue taken into account (``fbeamform.py:845-851``)."""
+        na = int(self.n)
+        nm = self.steer.mics.num_mics
+        if na < 0:
+            na = max(nm + na, 0)
+        return min(nm - 1, na)
+
+    def _eig_maps(self, ind, eig_first, num_eig, norm, eigvals=None, out_norm=False, clip=False):
+        """Eigenvalue-form maps of the bins ``ind`` for eigenvectors
+        ``[eig_first, eig_first + num_eig)`` as a float32 device tensor ``[len(ind), G]``."""
+        ev, evec = self.freq_data.eig_device()
+        if eigvals is not None:
+            ev = eigvals
+        return beamform_eig_device(ev, evec, self._f[ind], ind, self.steer, self.steer.steer_type, self.r_diag, norm,
+                                   eig_first, num_eig, self.phase_mode, clip=clip, out_norm=out_norm)
+
+    def _calc_custom(self, ind, eig_slice, norm, post=None):
+        # explicit steering vectors: the reference's per-bin loop over beamformerFreq('custom', ...)
+        _, steer_func = self._beamformer_params()
+        for i in ind:
+            eigvals = np.array(self.freq_data.eigvals[i], dtype='float64')
+            eigvecs = np.array(self.freq_data.eigvecs[i], dtype='complex128')
+            out = beamformerFreq('custom', self.r_diag, norm, steer_func(self._f[i]),
+                                 (eigvals[eig_slice], eigvecs[:, eig_slice]))[0]
+            self._ac[i] = post(out) if post else out
+            self._fr[i] = 1
+
+    def _calc(self, ind):
+        ind = np.asarray(ind, dtype=np.int64)
+        na = int(self.na)
+        normfactor = self.sig_loss_norm()
+        if type(self.steer) is not SteeringVector:
+
+            def clip(out):
+                if self.r_diag:
+                    out[np.sign(out) < 0] = 0
+                return out
+
+            self._calc_custom(ind, slice(na, na + 1), normfactor, clip)
+            return
+        maps = self._eig_maps(ind, na, 1, normfactor, clip=self.r_diag)
+        self._ac[ind] = maps.cpu().numpy().astype(self.precision)
+        self._fr[ind] = 1
+
+
+class BeamformerMusic(BeamformerEig):
+    """MUSIC beamforming (``fbeamform.py:891-945``): the eigenvalue form over the ``M - na``
+    smallest eigenpairs (the noise subspace for ``n`` assumed sources), mapped to
+    ``4e-10 * min(B) / B`` per bin.  The CSM diagonal is always kept."""
+
+    _fixed = {'r_diag': False, 'r_diag_norm': 1.0}
+
+    def __init__(self, **kwargs):
+        super().__init__(**kwargs)
+        if 'n' not in kwargs:
+            object.__setattr__(self, 'n', 1)  # assumed number of sources
+
+    def _calc(self, ind):
+        ind = np.asarray(ind, dtype=np.int64)
+        nmics = self.freq_data.num_channels
+        n = int(self.steer.mics.num_mics - self.na)
+        normfactor = self.sig_loss_norm() * nmics**2
+        if type(self.steer) is not SteeringVector:
+            self._calc_custom(ind, slice(0, n), normfactor, lambda out: 4e-10 * out.min() / out)
+            return
+        maps = self._eig_maps(ind, 0, n, normfactor).double()
+        maps = 4e-10 * maps.amin(dim=1, keepdim=True) / maps
+        self._ac[ind] = maps.cpu().numpy().astype(self.precision)
+        self._fr[ind] = 1
+
+
+class BeamformerFunctional(BeamformerBase):
+    """Functional beamforming (``fbeamform.py:688-774``): the eigenvalue form with eigenvalues
+    raised to ``1/gamma``, normalised by the steering norm and raised back to ``gamma``.
+    Only defined for the full CSM (``r_diag`` is always False)."""
+
+    _params = BeamformerBase._params + ('gamma',)
+    _fixed = {'r_diag': False, 'r_diag_norm': 1.0}
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'gamma', 1.0)
+        super().__init__(**kwargs)
+
+    def _calc(self, ind):
+        ind = np.asarray(ind, dtype=np.int64)
+        normfactor = self.sig_loss_norm()
+        gamma = float(self.gamma)
+        nm = self.freq_data.num_channels
+        if type(self.steer) is not SteeringVector:
+            _, steer_func = self._beamformer_params()
+            for i in ind:
+                eigvals = np.array(self.freq_data.eigvals[i], dtype='float64') ** (1.0 / gamma)
+                eigvecs = np.array(self.freq_data.eigvecs[i], dtype='complex128')
+                bo, sn = beamformerFreq('custom', self.r_diag, 1.0, steer_func(self._f[i]), (eigvals, eigvecs))
+                bo = bo / sn
+                self._ac[i] = (bo**gamma) * sn * normfactor
+                self._fr[i] = 1
+            return
+        ev, evec = self.freq_data.eig_device()
+        ev = ev ** (1.0 / gamma)
+        bo, sn = beamform_eig_device(ev, evec, self._f[ind], ind, self.steer, self.steer.steer_type, False, 1.0, 0, nm,
+                                     self.phase_mode, out_norm=True)
+        bo, sn = bo.double(), sn.double()
+        res = ((bo / sn) ** gamma) * sn * normfactor
+        self._ac[ind] = res.cpu().numpy().astype(self.precision)
+        self._fr[ind] = 1
+
+
+def _geometry(gpos, mpos, ref) -> _lib.AbfGeometry:
+    """``abf_geometry`` of device coordinate tensors and a reference point or distance."""
+    g = _lib.AbfGeometry()
+    g.gpos = gpos.data_ptr()
+    g.mpos = mpos.data_ptr()
+    g.num_grid = gpos.shape[1]
+    g.num_mics = mpos.shape[1]
+    if np.isscalar(ref):
+        g.ref_dist = float(ref) if ref > 0 else 0.0
+        g.ref[:] = [0.0, 0.0, 0.0]
+    else:
+        g.ref[:] = [float(x) for x in ref]
+        g.ref_dist = 0.0
+    return g
+
+
+def beamform_eig_device(eigvals, eigvecs, freqs, ind, steer: SteeringVector, steer_type: str, r_diag: bool,
+                        norm: float, eig_first: int, num_eig: int, phase_mode: str = 'reference', clip: bool = False,
+                        out_norm: bool = False, fast_trig: bool | None = None):
+    """One ``abf_beamform_eig_grid`` call: eigenvalue-form maps of the bins ``ind`` (rows of the
+    device eigendecomposition ``eigvals [F, M]`` float64, ``eigvecs [F, M, M]`` complex64) at the
+    frequencies ``freqs``.  Returns a float32 device tensor ``[len(ind), G]`` (and the steer
+    normalisation if asked)."""
+    torch = _lib.torch()
+    ind = np.asarray(ind, dtype=np.int64)
+    freqs = np.asarray(freqs, dtype=np.float64)
+    dev = _lib.device()
+    if eigvals.dtype != torch.float64 or eigvecs.dtype != torch.complex64:
+        raise TypeError('eigvals must be float64 and eigvecs complex64 device tensors')
+    if len(ind) != eigvals.shape[0] or not np.array_equal(ind, np.arange(eigvals.shape[0])):
+        sel = torch.from_numpy(ind).to(dev)
+        eigvals, eigvecs = eigvals.index_select(0, sel), eigvecs.index_select(0, sel)
+    eigvals, eigvecs = eigvals.contiguous(), eigvecs.contiguous()
+    gpos, mpos = steer.device_geometry()
+    G, M, F = gpos.shape[1], mpos.shape[1], len(ind)
+    if eigvecs.shape[1:] != (M, M) or eigvals.shape[1] != M:
+        msg = f'eigendecomposition of shape {tuple(eigvecs.shape[1:])} does not fit {M:d} mics'
+        raise ValueError(msg)
+    if fast_trig is None:
+        fast_trig = config['fast_trig']
+    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    out = torch.empty((F, G), dtype=torch.float32, device=dev)
+    onorm = torch.empty((F, G), dtype=torch.float32, device=dev) if out_norm else None
+    geom = _geometry(gpos, mpos, steer.ref)
+    pm = _lib.PHASE_MODES[phase_mode] | (_lib.FAST_TRIG if fast_trig else 0)
+    _lib.check(_lib.lib().abf_beamform_eig_grid(geom, _lib.ptr(k), F, _lib.ptr(eigvals), _lib.ptr(eigvecs),
+                                                int(eig_first), int(num_eig), _lib.STEER_CODES[steer_type],
+                                                int(bool(r_diag)), float(norm), pm, int(bool(clip)), _lib.ptr(out),
+                                                _lib.ptr(onorm), _lib.stream_handle()), 'eigenvalue beamform')
+    if out_norm:
+        return out, onorm
+    return out
+
+
 def beamform_device(csm_device, freqs, ind, steer: SteeringVector, steer_type: str, r_diag: bool, norm: float,
                     form: str = 'mfma_bf16x3', phase_mode: str = 'reference', clip: bool = False,
                     out_norm: bool = False, fast_trig: bool | None = None):
@@ -340,19 +523,7 @@ def make_job(csm, k, gpos, mpos, ref, steer_type, r_diag, norm, form, phase_mode
     if fast_trig is None:
         fast_trig = config['fast_trig']
     j = _lib.AbfGridJob()
-    j.geom.gpos = gpos.data_ptr()
-    j.geom.mpos = mpos.data_ptr()
-    j.geom.num_grid = gpos.shape[1]
-    j.geom.num_mics = mpos.shape[1]
-    if np.isscalar(ref):
-        if ref > 0:
-            j.geom.ref_dist = float(ref)
-        else:
-            j.geom.ref_dist = 0.0
-            j.geom.ref[:] = [0.0, 0.0, 0.0]
-    else:
-        j.geom.ref[:] = [float(x) for x in ref]
-        j.geom.ref_dist = 0.0
+    j.geom = _geometry(gpos, mpos, ref)
     j.csm = csm.data_ptr()
     j.wavenumber = k.data_ptr()
     j.num_freqs = csm.shape[0]

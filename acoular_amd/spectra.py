@@ -193,10 +193,45 @@ class PowerSpectra(BaseSpectra):
     def csm(self):
         return self._cached('csm', lambda: self.csm_device.cpu().numpy().astype(self.precision))
 
-    def _evd(self):
+    def _eig_input(self):
+        # the reference decomposes the CSM in its precision (spectra.py:356-362)
         torch = _lib.torch()
-        c = self.csm_device.to(torch.complex128)
-        ev, evec = torch.linalg.eigh(c)
+        return self.csm_device.to(torch.complex128 if self.precision == 'complex128' else torch.complex64)
+
+    def _evd_native(self):
+        # one batched abf_csm_eig call over every slice, in the precision of _eig_input
+        torch = _lib.torch()
+        c = self._eig_input().contiguous()
+        nf, nm = int(c.shape[0]), int(c.shape[1])
+        is128 = int(c.dtype == torch.complex128)
+        ev = torch.empty((nf, nm), dtype=torch.float64, device=c.device)
+        evec = torch.empty_like(c)
+        info = torch.zeros(nf, dtype=torch.int32, device=c.device)
+        lib = _lib.lib()
+        ws = _lib.workspace(lib.abf_csm_eig_workspace_size(nf, nm, is128))
+        _lib.check(lib.abf_csm_eig(_lib.ptr(c), nf, nm, is128, _lib.ptr(ev), _lib.ptr(evec), _lib.ptr(info),
+                                   _lib.ptr(ws), ws.numel(), _lib.stream_handle()), 'CSM eigendecomposition')
+        bad = torch.nonzero(info).flatten().cpu().numpy()
+        if bad.size:
+            msg = f'eigendecomposition did not converge for frequency indices {bad.tolist()}'
+            raise np.linalg.LinAlgError(msg)
+        return ev, evec
+
+    def eig_device(self):
+        """Eigendecomposition of every CSM slice, resident on the device (``spectra.py:351-364``):
+        eigenvalues float64 ``[F, M]`` in ascending order and eigenvectors complex64 ``[F, M, M]``
+        with eigenvector ``e`` in column ``e`` (the layout of ``numpy.linalg.eigh``), computed by
+        one batched ``abf_csm_eig`` call in the CSM's precision."""
+        torch = _lib.torch()
+
+        def calc():
+            ev, evec = self._cached('evd_native', self._evd_native)
+            return ev, evec.to(torch.complex64).contiguous()
+
+        return self._cached('evd_device', calc)
+
+    def _evd(self):
+        ev, evec = self._cached('evd_native', self._evd_native)
         rdt = 'float64' if self.precision == 'complex128' else 'float32'
         return ev.cpu().numpy().astype(rdt), evec.cpu().numpy().astype(self.precision)
 
@@ -223,11 +258,13 @@ class PowerSpectra(BaseSpectra):
 class PowerSpectraImport(PowerSpectra):
     """Injects a precomputed CSM and its frequencies (``spectra.py:591-697``)."""
 
-    _params = ('csm_array', 'frequencies')
+    _params = ('csm_array', 'frequencies', 'precision')
 
     def __init__(self, csm=None, frequencies=None, **kwargs):
         object.__setattr__(self, 'csm_array', None)
         object.__setattr__(self, '_csm_dev', None)
+        # inherited trait (spectra.py:119): the dtype of the host eigendecomposition
+        object.__setattr__(self, 'precision', 'complex128')
         self.frequencies = frequencies
         HasParams.__init__(self, **kwargs)
         self.cached = False
@@ -243,11 +280,13 @@ class PowerSpectraImport(PowerSpectra):
     def __setattr__(self, name, value):
         if name in ('window', 'overlap', 'block_size') and value is not None:
             return  # non-functional in this class, as in the reference
+        if name == 'precision' and value not in PRECISIONS:
+            raise ValueError(f'precision must be one of {PRECISIONS}')
         object.__setattr__(self, name, value)
 
     def _digest_items(self):
         return (id(self.csm_array), self.frequencies if not isinstance(self.frequencies, np.ndarray) else
-                tuple(np.ravel(self.frequencies)))
+                tuple(np.ravel(self.frequencies)), self.precision)
 
     @property
     def csm(self):
@@ -281,6 +320,16 @@ class PowerSpectraImport(PowerSpectra):
         if dev.dtype != torch.complex64:
             dev = dev.to(torch.complex64)
         return dev
+
+    def _eig_input(self):
+        # np.linalg.eigh of the imported array runs in that array's own precision
+        torch = _lib.torch()
+        if self._csm_dev is not None:
+            dt = torch.complex128 if self._csm_dev.dtype == torch.complex128 else torch.complex64
+            return self._csm_dev.to(dt)
+        host = np.asarray(self.csm_array)
+        dt = torch.complex128 if host.dtype in (np.complex128, np.float64) else torch.complex64
+        return _lib.to_device(host, dt)
 
     @property
     def num_channels(self):

@@ -425,6 +425,96 @@ def large3d_bench(args):
     }
 
 
+def eig_bench(args):
+    """Eigenvalue path on the 256-mic 201 x 201 513-bin workload (classic, r_diag): the batched
+    eigendecomposition of the resident CSM (PowerSpectra.eig_device) and the all-bins
+    eigenvalue-form beamformer (abf_beamform_eig_grid) for BeamformerEig (the largest eigenpair)
+    and BeamformerMusic (the M - 1 smallest)."""
+    import torch
+
+    import acoular_amd as aa
+    from acoular_amd.fbeamform import beamform_eig_device
+    from oracle import beamform as B
+
+    dev = torch.device('cuda', 0)
+    wl = build_workload(dev)
+    nm, freqs, gpos = wl['nm'], wl['freqs'], wl['gpos']
+    ps = aa.PowerSpectraImport(csm=wl['csm'], frequencies=freqs, precision='complex64')
+    grid = aa.ImportGrid(pos=gpos)
+    sv = aa.SteeringVector(grid=grid, mics=aa.MicGeom(pos_total=wl['mpos']), steer_type='classic')
+    norm = nm / (nm - 1.0)
+
+    def eig_step():
+        ps._cache.clear()
+        return ps.eig_device()
+
+    t_eig = time_steps(eig_step, max(1, args.steps // 5), 1)
+    log(f'eigh of {len(freqs)} x {nm} x {nm} complex64 on the device: {t_eig * 1e3:.1f} ms')
+    ev, evec = ps.eig_device()
+    ind = np.arange(len(freqs))
+    res = {}
+    outs = {}
+    for name, e0, k in (('eig', nm - 1, 1), ('music', 0, nm - 1)):
+        box = {}
+
+        def step(e0=e0, k=k, box=box):
+            box['out'] = beamform_eig_device(ev, evec, freqs, ind, sv, 'classic', name == 'eig', norm, e0, k,
+                                             clip=name == 'eig')
+
+        step()
+        t = event_time(step, max(1, args.steps if name == 'eig' else args.steps // 5))
+        bins = len(freqs) * gpos.shape[1]
+        res[name] = {'eig_first': e0, 'num_eig': k, 'ms_per_step': t * 1e3, 'map_bins_per_s': bins / t,
+                     'alg_tflops': bins * 8.0 * nm * k / t / 1e12}
+        outs[name] = box['out']
+        log(f"{name} (eigenvectors {e0}..{e0 + k - 1}): {t * 1e3:.2f} ms -> {bins / t:.4g} map-bins/s")
+    err = None
+    sample_bins = [40, 300]
+    if not args.no_compare:
+        r0 = B.steer_r0(gpos, np.zeros(3))
+        rm = B.dist_mat(gpos, wl['mpos'])
+        errs = []
+        for b in sample_bins:
+            lam, V = np.linalg.eigh(wl['csm'][b].cpu().numpy().astype(np.complex128))
+            ref = B.freq_beamformer_eig(lam[-1:], V[:, -1:], r0, rm, 2 * np.pi * freqs[b] / 343.0, norm, True,
+                                        'classic')[0]
+            ref = np.where(ref < 0, 0.0, ref)
+            got = outs['eig'][b].double().cpu().numpy()
+            errs.append(float(np.abs(got - ref).max() / np.abs(ref).max()))
+        err = max(errs)
+        log(f'BeamformerEig max relative map error on bins {sample_bins}: {err:.2e}')
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's eigenvalue form (numpy, one thread) on whole maps of the same workload
+        r0 = B.steer_r0(gpos, np.zeros(3))
+        rm = B.dist_mat(gpos, wl['mpos'])
+        evh, evech = ev.cpu().numpy(), evec.cpu().numpy().astype(np.complex128)
+        done, t0 = 0, time.perf_counter()
+        for b in range(7, len(freqs), 37):
+            B.freq_beamformer_eig(evh[b, -1:], evech[b][:, -1:], r0, rm, 2 * np.pi * freqs[b] / 343.0, norm, True,
+                                  'classic')
+            done += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {'value': gpos.shape[1] * done / dt, 'unit': 'map-bins/s', 'cores': 1, 'kind': 'port',
+               'sample': f'{done} full 201x201 maps of the eigenvalue form (largest eigenpair), oracle numpy, '
+                         f'{dt:.1f} s'}
+        log(f"CPU oracle: {cpu['value']:.4g} map-bins/s")
+    m = res['eig']
+    return {
+        'metric': 'map-bins/s (grid pts x freqs) at <=1e-4 rel err vs FP64', 'value': m['map_bins_per_s'],
+        'unit': 'map-bins/s', 'n_gpus': 1, 'ms_per_step': m['ms_per_step'], 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded analytic CSM)',
+        'config': {'workload': 'eigenvalue path: 256 mics (2 m disk), 201x201 grid, 513 bins, 8 sources, classic, '
+                               'r_diag; BeamformerEig (largest eigenpair) and BeamformerMusic (M - 1 smallest)',
+                   'num_mics': nm, 'num_grid': int(gpos.shape[1]), 'num_freqs': len(freqs),
+                   'parallelism': 'single GPU'},
+        'eigh_ms': t_eig * 1e3, 'forms': res, 'max_rel_err': err, 'max_rel_err_bins': sample_bins,
+        'cpu_baseline': cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -436,7 +526,7 @@ def main():
     ap.add_argument('--no-tune', action='store_true', help='default launch shapes (no autotuner table, no tuning)')
     ap.add_argument('--tune', action='store_true', help='tune the launch shape now even if the table has it')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
-    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d'],
+    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
                          "'serving': the mixed-size request loop (Poisson arrivals) against the unbatched "
                          "plain kernel; 'fused': Welch CSM + beamforming of 512 mics, 10 s at 51.2 kHz, "
@@ -472,6 +562,9 @@ def main():
         return
     if args.mode == 'large3d':
         print(json.dumps(large3d_bench(args)), flush=True)
+        return
+    if args.mode == 'eig':
+        print(json.dumps(eig_bench(args)), flush=True)
         return
     log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
     wl = build_workload(dev)

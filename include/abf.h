@@ -149,6 +149,28 @@ int abf_beamform_eig(const double *eigval, const void *eigvec, int32_t num_eig, 
                      size_t workspace_bytes, void *stream);
 size_t abf_beamform_eig_workspace_size(int32_t num_mics, int32_t num_eig);
 
+/* The eigenvalue form for every frequency of a resident eigendecomposition, steering vectors
+ * generated from coordinates: the loops of BeamformerEig._calc (fbeamform.py:852-888) and
+ * BeamformerMusic._calc (fbeamform.py:911-945) / BeamformerFunctional._calc (fbeamform.py:712-774)
+ * around _freqBeamformer_EigValues (fastFuncs.py:335-411).  eigval [F][M] (double), eigvec
+ * [F][M][M] complex64 with eigenvector e in column e (numpy.linalg.eigh layout of
+ * PowerSpectra.eigvecs, spectra.py:351-364); eigenvectors eig_first .. eig_first+num_eig-1 are
+ * summed.  out/out_norm [F][G]; negative results are set to 0 when clip_negative. */
+int abf_beamform_eig_grid(const abf_geometry *geom, const double *wavenumber, int32_t num_freqs, const double *eigval,
+                          const void *eigvec, int32_t eig_first, int32_t num_eig, int32_t steer_type, int32_t r_diag,
+                          double norm_factor, int32_t phase_mode, int32_t clip_negative, float *out, float *out_norm,
+                          void *stream);
+
+/* Eigendecomposition of every CSM slice (PowerSpectra._get__evd, spectra.py:351-364, which calls
+ * numpy.linalg.eigh per frequency): csm [F][M][M] complex64 (is_c128 = 0) or complex128, read
+ * through its lower triangle as numpy does; eigval [F][M] double in ascending order; eigvec
+ * [F][M][M] of the CSM's type with eigenvector e in column e (must not overlap csm); info [F]
+ * (may be NULL) receives the solver's per-slice status, 0 on success.  One batched call of the
+ * rocSOLVER divide-and-conquer solver, whose scratch memory is carved from the workspace. */
+size_t abf_csm_eig_workspace_size(int32_t num_freqs, int32_t num_mics, int32_t is_c128);
+int abf_csm_eig(const void *csm, int32_t num_freqs, int32_t num_mics, int32_t is_c128, double *eigval, void *eigvec,
+                int32_t *info, void *workspace, size_t workspace_bytes, void *stream);
+
 /* _freqBeamformer_SpecificSteerVec_FullCSM / _CsmRemovedDiag (fastFuncs.py:414-476): explicit
  * steering vectors steer [G][M] complex64. out = q * norm, out_norm = sum |h|^2. */
 int abf_beamform_custom(const void *csm, const void *steer, int64_t num_grid, int32_t num_mics, int32_t r_diag,

@@ -1,0 +1,56 @@
+"""Host logic of the eigenvalue beamformers (no GPU): parameters the reference pins with
+single-value ``Enum`` traits, the eigenvalue index ``na`` (fbeamform.py:845-851) and the
+digests that make results recompute when ``n`` or ``gamma`` change."""
+
+import numpy as np
+import pytest
+
+import acoular_amd as aa
+
+
+def _steer(nm=8):
+    pos = np.zeros((3, nm))
+    pos[0] = np.arange(nm) * 0.1
+    grid = aa.RectGrid(x_min=-0.1, x_max=0.1, y_min=-0.1, y_max=0.1, z=0.5, increment=0.1)
+    return aa.SteeringVector(grid=grid, mics=aa.MicGeom(pos_total=pos))
+
+
+@pytest.mark.parametrize(('n', 'na'), [(-1, 7), (-8, 0), (-20, 0), (0, 0), (3, 3), (7, 7), (12, 7)])
+def test_eig_index(n, na):
+    assert aa.BeamformerEig(steer=_steer(), n=n).na == na
+
+
+def test_defaults_and_fixed_parameters():
+    eig = aa.BeamformerEig(steer=_steer())
+    assert eig.n == -1 and eig.r_diag is True
+    music = aa.BeamformerMusic(steer=_steer())
+    assert music.n == 1 and music.r_diag is False and music.r_diag_norm == 1.0 and music.na == 1
+    assert aa.BeamformerMusic(steer=_steer(), n=3).n == 3
+    func = aa.BeamformerFunctional(steer=_steer())
+    assert func.gamma == 1.0 and func.r_diag is False and func.r_diag_norm == 1.0
+    for cls in (aa.BeamformerMusic, aa.BeamformerFunctional):
+        with pytest.raises(ValueError):
+            cls(steer=_steer(), r_diag=True)
+        with pytest.raises(ValueError):
+            cls(steer=_steer(), r_diag_norm=0.0)
+        bf = cls(steer=_steer())
+        with pytest.raises(ValueError):
+            bf.r_diag = True
+    with pytest.raises(TypeError):
+        aa.BeamformerEig(steer=_steer(), gamma=2.0)
+
+
+def test_digest_follows_eig_parameters():
+    a, b = aa.BeamformerEig(steer=_steer(), n=-1), aa.BeamformerEig(steer=_steer(), n=-2)
+    assert a.digest != b.digest
+    f1, f2 = aa.BeamformerFunctional(steer=_steer()), aa.BeamformerFunctional(steer=_steer(), gamma=3.0)
+    assert f1.digest != f2.digest
+    assert aa.BeamformerMusic(steer=_steer()).digest != aa.BeamformerEig(steer=_steer(), n=1).digest
+
+
+def test_import_precision_parameter():
+    csm = np.zeros((1, 2, 2), complex)
+    assert aa.PowerSpectraImport(csm=csm, frequencies=1.0).precision == 'complex128'
+    assert aa.PowerSpectraImport(csm=csm, frequencies=1.0, precision='complex64').precision == 'complex64'
+    with pytest.raises(ValueError):
+        aa.PowerSpectraImport(csm=csm, frequencies=1.0, precision='float32')

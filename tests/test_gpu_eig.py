@@ -1,0 +1,206 @@
+"""GPU parity of the eigenvalue path: the resident eigendecomposition of the CSM
+(``PowerSpectra.eig_device``, spectra.py:351-364) and the all-bins eigenvalue-form beamformer
+(``abf_beamform_eig_grid``) behind ``BeamformerEig`` (fbeamform.py:821-888), ``BeamformerMusic``
+(fbeamform.py:891-945) and ``BeamformerFunctional`` (fbeamform.py:688-774), against the oracle's
+``freq_beamformer_eig`` (fastFuncs.py:335-411) on numpy's eigendecomposition of the same CSM.
+
+Tolerance: peak-normalised map error <= 1e-4 (float32 steering and sums against float64);
+steer normalisation to 1e-5 relative.  The CSMs are signal plus a Wishart noise matrix, so
+every eigenvalue is simple and the eigenvectors are fixed up to a phase, which the eigenvalue
+form does not see.
+"""
+
+import numpy as np
+import pytest
+
+import acoular_amd as aa
+from acoular_amd import _lib, synthetic
+from acoular_amd.fbeamform import beamform_eig_device
+from oracle import beamform as B
+from tests.conftest import peak_rel_err
+
+pytestmark = pytest.mark.gpu
+
+STEERS = ['classic', 'inverse', 'true level', 'true location']
+TOL = 1e-4
+
+
+def _noisy_scene(nm, freqs, seed=5, grid_inc=0.04):
+    mpos = synthetic.disk_array(nm, 1.0, seed)
+    spos = synthetic.source_positions(3, [-0.2, -0.2, 0.5], [0.2, 0.2, 0.5], seed + 1)
+    p = synthetic.level_powers(3, 10.0, seed + 2)
+    csm = synthetic.analytic_csm(mpos, spos, p, freqs, dtype=np.complex128)
+    rng = np.random.default_rng(seed + 3)
+    for f in range(len(freqs)):
+        a = rng.standard_normal((nm, 2 * nm)) + 1j * rng.standard_normal((nm, 2 * nm))
+        noise = a @ a.conj().T / (2 * nm)
+        csm[f] += 0.05 * np.real(np.trace(csm[f])) / nm * noise
+    grid = aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.3, y_max=0.3, z=0.5, increment=grid_inc)
+    return aa.MicGeom(pos_total=mpos), grid, csm
+
+
+def _oracle_eig_maps(csm, freqs, mics, grid, steer, r_diag, norm, sel):
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    maps, norms = [], []
+    for f, fr in enumerate(freqs):
+        lam, V = np.linalg.eigh(csm[f])
+        k = 2 * np.pi * fr / 343.0
+        q, n = B.freq_beamformer_eig(lam[sel], V[:, sel], r0, rm, k, norm, r_diag, steer)
+        maps.append(q)
+        norms.append(n)
+    return np.array(maps), np.array(norms)
+
+
+@pytest.mark.parametrize('nm', [24, 300, 1100])
+@pytest.mark.parametrize('steer', STEERS)
+@pytest.mark.parametrize('r_diag', [False, True])
+def test_eig_grid_kernel_matches_oracle(gpu, nm, steer, r_diag):
+    """All bins in one launch, for the eigenvector ranges BeamformerEig (one pair), Music (all
+    but the largest) and Functional (all) use, and an interior range; 24, 300 and 1100 mics
+    take the 64-, 16- and 4-point workgroup tiles."""
+    torch = _lib.torch()
+    freqs = np.array([1500.0, 7000.0])
+    mics, grid, csm = _noisy_scene(nm, freqs, grid_inc=0.04 if nm < 1000 else 0.1)
+    lam, V = np.linalg.eigh(csm)
+    ev = torch.from_numpy(lam).to(gpu)
+    evec = torch.from_numpy(V.astype(np.complex64)).to(gpu)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer)
+    norm = 1.25
+    for e0, K in [(nm - 1, 1), (0, nm - 1), (0, nm), (3, 5)]:
+        sel = slice(e0, e0 + K)
+        got, gnorm = beamform_eig_device(ev, evec, freqs, np.arange(2), sv, steer, r_diag, norm, e0, K,
+                                         out_norm=True)
+        want, wnorm = _oracle_eig_maps(csm, freqs, mics, grid, steer, r_diag, norm, sel)
+        err = peak_rel_err(got.cpu().numpy(), want)
+        assert err <= TOL, (e0, K, err)
+        np.testing.assert_allclose(gnorm.cpu().numpy(), wnorm, rtol=1e-5)
+
+
+def test_eig_grid_rejects_bad_ranges(gpu):
+    torch = _lib.torch()
+    freqs = np.array([2000.0])
+    mics, grid, csm = _noisy_scene(8, freqs)
+    lam, V = np.linalg.eigh(csm)
+    ev = torch.from_numpy(lam).to(gpu)
+    evec = torch.from_numpy(V.astype(np.complex64)).to(gpu)
+    sv = aa.SteeringVector(grid=grid, mics=mics)
+    with pytest.raises(ValueError):
+        beamform_eig_device(ev, evec, freqs, [0], sv, 'true level', False, 1.0, 6, 3)
+    with pytest.raises(TypeError):
+        beamform_eig_device(ev.float(), evec, freqs, [0], sv, 'true level', False, 1.0, 0, 1)
+
+
+def test_eig_device_matches_numpy(gpu):
+    """Eigenvalues of the batched device eigensolver against numpy.linalg.eigh; eigenvectors
+    through the reconstruction V diag(l) V^H = C and unitarity."""
+    freqs = np.array([1000.0, 3000.0, 9000.0])
+    mics, _, csm = _noisy_scene(40, freqs)
+    ps = aa.PowerSpectraImport(csm=csm, frequencies=freqs)
+    ev, evec = ps.eig_device()
+    lam = np.linalg.eigvalsh(csm)
+    np.testing.assert_allclose(ev.cpu().numpy(), lam, rtol=0, atol=1e-10 * np.abs(lam).max())
+    V = evec.cpu().numpy().astype(np.complex128)
+    rec = np.einsum('fme,fe,fne->fmn', V, lam, V.conj())
+    assert np.abs(rec - csm).max() <= 1e-5 * np.abs(csm).max()
+    eye = np.einsum('fme,fmd->fed', V.conj(), V)
+    assert np.abs(eye - np.eye(40)).max() <= 1e-5
+    # host copies keep the precision of the decomposition
+    assert ps.eigvals.dtype == np.float64 and ps.eigvecs.dtype == np.complex128
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+def test_csm_eig_reads_the_lower_triangle(gpu, dtype):
+    """abf_csm_eig reads the row-major lower triangle as numpy.linalg.eigh does by default:
+    garbage in the strict upper triangle changes nothing."""
+    freqs = np.array([1000.0, 5000.0])
+    _, _, csm = _noisy_scene(48, freqs)
+    rng = np.random.default_rng(9)
+    junk = csm.copy()
+    iu = np.triu_indices(48, 1)
+    junk[:, iu[0], iu[1]] = rng.standard_normal((2, len(iu[0]))) + 1j * rng.standard_normal((2, len(iu[0])))
+    junk = junk.astype(dtype)
+    ps = aa.PowerSpectraImport(csm=junk, frequencies=freqs)
+    ev, evec = ps.eig_device()
+    lam = np.linalg.eigvalsh(junk.astype(np.complex128))
+    tol = 1e-10 if dtype == np.complex128 else 1e-5
+    np.testing.assert_allclose(ev.cpu().numpy(), lam, rtol=0, atol=tol * np.abs(lam).max())
+    V = evec.cpu().numpy().astype(np.complex128)
+    rec = np.einsum('fme,fe,fne->fmn', V, ev.cpu().numpy(), V.conj())
+    assert np.abs(rec - csm).max() <= 1e-5 * np.abs(csm).max()
+    host_vals = ps.eigvals
+    assert host_vals.dtype == np.float64  # PowerSpectraImport's precision defaults to complex128
+
+
+def test_csm_eig_matches_numpy_per_slice(gpu):
+    """Eigenvectors of simple eigenvalues equal numpy's up to a unit phase per column."""
+    freqs = np.array([2200.0])
+    _, _, csm = _noisy_scene(20, freqs)
+    ps = aa.PowerSpectraImport(csm=csm, frequencies=freqs)
+    ps.precision = 'complex128'
+    V = ps.eigvecs[0]
+    lam, W = np.linalg.eigh(csm[0])
+    np.testing.assert_allclose(ps.eigvals[0], lam, rtol=1e-12, atol=1e-12 * np.abs(lam).max())
+    ph = np.einsum('me,me->e', W.conj(), V)
+    np.testing.assert_allclose(np.abs(ph), 1.0, atol=1e-9)
+    np.testing.assert_allclose(V, W * ph, atol=1e-9)
+
+
+def _bf_setup(nm=32, seed=11):
+    freqs = np.array([1200.0, 2500.0, 6000.0, 11000.0])
+    mics, grid, csm = _noisy_scene(nm, freqs, seed=seed)
+    ps = aa.PowerSpectraImport(csm=csm, frequencies=freqs)
+    return freqs, mics, grid, csm, ps
+
+
+@pytest.mark.parametrize('steer', STEERS)
+@pytest.mark.parametrize('r_diag', [False, True])
+@pytest.mark.parametrize('n', [-1, -3, 2])
+def test_beamformer_eig_matches_oracle(gpu, steer, r_diag, n):
+    freqs, mics, grid, csm, ps = _bf_setup()
+    bf = aa.BeamformerEig(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type=steer),
+                          r_diag=r_diag, n=n)
+    got = bf.result[:]
+    nm = mics.num_mics
+    na = min(nm - 1, max(nm + n, 0) if n < 0 else n)
+    assert bf.na == na
+    norm = B.sig_loss_norm(r_diag, 0.0, nm)
+    want, _ = _oracle_eig_maps(csm, freqs, mics, grid, steer, r_diag, norm, slice(na, na + 1))
+    if r_diag:
+        want = np.where(want < 0, 0.0, want)
+    assert peak_rel_err(got, want) <= TOL
+
+
+@pytest.mark.parametrize('steer', ['true level', 'classic'])
+@pytest.mark.parametrize('n', [1, 3])
+def test_beamformer_music_matches_oracle(gpu, steer, n):
+    freqs, mics, grid, csm, ps = _bf_setup()
+    bf = aa.BeamformerMusic(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type=steer), n=n)
+    got = bf.result[:]
+    nm = mics.num_mics
+    k = nm - bf.na
+    want, _ = _oracle_eig_maps(csm, freqs, mics, grid, steer, False, float(nm) ** 2, slice(0, k))
+    want = 4e-10 * want.min(axis=1, keepdims=True) / want
+    assert peak_rel_err(got, want) <= TOL
+
+
+@pytest.mark.parametrize('steer', ['true level', 'true location', 'classic'])
+@pytest.mark.parametrize('gamma', [1.0, 4.0])
+def test_beamformer_functional_matches_oracle(gpu, steer, gamma):
+    freqs, mics, grid, csm, ps = _bf_setup()
+    bf = aa.BeamformerFunctional(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type=steer),
+                                 gamma=gamma)
+    got = bf.result[:]
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    want = []
+    for f, fr in enumerate(freqs):
+        lam, V = np.linalg.eigh(csm[f])
+        q, sn = B.freq_beamformer_eig(lam ** (1.0 / gamma), V, r0, rm, 2 * np.pi * fr / 343.0, 1.0, False, steer)
+        want.append(((q / sn) ** gamma) * sn)
+    assert peak_rel_err(got, np.array(want)) <= TOL
+    if gamma == 1.0:
+        # gamma = 1 is the conventional beamformer on the full CSM
+        base = aa.BeamformerBase(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type=steer),
+                                 r_diag=False, form='plain_f32')
+        assert peak_rel_err(got, base.result[:]) <= TOL
