@@ -11,15 +11,19 @@ from .environments import Environment, dist_mat
 from .fastFuncs import beamformerFreq, calcCSM, calcTransfer
 from .fbeamform import (
     BeamformerBase,
+    BeamformerDamas,
     BeamformerEig,
     BeamformerFunctional,
     BeamformerMusic,
     LazyBfResult,
     L_p,
+    PointSpreadFunction,
     SteeringVector,
     beamform_device,
     beamform_eig_device,
+    damas_device,
     integrate,
+    psf_device,
 )
 from .grids import Grid, ImportGrid, RectGrid, RectGrid3D
 from .microphones import MicGeom
@@ -29,6 +33,7 @@ from .spectra import BaseSpectra, PowerSpectra, PowerSpectraImport
 __all__ = [
     'BaseSpectra',
     'BeamformerBase',
+    'BeamformerDamas',
     'BeamformerEig',
     'BeamformerFunctional',
     'BeamformerMusic',
@@ -39,6 +44,7 @@ __all__ = [
     'LazyBfResult',
     'MicGeom',
     'PowerSpectra',
+    'PointSpreadFunction',
     'PowerSpectraImport',
     'RectGrid',
     'RectGrid3D',
@@ -46,9 +52,11 @@ __all__ = [
     'TimeSamples',
     'beamform_device',
     'beamform_eig_device',
+    'damas_device',
     'beamformerFreq',
     'calcCSM',
     'calcTransfer',
     'dist_mat',
     'integrate',
+    'psf_device',
 ]

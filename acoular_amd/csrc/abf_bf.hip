@@ -23,26 +23,6 @@
 namespace abf {
 
 // -------------------------------------------------------------------------------------------
-// steering element: v = w * (cos phi, -sin phi), phi = k r
-// -------------------------------------------------------------------------------------------
-__device__ __forceinline__ void phasor_rev(double phi, int phase_mode, int fast, float &c, float &s)
-{
-    if (phase_mode == ABF_PHASE_REFERENCE) phi = (double)(float)phi;
-    if (fast) {
-        // hardware sin/cos take their argument in revolutions; reduce to [-1/2, 1/2] in float64
-        const double u = phi * kInvTwoPi;
-        const float t = (float)(u - rint(u));
-        s = __builtin_amdgcn_sinf(t);
-        c = __builtin_amdgcn_cosf(t);
-    } else {
-        const double n = rint(phi * kInvTwoPi);
-        double t = fma(-n, kTwoPiHi, phi);
-        t = fma(-n, kTwoPiLo, t);
-        sincosf((float)t, &s, &c);
-    }
-}
-
-// -------------------------------------------------------------------------------------------
 // U^T for the plain kernel: ut[f][j][i] = U_ij, zero padded to Mp x Mp
 // -------------------------------------------------------------------------------------------
 __global__ void prep_plain_kernel(const float2 *__restrict__ csm, int M, int Mp, int r_diag, float2 *__restrict__ ut)

@@ -27,6 +27,25 @@ __device__ __forceinline__ void phasor(double phi, int phase_mode, float &c, flo
     sincosf((float)t, &s, &c);
 }
 
+// cos/sin of phi as phasor()'s, with fast != 0 taken by the hardware transcendental unit:
+// v_sin/v_cos take their argument in revolutions, so the phase is reduced to [-1/2, 1/2]
+// revolutions in float64 first.
+__device__ __forceinline__ void phasor_rev(double phi, int phase_mode, int fast, float &c, float &s)
+{
+    if (phase_mode == ABF_PHASE_REFERENCE) phi = (double)(float)phi;
+    if (fast) {
+        const double u = phi * kInvTwoPi;
+        const float t = (float)(u - rint(u));
+        s = __builtin_amdgcn_sinf(t);
+        c = __builtin_amdgcn_cosf(t);
+    } else {
+        const double n = rint(phi * kInvTwoPi);
+        double t = fma(-n, kTwoPiHi, phi);
+        t = fma(-n, kTwoPiLo, t);
+        sincosf((float)t, &s, &c);
+    }
+}
+
 // round-to-nearest-even float -> bf16 bit pattern (finite inputs)
 __device__ __forceinline__ uint32_t bf16_bits(float x)
 {

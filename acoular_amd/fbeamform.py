@@ -427,6 +427,203 @@ class BeamformerFunctional(BeamformerBase):
         self._fr[ind] = 1
 
 
+class PointSpreadFunction(HasParams):
+    """The point spread function of a grid and microphone array (``fbeamform.py:948-1124``).
+
+    ``psf`` is ``[G, len(grid_indices)]`` (``[G, G]`` for all points): column ``c`` is the map a
+    unit monopole at grid point ``grid_indices[c]`` produces.  Computed columns are kept per
+    ``(digest, freq)`` in memory, where the reference keeps them in an HDF5 cache node
+    ``Hz_<freq:.2f>`` (``fbeamform.py:1005-1035``); ``calcmode`` keeps its meaning ('readonly'
+    refuses to compute columns missing from a stored entry).  Every mode computes the missing
+    columns in one ``abf_psf`` launch.
+    """
+
+    _params = ('steer', 'grid_indices', 'calcmode', 'precision', 'freq', 'cached', 'phase_mode')
+
+    def __init__(self, **kwargs):
+        self.steer = SteeringVector()
+        self.grid_indices = np.array([], dtype=int)
+        self.calcmode = 'single'
+        self.precision = 'float64'
+        self.freq = 1.0
+        self.cached = True
+        self.phase_mode = config['phase_mode']
+        object.__setattr__(self, '_store', {})
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name == 'steer' and not isinstance(value, SteeringVector):
+            raise TypeError('steer must be a SteeringVector')
+        if name == 'calcmode' and value not in ('single', 'block', 'full', 'readonly'):
+            raise ValueError("calcmode must be one of 'single', 'block', 'full', 'readonly'")
+        if name == 'precision' and value not in ('float64', 'float32'):
+            raise ValueError("precision must be 'float64' or 'float32'")
+        if name == 'grid_indices':
+            value = np.atleast_1d(np.asarray(value, dtype=int))
+        if name == 'freq':
+            value = float(value)
+        object.__setattr__(self, name, value)
+
+    def _digest_items(self):
+        # the reference's digest depends on the steering vector and the precision only
+        return (self.steer, self.precision)
+
+    @property
+    def psf(self):
+        """``_get_psf`` (``fbeamform.py:1037-1071``)."""
+        gs = self.steer.grid.size
+        grid_indices = self.grid_indices if self.grid_indices.size else np.arange(gs)
+        key = (self.digest, f'Hz_{self.freq:.2f}')
+        entry = self._store.get(key) if self.cached else None
+        from_cache = entry is not None
+        if from_cache:
+            ac, gp = entry
+        else:
+            ac = np.zeros((gs, gs), dtype=self.precision)
+            gp = np.zeros((gs,), dtype='int8')
+        if not gp[grid_indices].astype(bool).all():
+            if from_cache and self.calcmode == 'readonly':
+                msg = "Cannot calculate missing PSF (points) in 'readonly' mode."
+                raise ValueError(msg)
+            self.calc_psf(ac, gp, grid_indices)
+            if self.cached:
+                self._store[key] = (ac, gp)
+        if grid_indices.size == gs and np.array_equal(grid_indices, np.arange(gs)):
+            return ac
+        return ac[:, grid_indices].copy()
+
+    def calc_psf(self, ac, gp, grid_indices=None):
+        """Fill the columns ``grid_indices`` of ``ac`` and mark them in ``gp``
+        (``fbeamform.py:1073-1094``)."""
+        if grid_indices is None:
+            grid_indices = self.grid_indices if self.grid_indices.size else np.arange(self.steer.grid.size)
+        if self.calcmode == 'full':
+            gp[:] = 1
+            ac[:] = self._psf_call(np.arange(self.steer.grid.size))
+            return
+        todo = grid_indices[~gp[grid_indices].astype(bool)]
+        if todo.size:
+            ac[:, todo] = self._psf_call(todo)
+            gp[todo] = 1
+
+    def _psf_call(self, ind):
+        """``_psf_call`` (``fbeamform.py:1096-1124``): ``[G, len(ind)]`` of ``precision``."""
+        ind = np.asarray(ind, dtype=np.int64)
+        if type(self.steer) is SteeringVector:
+            out = psf_device(self.steer, [self.freq], ind, self.precision, self.phase_mode)
+            return out[0].cpu().numpy()
+        product = np.dot(self.steer.steer_vector(self.freq).conj(), self.steer.transfer(self.freq, ind).T)
+        return (product * product.conj()).real.astype(self.precision)
+
+
+class BeamformerDamas(BeamformerBase):
+    """DAMAS deconvolution (``fbeamform.py:1127-1208``): the conventional map of every requested
+    bin, then ``n_iter`` relaxed Gauss-Seidel sweeps (damping ``damp``) against the full point
+    spread function.  The dirty maps, the PSFs of all bins and the sweeps stay on the device:
+    one ``abf_beamform`` job, one ``abf_psf`` launch and one ``abf_damas`` launch per group of
+    bins whose PSFs fit the memory budget.
+    """
+
+    _params = BeamformerBase._params + ('n_iter', 'damp', 'psf_precision', 'calcmode')
+    #: bytes of PSF matrices held on the device at once
+    psf_budget = 8 << 30
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'psf_precision', 'float64')
+        object.__setattr__(self, 'n_iter', 100)
+        object.__setattr__(self, 'damp', 1.0)
+        object.__setattr__(self, 'calcmode', 'full')
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name == 'psf_precision' and value not in ('float64', 'float32'):
+            raise ValueError("psf_precision must be 'float64' or 'float32'")
+        if name == 'calcmode' and value not in ('full', 'single', 'block', 'readonly'):
+            raise ValueError("calcmode must be one of 'full', 'single', 'block', 'readonly'")
+        if name == 'n_iter':
+            value = int(value)
+        if name == 'damp':
+            value = float(value)
+        super().__setattr__(name, value)
+
+    def _digest_items(self):
+        # calcmode does not change the result (fbeamform.py:1158-1160)
+        return tuple(getattr(self, p) for p in self._params if p != 'calcmode')
+
+    def _calc(self, ind):
+        torch = _lib.torch()
+        ind = np.asarray(ind, dtype=np.int64)
+        normfactor = self.sig_loss_norm()
+        G = self.steer.grid.size
+        f64 = self.psf_precision == 'float64'
+        if type(self.steer) is not SteeringVector:
+            p = PointSpreadFunction(steer=self.steer, calcmode=self.calcmode, precision=self.psf_precision,
+                                    cached=self.cached, phase_mode=self.phase_mode)
+            _, steer_func = self._beamformer_params()
+            for i in ind:
+                csm = np.array(self.freq_data.csm[i], dtype='complex128')
+                y = beamformerFreq('custom', self.r_diag, normfactor, steer_func(self._f[i]), csm)[0]
+                if self.r_diag:
+                    y[np.sign(y) < 0] = 0.0
+                p.freq = self._f[i]
+                self._ac[i] = damas_device(_lib.to_device(np.ascontiguousarray(p.psf)), _lib.to_device(y[None]),
+                                           self.n_iter, self.damp)[0].cpu().numpy()
+                self._fr[i] = 1
+            return
+        per_bin = G * G * (8 if f64 else 4)
+        chunk = max(1, int(self.psf_budget // max(per_bin, 1)))
+        for c0 in range(0, len(ind), chunk):
+            sel = ind[c0:c0 + chunk]
+            y = beamform_device(self.freq_data.csm_device, self._f[sel], sel, self.steer, self.steer.steer_type,
+                                self.r_diag, normfactor, self.form, self.phase_mode, clip=self.r_diag)
+            psf = psf_device(self.steer, self._f[sel], np.arange(G), self.psf_precision, self.phase_mode)
+            x = damas_device(psf, y.to(torch.float64), self.n_iter, self.damp)
+            del psf
+            self._ac[sel] = x.cpu().numpy().astype(self.precision)
+            self._fr[sel] = 1
+
+
+def psf_device(steer: SteeringVector, freqs, sources, precision: str = 'float64', phase_mode: str = 'reference',
+               fast_trig: bool | None = None):
+    """One ``abf_psf`` launch: the PSFs of the grid points ``sources`` at every frequency of
+    ``freqs`` as a device tensor ``[F, G, len(sources)]`` of ``precision``."""
+    sources = np.asarray(sources, dtype=np.int64).ravel()
+    G = steer.grid.size
+    if sources.size and (sources.min() < 0 or sources.max() >= G):
+        raise IndexError('grid index out of range')
+    torch = _lib.torch()
+    dev = _lib.device()
+    freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+    if fast_trig is None:
+        fast_trig = config['fast_trig']
+    gpos, mpos = steer.device_geometry()
+    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    src = torch.from_numpy(sources).to(dev)
+    dt = torch.float64 if precision == 'float64' else torch.float32
+    out = torch.empty((len(freqs), G, len(sources)), dtype=dt, device=dev)
+    pm = _lib.PHASE_MODES[phase_mode] | (_lib.FAST_TRIG if fast_trig else 0)
+    _lib.check(_lib.lib().abf_psf(_geometry(gpos, mpos, steer.ref), _lib.ptr(k), len(freqs), _lib.ptr(src),
+                                  len(sources), _lib.STEER_CODES[steer.steer_type], pm, int(dt == torch.float64),
+                                  _lib.ptr(out), _lib.stream_handle()), 'point spread function')
+    return out
+
+
+def damas_device(psf, dirty, n_iter: int, damp: float):
+    """One ``abf_damas`` launch: ``n_iter`` Gauss-Seidel sweeps of ``psf [F, G, G] x = dirty
+    [F, G]`` (float64 device tensors; ``psf`` may be float32) from ``x = dirty``."""
+    torch = _lib.torch()
+    if psf.dtype not in (torch.float64, torch.float32) or dirty.dtype != torch.float64:
+        raise TypeError('psf must be float64 or float32 and dirty float64')
+    psf, dirty = psf.contiguous(), dirty.contiguous()
+    F, G = dirty.shape
+    if tuple(psf.shape) != (F, G, G):
+        raise ValueError(f'psf of shape {tuple(psf.shape)} does not fit dirty maps of shape {(F, G)}')
+    x = dirty.clone()
+    _lib.check(_lib.lib().abf_damas(_lib.ptr(psf), int(psf.dtype == torch.float64), _lib.ptr(dirty), _lib.ptr(x),
+                                    G, F, int(n_iter), float(damp), _lib.stream_handle()), 'DAMAS')
+    return x
+
+
 def _geometry(gpos, mpos, ref) -> _lib.AbfGeometry:
     """``abf_geometry`` of device coordinate tensors and a reference point or distance."""
     g = _lib.AbfGeometry()

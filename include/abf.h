@@ -171,6 +171,23 @@ size_t abf_csm_eig_workspace_size(int32_t num_freqs, int32_t num_mics, int32_t i
 int abf_csm_eig(const void *csm, int32_t num_freqs, int32_t num_mics, int32_t is_c128, double *eigval, void *eigvec,
                 int32_t *info, void *workspace, size_t workspace_bytes, void *stream);
 
+/* Point spread functions (calcPointSpreadFunction, fastFuncs.py:563-801, called by
+ * PointSpreadFunction._psf_call, fbeamform.py:1096-1124): for every frequency f and every
+ * source grid point s = sources[c] (device int64 [S], indices into the grid),
+ * out[f][g][c] = |h_g^H a_s|^2 in the normalisation of steer_type, float64 (out_f64) or float32,
+ * with the float32-rounded phase difference of the reference (phase_mode REFERENCE).  An index
+ * outside [0, num_grid) yields a NaN column. */
+int abf_psf(const abf_geometry *geom, const double *wavenumber, int32_t num_freqs, const int64_t *sources,
+            int64_t num_sources, int32_t steer_type, int32_t phase_mode, int32_t out_f64, void *out, void *stream);
+
+/* DAMAS (damasSolverGaussSeidel, fastFuncs.py:820-855, called by BeamformerDamas._calc,
+ * fbeamform.py:1167-1208): n_iter relaxed Gauss-Seidel sweeps with the positivity clip on every
+ * frequency's system psf[f] (float64 if psf_f64, else float32; [F][G][G], row g = grid point)
+ * x[f] = dirty[f]; x [F][G] holds the start values on entry and the solutions on return.
+ * num_grid <= 15360 (the solution vector lives in LDS). */
+int abf_damas(const void *psf, int32_t psf_f64, const double *dirty, double *x, int64_t num_grid, int32_t num_freqs,
+              int32_t n_iter, double relax, void *stream);
+
 /* _freqBeamformer_SpecificSteerVec_FullCSM / _CsmRemovedDiag (fastFuncs.py:414-476): explicit
  * steering vectors steer [G][M] complex64. out = q * norm, out_norm = sum |h|^2. */
 int abf_beamform_custom(const void *csm, const void *steer, int64_t num_grid, int32_t num_mics, int32_t r_diag,

@@ -13,6 +13,11 @@
  *   cref_calc_csm             <- calcCSM                   acoular/fastFuncs.py:30-58
  *       (prange over frequencies, upper triangle incl. diagonal)
  *   cref_dist_mat             <- dist_mat                  acoular/environments.py:65-103
+ *   cref_psf                  <- calcPointSpreadFunction   acoular/fastFuncs.py:563-801
+ *       (float32-rounded phase difference and float32 cos/sin, float64 sums; OpenMP over grid
+ *        points like the guvectorize target)
+ *   cref_damas_gauss_seidel   <- damasSolverGaussSeidel    acoular/fastFuncs.py:820-855
+ *       (the same row order and summation order; OpenMP over frequencies)
  *
  * Complex arrays are interleaved (re, im) doubles, C-contiguous, as numpy complex128.
  */
@@ -132,6 +137,71 @@ void cref_calc_csm(double *csm, const double *spec, int64_t nf, int64_t nm)
                 double sr = s[2 * row], si = s[2 * row + 1];
                 c[2 * (row * nm + col)] += tr * sr - ti * si;
                 c[2 * (row * nm + col) + 1] += tr * si + ti * sr;
+            }
+        }
+    }
+}
+
+/* psf [ng][ns]: column c is the PSF of source grid point ind[c] */
+void cref_psf(const double *r0, const double *rm, int64_t ng, int64_t nm, double k, const int64_t *ind, int64_t ns,
+              int steer_type, double *psf)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < ng; ++g) {
+        const double *rg = rm + g * nm;
+        double help = 0.0;
+        for (int64_t m = 0; m < nm; ++m) help += 1.0 / (rg[m] * rg[m]);
+        for (int64_t c = 0; c < ns; ++c) {
+            const int64_t s = ind[c];
+            const double *rs = rm + s * nm;
+            double pr = 0.0, pi = 0.0;
+            for (int64_t m = 0; m < nm; ++m) {
+                float arg = (float)(k * (rg[m] - rs[m]));
+                double w = 1.0 / rs[m];
+                if (steer_type == 2) w *= rg[m];
+                if (steer_type == 3 || steer_type == 4) w /= rg[m];
+                pr += (double)cosf(arg) * w;
+                pi -= (double)sinf(arg) * w;
+            }
+            const double sq = pr * pr + pi * pi;
+            double nf;
+            switch (steer_type) {
+            case 1:
+                nf = r0[s] / nm;
+                psf[g * ns + c] = sq * nf * nf;
+                break;
+            case 2:
+                nf = r0[s] / r0[g] / nm;
+                psf[g * ns + c] = sq * nf * nf;
+                break;
+            case 3:
+                nf = r0[s] / r0[g] / help;
+                psf[g * ns + c] = sq * nf * nf;
+                break;
+            default:
+                psf[g * ns + c] = sq * (r0[s] * r0[s]) / nm / help;
+                break;
+            }
+        }
+    }
+}
+
+/* A [nf][ng][ng], dirty [nf][ng], x [nf][ng] (start value in, solution out) */
+void cref_damas_gauss_seidel(const double *A, const double *dirty, double *x, int64_t nf, int64_t ng,
+                             int64_t n_iter, double relax)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t f = 0; f < nf; ++f) {
+        const double *a = A + f * ng * ng;
+        const double *y = dirty + f * ng;
+        double *xs = x + f * ng;
+        for (int64_t it = 0; it < n_iter; ++it) {
+            for (int64_t i = 0; i < ng; ++i) {
+                double s = 0.0;
+                for (int64_t j = 0; j < ng; ++j) s += a[i * ng + j] * xs[j];
+                s -= a[i * ng + i] * xs[i];
+                s = (1 - relax) * xs[i] + relax * (y[i] - s);
+                xs[i] = s > 0.0 ? s : 0.0;
             }
         }
     }

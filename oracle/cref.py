@@ -32,6 +32,11 @@ def lib():
         _lib.cref_calc_csm.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int64]
         _lib.cref_dist_mat.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int64, dp]
         _lib.cref_num_threads.restype = ctypes.c_int
+        ip = ctypes.POINTER(ctypes.c_int64)
+        _lib.cref_psf.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ip, ctypes.c_int64,
+                                  ctypes.c_int, dp]
+        _lib.cref_damas_gauss_seidel.argtypes = [dp, dp, dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                 ctypes.c_double]
     return _lib
 
 
@@ -72,3 +77,26 @@ def calc_csm(csm, spec):
     lib().cref_calc_csm(csm.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                         s.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)), s.shape[0], s.shape[1])
     return csm
+
+
+def psf(steer_type, r0, rm, k, ind):
+    """``calcPointSpreadFunction`` (``fastFuncs.py:563-801``) via the C loop nest: ``[G, len(ind)]``."""
+    r0 = np.ascontiguousarray(r0, dtype=np.float64)
+    rm = np.ascontiguousarray(rm, dtype=np.float64)
+    ind = np.ascontiguousarray(ind, dtype=np.int64).ravel()
+    ng, nm = rm.shape
+    out = np.empty((ng, len(ind)))
+    st = {'classic': 1, 'inverse': 2, 'true level': 3, 'true location': 4}[steer_type]
+    lib().cref_psf(_p(r0), _p(rm), ng, nm, float(k), ind.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(ind),
+                   st, _p(out))
+    return out
+
+
+def damas_gauss_seidel(A, dirty, x, n_iter, relax):
+    """``damasSolverGaussSeidel`` (``fastFuncs.py:820-855``) over ``[F, G, G]`` systems, in place on ``x``."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    y = np.ascontiguousarray(dirty, dtype=np.float64)
+    assert x.dtype == np.float64 and x.flags.c_contiguous and x.shape == y.shape
+    nf, ng = y.shape
+    lib().cref_damas_gauss_seidel(_p(A), _p(y), _p(x), nf, ng, int(n_iter), float(relax))
+    return x

@@ -1,0 +1,58 @@
+"""Host logic of PointSpreadFunction and BeamformerDamas (no GPU): parameters, digests and the
+index check that runs before any device work (fbeamform.py:948-1208)."""
+
+import numpy as np
+import pytest
+
+import acoular_amd as aa
+
+
+def _steer(nm=6):
+    pos = np.zeros((3, nm))
+    pos[0] = np.arange(nm) * 0.1
+    grid = aa.RectGrid(x_min=-0.1, x_max=0.1, y_min=-0.1, y_max=0.1, z=0.5, increment=0.1)
+    return aa.SteeringVector(grid=grid, mics=aa.MicGeom(pos_total=pos))
+
+
+def test_psf_parameters():
+    p = aa.PointSpreadFunction(steer=_steer())
+    assert p.calcmode == 'single' and p.precision == 'float64' and p.freq == 1.0 and p.grid_indices.size == 0
+    p.grid_indices = [1, 4]
+    assert p.grid_indices.dtype.kind == 'i'
+    with pytest.raises(ValueError):
+        p.calcmode = 'all'
+    with pytest.raises(ValueError):
+        p.precision = 'float16'
+    with pytest.raises(TypeError):
+        p.steer = None
+
+
+def test_psf_digest_depends_on_steer_and_precision_only():
+    sv = _steer()
+    a = aa.PointSpreadFunction(steer=sv, freq=1000.0)
+    b = aa.PointSpreadFunction(steer=sv, freq=2000.0, calcmode='full', grid_indices=[2])
+    assert a.digest == b.digest
+    assert aa.PointSpreadFunction(steer=sv, precision='float32').digest != a.digest
+    assert aa.PointSpreadFunction(steer=_steer(7)).digest != a.digest
+
+
+def test_psf_device_rejects_out_of_range_indices():
+    with pytest.raises(IndexError):
+        aa.psf_device(_steer(), [1000.0], [0, 9])
+    with pytest.raises(IndexError):
+        aa.psf_device(_steer(), [1000.0], [-1])
+
+
+def test_damas_parameters():
+    bf = aa.BeamformerDamas(steer=_steer())
+    assert bf.n_iter == 100 and bf.damp == 1.0 and bf.psf_precision == 'float64' and bf.calcmode == 'full'
+    assert bf.r_diag is True
+    with pytest.raises(ValueError):
+        bf.psf_precision = 'float16'
+    with pytest.raises(ValueError):
+        bf.calcmode = 'all'
+    d = bf.digest
+    bf.calcmode = 'block'
+    assert bf.digest == d  # calcmode does not change the result
+    bf.n_iter = 50
+    assert bf.digest != d

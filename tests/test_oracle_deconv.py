@@ -1,0 +1,125 @@
+"""Pins of the oracle's point spread function and DAMAS solver (oracle/deconv.py, oracle/cpu_ref.c).
+
+* PSF (fastFuncs.py:563-801): closed forms on the diagonal (a source seen at its own grid point:
+  1 for 'inverse' and 'true level', r0^2 sum(1/rm^2) / M for 'true location',
+  (r0/M)^2 (sum 1/rm)^2 for 'classic'), and, with the exact float64 phase, equality with the
+  already pinned beamformer oracle applied to the CSM a_s a_s^H of a unit monopole at the
+  source point (the identity the reference's docstring states, fastFuncs.py:592-601); the C
+  loop nest equals the numpy restatement.
+* DAMAS Gauss-Seidel (fastFuncs.py:820-855): one sweep on the identity is the clipped dirty
+  map; a hand-worked 2 x 2 sweep fixes the row order (updated values are used at once) and the
+  relaxation formula; many sweeps on a diagonally dominant system reach numpy.linalg.solve; the
+  C loop nest equals the numpy restatement.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import beamform as B
+from oracle import cref
+from oracle import deconv as D
+
+STEERS = ['classic', 'inverse', 'true level', 'true location']
+
+
+def _geometry(nm=16, n=7, seed=3):
+    rng = np.random.default_rng(seed)
+    r = np.sqrt(rng.uniform(0, 1, nm)) * 0.5
+    ph = rng.uniform(0, 2 * np.pi, nm)
+    mpos = np.stack([r * np.cos(ph), r * np.sin(ph), np.zeros(nm)])
+    x = np.linspace(-0.3, 0.3, n)
+    gpos = np.stack([np.repeat(x, n), np.tile(x, n), np.full(n * n, 0.6)])
+    return gpos, mpos
+
+
+@pytest.mark.parametrize('steer', STEERS)
+def test_psf_diagonal_closed_forms(steer):
+    gpos, mpos = _geometry()
+    r0 = B.steer_r0(gpos, np.zeros(3))
+    rm = B.dist_mat(gpos, mpos)
+    nm = mpos.shape[1]
+    ind = np.arange(gpos.shape[1])
+    p = D.psf(steer, r0, rm, 2 * np.pi * 3000 / 343.0, ind)
+    d = np.diag(p)
+    help_g = np.sum(1.0 / rm**2, axis=1)
+    want = {'inverse': np.ones_like(d), 'true level': np.ones_like(d),
+            'true location': r0**2 * help_g / nm,
+            'classic': (r0 / nm) ** 2 * np.sum(1.0 / rm, axis=1) ** 2}[steer]
+    np.testing.assert_allclose(d, want, rtol=1e-13)
+
+
+@pytest.mark.parametrize('steer', STEERS)
+def test_psf_equals_beamformer_of_point_source(steer):
+    gpos, mpos = _geometry()
+    r0 = B.steer_r0(gpos, np.zeros(3))
+    rm = B.dist_mat(gpos, mpos)
+    k = 2 * np.pi * 5000 / 343.0
+    ind = np.array([0, 11, 24, 48])
+    p = D.psf(steer, r0, rm, k, ind, phase_mode='fp64')
+    for c, s in enumerate(ind):
+        a = B.calc_transfer(r0[s:s + 1], rm[s:s + 1], k, phase_mode='fp64')[0]
+        csm = np.outer(a, a.conj())
+        bf = B.freq_beamformer_full_csm(csm, r0, rm, k, 1.0, False, steer, phase_mode='fp64')[0]
+        np.testing.assert_allclose(p[:, c], bf, rtol=1e-10, atol=1e-13 * np.abs(bf).max())
+
+
+@pytest.mark.parametrize('steer', STEERS)
+def test_psf_c_loop_equals_numpy(steer):
+    gpos, mpos = _geometry(nm=24, n=9, seed=8)
+    r0 = B.steer_r0(gpos, np.zeros(3))
+    rm = B.dist_mat(gpos, mpos)
+    k = 2 * np.pi * 9000 / 343.0
+    ind = np.array([3, 40, 80, 5])
+    # libm cosf/sinf and numpy's correctly rounded float32 cos/sin may differ by one float32 ulp
+    a, b = cref.psf(steer, r0, rm, k, ind), D.psf(steer, r0, rm, k, ind)
+    assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+
+
+def test_damas_identity_one_sweep_is_clipped_dirty_map():
+    y = np.array([0.5, -0.2, 1.5, 0.0])
+    x = y.copy()
+    D.damas_gauss_seidel(np.eye(4), y, 1, 1.0, x)
+    np.testing.assert_array_equal(x, np.maximum(y, 0))
+
+
+def test_damas_sweep_order_and_relaxation():
+    A = np.array([[1.0, 0.5], [0.25, 1.0]])
+    y = np.array([1.0, 2.0])
+    x0 = np.array([0.4, 0.8])
+    w = 0.7
+    # row 0 uses the old x1, row 1 already the new x0 (Gauss-Seidel, not Jacobi)
+    n0 = max(0.0, (1 - w) * x0[0] + w * (y[0] - 0.5 * x0[1]))
+    n1 = max(0.0, (1 - w) * x0[1] + w * (y[1] - 0.25 * n0))
+    x = x0.copy()
+    D.damas_gauss_seidel(A, y, 1, w, x)
+    np.testing.assert_allclose(x, [n0, n1], rtol=1e-15)
+    xc = x0.copy()[None]
+    cref.damas_gauss_seidel(A[None], y[None], xc, 1, w)
+    np.testing.assert_allclose(xc[0], [n0, n1], rtol=1e-15)
+
+
+def test_damas_converges_to_the_solution():
+    rng = np.random.default_rng(4)
+    n = 12
+    A = rng.uniform(0, 0.05, (n, n))
+    np.fill_diagonal(A, 1.0)
+    xs = rng.uniform(0.1, 1.0, n)
+    y = A @ xs
+    x = y.copy()
+    D.damas_gauss_seidel(A, y, 200, 1.0, x)
+    np.testing.assert_allclose(x, np.linalg.solve(A, y), rtol=1e-12)
+
+
+def test_damas_c_loop_equals_numpy():
+    rng = np.random.default_rng(6)
+    nf, n = 3, 30
+    A = rng.uniform(0, 0.2, (nf, n, n))
+    for f in range(nf):
+        np.fill_diagonal(A[f], 1.0)
+    y = rng.uniform(-0.2, 1.0, (nf, n))
+    xc = y.copy()
+    cref.damas_gauss_seidel(A, y, xc, 25, 0.9)
+    for f in range(nf):
+        xn = y[f].copy()
+        D.damas_gauss_seidel(A[f], y[f], 25, 0.9, xn)
+        np.testing.assert_allclose(xc[f], xn, rtol=1e-12, atol=1e-15)
