@@ -21,6 +21,7 @@ from .fbeamform import (
     SteeringVector,
     beamform_device,
     beamform_eig_device,
+    beamform_f64_device,
     damas_device,
     integrate,
     psf_device,
@@ -52,6 +53,7 @@ __all__ = [
     'TimeSamples',
     'beamform_device',
     'beamform_eig_device',
+    'beamform_f64_device',
     'damas_device',
     'beamformerFreq',
     'calcCSM',

@@ -634,6 +634,58 @@ int abf_beamform_eig_custom(const double *eigval, const void *eigvec, int32_t nu
     return hip_status("eig custom beamformer launch");
 }
 
+size_t abf_beamform_f64_workspace_size(int32_t num_mics, int32_t num_freqs)
+{
+    if (num_mics < 1 || num_freqs < 0) return 0;
+    return align256((size_t)num_freqs * round_up(num_mics, 16) * round_up(num_mics, 16) * sizeof(double2));
+}
+
+int abf_beamform_f64(const abf_geometry *geom, const void *csm, const double *wavenumber, int32_t num_freqs,
+                     int32_t steer_type, int32_t r_diag, double norm_factor, int32_t phase_mode, int32_t clip_negative,
+                     double *out, double *out_norm, void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (!geom) return fail(ABF_ERR_INVALID, "null geometry");
+    if (int r = check_steer(steer_type)) return r;
+    if (int r = check_phase(phase_mode)) return r;
+    const int M = geom->num_mics;
+    const int64_t G = geom->num_grid;
+    if (M < 1 || G < 0 || num_freqs < 0) return fail(ABF_ERR_INVALID, "bad sizes");
+    if (num_freqs > 65535) return fail(ABF_ERR_INVALID, "at most 65535 frequencies per call");
+    if (G == 0 || num_freqs == 0) return ABF_OK;
+    if (!geom->gpos || !geom->mpos || !csm || !wavenumber || !out) return fail(ABF_ERR_INVALID, "null pointer");
+    if (!workspace || workspace_bytes < abf_beamform_f64_workspace_size(M, num_freqs))
+        return fail(ABF_ERR_WORKSPACE, "workspace too small");
+    const int Mp = round_up(M, 16);
+    hipStream_t s = S(stream);
+    double2 *ut = reinterpret_cast<double2 *>(workspace);
+    hipLaunchKernelGGL(prep_plain64_kernel, dim3((unsigned)cdiv((int64_t)Mp * Mp, 256), num_freqs), dim3(256), 0, s,
+                       reinterpret_cast<const double2 *>(csm), M, Mp, r_diag, ut);
+    Plain64Args a{};
+    a.ut = ut;
+    a.gpos = geom->gpos;
+    a.mpos = geom->mpos;
+    a.wavenumber = wavenumber;
+    a.out = out;
+    a.out_norm = out_norm;
+    for (int i = 0; i < 3; ++i) a.ref[i] = geom->ref[i];
+    a.ref_dist = geom->ref_dist;
+    a.norm = norm_factor;
+    a.G = G;
+    a.M = M;
+    a.Mp = Mp;
+    a.steer_type = steer_type;
+    a.phase_mode = phase_mode & kPhaseMask;
+    a.clip = clip_negative;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)bf_plain64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
+        attr = true;
+    }
+    const size_t lds = (size_t)std::min(128, Mp) * 64 * sizeof(double2);
+    hipLaunchKernelGGL(bf_plain64_kernel, dim3((unsigned)cdiv(G, 64), num_freqs), dim3(256), lds, s, a);
+    return hip_status("float64 beamformer launch");
+}
+
 int abf_beamform_eig_grid(const abf_geometry *geom, const double *wavenumber, int32_t num_freqs, const double *eigval,
                           const void *eigvec, int32_t eig_first, int32_t num_eig, int32_t steer_type, int32_t r_diag,
                           double norm_factor, int32_t phase_mode, int32_t clip_negative, float *out, float *out_norm,

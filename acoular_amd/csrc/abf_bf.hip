@@ -168,6 +168,142 @@ __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
 }
 
 // -------------------------------------------------------------------------------------------
+// float64 form: the plain kernel's structure with float64 steering vectors, CSM and sums, for
+// callers that amplify input rounding (DAMAS): the reference's own arithmetic is float64 except
+// for the float32 phase argument and the float32 cos/sin of it (fastFuncs.py:259-261), which is
+// kept (cos/sin evaluated in float64 and rounded to float32, phase_mode REFERENCE)
+// -------------------------------------------------------------------------------------------
+__global__ void prep_plain64_kernel(const double2 *__restrict__ csm, int M, int Mp, int r_diag,
+                                    double2 *__restrict__ ut)
+{
+    const int f = blockIdx.y;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)Mp * Mp) return;
+    const int j = (int)(idx / Mp), i = (int)(idx % Mp);
+    double2 v = make_double2(0.0, 0.0);
+    if (i < M && j < M) {
+        const double2 c = csm[((int64_t)f * M + i) * M + j];
+        if (j > i)
+            v = make_double2(2.0 * c.x, 2.0 * c.y);
+        else if (j == i && !r_diag)
+            v = make_double2(c.x, 0.0);
+    }
+    ut[((int64_t)f * Mp + j) * Mp + i] = v;
+}
+
+constexpr int kP64JC = 128; // microphones per LDS chunk ([128][64] double2 = 128 KiB)
+
+__device__ __forceinline__ double2 plain64_steer(const Plain64Args &a, double gx, double gy, double gz, bool valid,
+                                                 int m, double k, double &help)
+{
+    help = 0.0;
+    if (!valid || m >= a.M) return make_double2(0.0, 0.0);
+    const double r = dist3(gx, gy, gz, a.mpos[m], a.mpos[a.M + m], a.mpos[2 * a.M + m]);
+    double phi = k * r, c, s;
+    if (a.phase_mode == ABF_PHASE_REFERENCE) {
+        phi = (double)(float)phi;
+        sincos(phi, &s, &c);
+        c = (double)(float)c;
+        s = (double)(float)s;
+    } else {
+        sincos(phi, &s, &c);
+    }
+    double w = 1.0;
+    if (a.steer_type == ABF_STEER_INVERSE) {
+        w = r;
+        help = r * r;
+    } else if (a.steer_type != ABF_STEER_CLASSIC) {
+        w = 1.0 / r;
+        help = 1.0 / (r * r);
+    }
+    return make_double2(w * c, -w * s);
+}
+
+__global__ __launch_bounds__(256) void bf_plain64_kernel(Plain64Args a)
+{
+    extern __shared__ double2 vl64[]; // [kP64JC][kPlainNP]
+    __shared__ double s_help[kPlainWaves][kPlainNP];
+    __shared__ double s_q[kPlainWaves][kPlainNP];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int f = blockIdx.y;
+    const int64_t g = (int64_t)blockIdx.x * kPlainNP + lane;
+    const bool valid = g < a.G;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (valid) {
+        gx = a.gpos[g];
+        gy = a.gpos[a.G + g];
+        gz = a.gpos[2 * a.G + g];
+    }
+    const double k = a.wavenumber[f];
+    const double2 *__restrict__ ut = a.ut + (int64_t)f * a.Mp * a.Mp;
+    double help = 0.0, q = 0.0;
+    for (int c0 = 0; c0 < a.Mp; c0 += kP64JC) {
+        const int jn = min(kP64JC, a.Mp - c0);
+        __syncthreads();
+        for (int mm = wave; mm < jn; mm += kPlainWaves) {
+            double h;
+            vl64[mm * kPlainNP + lane] = plain64_steer(a, gx, gy, gz, valid, c0 + mm, k, h);
+            help += h;
+        }
+        __syncthreads();
+        const int nrb = (c0 + jn) / kPlainRB;
+        const int rounds = (nrb + kPlainWaves - 1) / kPlainWaves;
+        for (int rd = 0; rd < rounds; ++rd) {
+            const int rb = rd * kPlainWaves + ((rd & 1) ? (kPlainWaves - 1 - wave) : wave);
+            if (rb >= nrb) continue;
+            const int i0 = rb * kPlainRB;
+            const int js = max(i0, c0);
+            double ar[kPlainRB], ai[kPlainRB];
+#pragma unroll
+            for (int r = 0; r < kPlainRB; ++r) ar[r] = ai[r] = 0.0;
+            for (int j = js; j < c0 + jn; ++j) {
+                const double2 vj = vl64[(j - c0) * kPlainNP + lane];
+                const double2 *__restrict__ u = ut + (int64_t)j * a.Mp + i0;
+#pragma unroll
+                for (int r = 0; r < kPlainRB; ++r) {
+                    const double2 uu = u[r];
+                    ar[r] = fma(uu.x, vj.x, ar[r]);
+                    ar[r] = fma(-uu.y, vj.y, ar[r]);
+                    ai[r] = fma(uu.x, vj.y, ai[r]);
+                    ai[r] = fma(uu.y, vj.x, ai[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kPlainRB; ++r) {
+                const int i = i0 + r;
+                double2 vi;
+                if (i >= c0) {
+                    vi = vl64[(i - c0) * kPlainNP + lane];
+                } else {
+                    double h;
+                    vi = plain64_steer(a, gx, gy, gz, valid, i, k, h);
+                }
+                q = fma(vi.x, ar[r], q);
+                q = fma(vi.y, ai[r], q);
+            }
+        }
+    }
+    s_q[wave][lane] = q;
+    s_help[wave][lane] = help;
+    __syncthreads();
+    if (wave == 0 && valid) {
+        double qs = 0.0, hs = 0.0;
+#pragma unroll
+        for (int w = 0; w < kPlainWaves; ++w) {
+            qs += s_q[w][lane];
+            hs += s_help[w][lane];
+        }
+        const double r0 = a.ref_dist > 0.0 ? a.ref_dist : dist3(gx, gy, gz, a.ref[0], a.ref[1], a.ref[2]);
+        double res, nst;
+        normalise(a.steer_type, qs, r0, hs, (double)a.M, a.norm, res, nst);
+        if (a.clip && res < 0.0) res = 0.0;
+        a.out[(int64_t)f * a.G + g] = res;
+        if (a.out_norm) a.out_norm[(int64_t)f * a.G + g] = nst;
+    }
+}
+
+// -------------------------------------------------------------------------------------------
 // MFMA form
 // -------------------------------------------------------------------------------------------
 // Fragment layout of U for one frequency: upper 32x32 blocks (I <= J) in row-major upper

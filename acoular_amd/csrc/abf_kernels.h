@@ -23,6 +23,22 @@ struct PlainArgs {
     int steer_type, phase_mode, clip, fast_trig;
 };
 
+// float64 plain form (U^T and steering vectors in float64, float64 outputs)
+struct Plain64Args {
+    const double2 *ut;         // [F][Mp][Mp] U^T
+    const double *gpos;        // [3][G]
+    const double *mpos;        // [3][M]
+    const double *wavenumber;  // [F]
+    double *out;               // [F][G]
+    double *out_norm;          // [F][G] or null
+    double ref[3];
+    double ref_dist;
+    double norm;
+    int64_t G;
+    int M, Mp;
+    int steer_type, phase_mode, clip;
+};
+
 // one beamforming job of an MFMA launch
 struct MfJob {
     const void *prep;          // U in fragment order, [F][nUpper][2][4 KiB]
@@ -80,6 +96,8 @@ struct CustomArgs {
 
 __global__ void prep_plain_kernel(const float2 *csm, int M, int Mp, int r_diag, float2 *ut);
 __global__ void bf_plain_kernel(PlainArgs a);
+__global__ void prep_plain64_kernel(const double2 *csm, int M, int Mp, int r_diag, double2 *ut);
+__global__ void bf_plain64_kernel(Plain64Args a);
 __global__ void prep_mfma_kernel(const float2 *csm, int M, int Mp, int r_diag, int bf16x3, char *prep);
 // VAR = fast trig (bit 0) | steering weight kind (0 none, 1 r, 2 1/r) << 1
 // TT = row tiles (of 16) per work unit: 1, or 2 (both halves of a 32-row block); PF = software-

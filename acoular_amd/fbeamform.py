@@ -572,15 +572,50 @@ class BeamformerDamas(BeamformerBase):
             return
         per_bin = G * G * (8 if f64 else 4)
         chunk = max(1, int(self.psf_budget // max(per_bin, 1)))
+        csm = self.freq_data.csm_device_f64()
         for c0 in range(0, len(ind), chunk):
             sel = ind[c0:c0 + chunk]
-            y = beamform_device(self.freq_data.csm_device, self._f[sel], sel, self.steer, self.steer.steer_type,
-                                self.r_diag, normfactor, self.form, self.phase_mode, clip=self.r_diag)
+            # the sweeps amplify the rounding of the dirty map by up to ~1e3 at low frequencies, so
+            # it is formed in float64 like the reference's (fbeamform.py:1191-1197)
+            y = beamform_f64_device(csm, self._f[sel], sel, self.steer, self.steer.steer_type, self.r_diag,
+                                    normfactor, self.phase_mode, clip=self.r_diag)
             psf = psf_device(self.steer, self._f[sel], np.arange(G), self.psf_precision, self.phase_mode)
-            x = damas_device(psf, y.to(torch.float64), self.n_iter, self.damp)
+            x = damas_device(psf, y, self.n_iter, self.damp)
             del psf
             self._ac[sel] = x.cpu().numpy().astype(self.precision)
             self._fr[sel] = 1
+
+
+def beamform_f64_device(csm, freqs, ind, steer: SteeringVector, steer_type: str, r_diag: bool, norm: float,
+                        phase_mode: str = 'reference', clip: bool = False, out_norm: bool = False):
+    """One ``abf_beamform_f64`` call: float64 maps ``[len(ind), G]`` of the complex128 device CSM
+    slices ``ind`` at the frequencies ``freqs`` (and the steer normalisation if asked)."""
+    torch = _lib.torch()
+    ind = np.asarray(ind, dtype=np.int64)
+    freqs = np.asarray(freqs, dtype=np.float64)
+    if csm.dtype != torch.complex128:
+        raise TypeError('csm must be a complex128 device tensor')
+    dev = _lib.device()
+    if len(ind) != csm.shape[0] or not np.array_equal(ind, np.arange(csm.shape[0])):
+        csm = csm.index_select(0, torch.from_numpy(ind).to(dev))
+    csm = csm.contiguous()
+    gpos, mpos = steer.device_geometry()
+    G, M, F = gpos.shape[1], mpos.shape[1], len(ind)
+    if csm.shape[1] != M:
+        msg = f'{csm.shape[1]:d} channels do not fit {M:d} mics'
+        raise ValueError(msg)
+    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    out = torch.empty((F, G), dtype=torch.float64, device=dev)
+    onorm = torch.empty((F, G), dtype=torch.float64, device=dev) if out_norm else None
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.abf_beamform_f64_workspace_size(M, F))
+    _lib.check(lib.abf_beamform_f64(_geometry(gpos, mpos, steer.ref), _lib.ptr(csm), _lib.ptr(k), F,
+                                    _lib.STEER_CODES[steer_type], int(bool(r_diag)), float(norm),
+                                    _lib.PHASE_MODES[phase_mode], int(bool(clip)), _lib.ptr(out), _lib.ptr(onorm),
+                                    _lib.ptr(ws), ws.numel(), _lib.stream_handle()), 'float64 beamform')
+    if out_norm:
+        return out, onorm
+    return out
 
 
 def psf_device(steer: SteeringVector, freqs, sources, precision: str = 'float64', phase_mode: str = 'reference',

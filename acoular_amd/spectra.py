@@ -185,6 +185,11 @@ class PowerSpectra(BaseSpectra):
         """The CSM as a resident complex64 device tensor ``[F, M, M]``."""
         return self._cached('csm_device', self._calc_csm_device)
 
+    def csm_device_f64(self):
+        """The CSM as a complex128 device tensor ``[F, M, M]`` (input of the float64 beamformer)."""
+        torch = _lib.torch()
+        return self._cached('csm_device_f64', lambda: self.csm_device.to(torch.complex128))
+
     def calc_csm(self):
         """``PowerSpectra.calc_csm`` (``spectra.py:366-416``) as a numpy array of ``precision``."""
         return self._calc_csm_device().cpu().numpy().astype(self.precision)
@@ -320,6 +325,16 @@ class PowerSpectraImport(PowerSpectra):
         if dev.dtype != torch.complex64:
             dev = dev.to(torch.complex64)
         return dev
+
+    def csm_device_f64(self):
+        torch = _lib.torch()
+
+        def calc():
+            if self._csm_dev is not None:
+                return self._csm_dev.to(torch.complex128)
+            return _lib.to_device(np.asarray(self.csm_array), torch.complex128)
+
+        return self._cached('csm_device_f64', calc)
 
     def _eig_input(self):
         # np.linalg.eigh of the imported array runs in that array's own precision

@@ -515,6 +515,104 @@ def eig_bench(args):
     }
 
 
+def damas_bench(args):
+    """DAMAS on the 64-mic workload: 64 mics in a 1 m disk, RectGrid 41 x 41 (increment 0.01,
+    z = 0.3), 65 bins of block size 128 at 51.2 kHz (bins 1..64 plus DC) of an analytic complex128
+    CSM of 3 seeded sources, true level, r_diag, 100 sweeps: the conventional maps
+    (abf_beamform), the full PSFs of every bin (abf_psf, float64) and the Gauss-Seidel sweeps
+    (abf_damas), all bins per launch."""
+    import torch
+
+    import acoular_amd as aa
+    from acoular_amd import synthetic
+    from oracle import beamform as B
+    from oracle import cref
+
+    dev = torch.device('cuda', 0)
+    nm, seed, n_iter = 64, 1001, 100
+    mpos = synthetic.disk_array(nm, 1.0, seed)
+    grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.3, increment=0.01)
+    G = grid.size
+    assert G == 41 * 41
+    spos = synthetic.source_positions(3, [-0.2, -0.2, 0.3], [0.2, 0.2, 0.3], seed + 1)
+    freqs = np.arange(65) * 51200.0 / 128
+    csm = synthetic.analytic_csm(mpos, spos, synthetic.level_powers(3, 10.0, seed + 2), freqs, dtype=np.complex128)
+    mics = aa.MicGeom(pos_total=mpos)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='true level')
+    dcsm = torch.from_numpy(csm).to(dev)
+    ind = np.arange(len(freqs))
+    norm = nm / (nm - 1.0)
+    box = {}
+
+    def dirty():
+        # float64 dirty maps, as BeamformerDamas forms them
+        box['y'] = aa.beamform_f64_device(dcsm, freqs, ind, sv, 'true level', True, norm, clip=True)
+
+    def psf():
+        box['A'] = aa.psf_device(sv, freqs, np.arange(G))
+
+    def solve():
+        box['x'] = aa.damas_device(box['A'], box['y'], n_iter, 1.0)
+
+    def step():
+        dirty()
+        psf()
+        solve()
+
+    step()
+    t_step = time_steps(step, max(1, args.steps), 1)
+    parts = {name: event_time(fn, 3) * 1e3 for name, fn in (('dirty_maps', dirty), ('psf', psf), ('damas', solve))}
+    bins = len(freqs) * G
+    log(f'DAMAS step {t_step * 1e3:.2f} ms ({parts}) -> {bins / t_step:.4g} map-bins/s')
+    # DAMAS kernel: each sweep reads every bin's G x G float64 PSF once
+    damas_bytes = len(freqs) * n_iter * G * G * 8.0
+    roof = {'bound': 'hbm', 'achieved': round(damas_bytes / (parts['damas'] / 1e3) / 1e9, 1), 'peak': 8000.0,
+            'unit': 'GB/s', 'kernel': 'damas_kernel<true>'}
+    roof['frac'] = round(roof['achieved'] / roof['peak'], 4)
+    roof['traffic'] = None
+    err = None
+    if not args.no_compare:
+        r0 = B.steer_r0(grid.pos, np.zeros(3))
+        rm = B.dist_mat(grid.pos, mpos)
+        sample = [5, 33]
+        y = B.beamformer_map(csm, freqs, grid.pos, mpos, 'true level', True, ind=sample)
+        A = np.stack([cref.psf('true level', r0, rm, 2 * np.pi * freqs[b] / 343.0, np.arange(G)) for b in sample])
+        want = y.copy()
+        cref.damas_gauss_seidel(A, y, want, n_iter, 1.0)
+        got = box['x'][sample].cpu().numpy()
+        err = max(float(np.abs(got[i] - want[i]).max() / np.abs(want[i]).max()) for i in range(len(sample)))
+        log(f'DAMAS max relative map error on bins {sample}: {err:.2e}')
+    cpu = None
+    if not args.no_cpu:
+        # the C restatement (OpenMP) of the PSF and the sweeps on whole bins of the same workload
+        r0 = B.steer_r0(grid.pos, np.zeros(3))
+        rm = B.dist_mat(grid.pos, mpos)
+        y = box['y'].cpu().numpy()
+        done, t0 = 0, time.perf_counter()
+        for b in range(3, len(freqs), 7):
+            A = cref.psf('true level', r0, rm, 2 * np.pi * freqs[b] / 343.0, np.arange(G))[None]
+            x = y[b:b + 1].copy()
+            cref.damas_gauss_seidel(A, y[b:b + 1], x, n_iter, 1.0)
+            done += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {'value': G * done / dt, 'unit': 'map-bins/s', 'cores': cref.num_threads(), 'kind': 'port',
+               'sample': f'{done} bins of the same workload (PSF + {n_iter} sweeps, C + OpenMP; the dirty map not '
+                         f'timed), {dt:.1f} s'}
+        log(f"CPU: {cpu['value']:.4g} map-bins/s")
+    return {
+        'metric': 'map-bins/s (grid pts x freqs) at <=1e-4 rel err vs FP64', 'value': bins / t_step,
+        'unit': 'map-bins/s', 'n_gpus': 1, 'ms_per_step': t_step * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic (seeded analytic CSM)',
+        'config': {'workload': 'DAMAS: 64 mics (1 m disk), RectGrid 41x41 z=0.3, 65 bins (bs 128, 51.2 kHz), '
+                               '3 sources, true level, r_diag, 100 sweeps', 'num_mics': nm, 'num_grid': G,
+                   'num_freqs': len(freqs), 'parallelism': 'single GPU'},
+        'parts_ms': parts, 'roofline': roof, 'max_rel_err': err, 'cpu_baseline': cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -526,7 +624,8 @@ def main():
     ap.add_argument('--no-tune', action='store_true', help='default launch shapes (no autotuner table, no tuning)')
     ap.add_argument('--tune', action='store_true', help='tune the launch shape now even if the table has it')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
-    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig'],
+    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig',
+                                                          'damas'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
                          "'serving': the mixed-size request loop (Poisson arrivals) against the unbatched "
                          "plain kernel; 'fused': Welch CSM + beamforming of 512 mics, 10 s at 51.2 kHz, "
@@ -565,6 +664,9 @@ def main():
         return
     if args.mode == 'eig':
         print(json.dumps(eig_bench(args)), flush=True)
+        return
+    if args.mode == 'damas':
+        print(json.dumps(damas_bench(args)), flush=True)
         return
     log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
     wl = build_workload(dev)

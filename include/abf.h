@@ -149,6 +149,17 @@ int abf_beamform_eig(const double *eigval, const void *eigvec, int32_t num_eig, 
                      size_t workspace_bytes, void *stream);
 size_t abf_beamform_eig_workspace_size(int32_t num_mics, int32_t num_eig);
 
+/* _freqBeamformer_FullCSM (fastFuncs.py:238-311) for every frequency of a complex128 CSM
+ * [F][M][M] in float64 arithmetic, as the reference computes it: float64 steering vectors whose
+ * phase argument is rounded to float32 and whose cos/sin are rounded to float32
+ * (fastFuncs.py:259-261, phase_mode REFERENCE), float64 sums and float64 results out/out_norm
+ * [F][G].  For callers that amplify the rounding of the map (DAMAS, fbeamform.py:1191-1206).
+ * The workspace holds the re-laid-out CSM. */
+size_t abf_beamform_f64_workspace_size(int32_t num_mics, int32_t num_freqs);
+int abf_beamform_f64(const abf_geometry *geom, const void *csm, const double *wavenumber, int32_t num_freqs,
+                     int32_t steer_type, int32_t r_diag, double norm_factor, int32_t phase_mode, int32_t clip_negative,
+                     double *out, double *out_norm, void *workspace, size_t workspace_bytes, void *stream);
+
 /* The eigenvalue form for every frequency of a resident eigendecomposition, steering vectors
  * generated from coordinates: the loops of BeamformerEig._calc (fbeamform.py:852-888) and
  * BeamformerMusic._calc (fbeamform.py:911-945) / BeamformerFunctional._calc (fbeamform.py:712-774)

@@ -4,8 +4,9 @@ fbeamform.py:948-1124; ``BeamformerDamas``, fbeamform.py:1127-1208) against the 
 (oracle/deconv.py, oracle/cpu_ref.c).
 
 Tolerances: PSFs to 1e-5 of their largest value (float32 cos/sin and float32 partial sums
-against float64); the Gauss-Seidel sweeps on identical float64 inputs to 1e-10 relative (only
-the float64 summation order differs); whole DAMAS maps to 1e-4 of their peak.
+against float64); the float64 beamformer to 1e-9 of the map peak and the Gauss-Seidel sweeps on
+identical float64 inputs to 1e-10 relative (only the float64 summation order differs); whole
+DAMAS maps to 1e-4 of their peak.
 """
 
 import numpy as np
@@ -110,6 +111,30 @@ def test_damas_rejects_mismatched_shapes(gpu):
                         torch.zeros((1, 5), dtype=torch.float32, device=gpu), 3, 1.0)
 
 
+@pytest.mark.parametrize('steer', STEERS)
+@pytest.mark.parametrize('r_diag', [True, False])
+@pytest.mark.parametrize('phase_mode', ['reference', 'fp64'])
+@pytest.mark.parametrize('nm', [48, 200])
+def test_f64_beamformer_matches_oracle(gpu, steer, r_diag, phase_mode, nm):
+    """abf_beamform_f64 against the float64 oracle: only the float64 summation order differs
+    (200 mics take the chunked path that regenerates the steering of earlier chunks)."""
+    mics, grid = _setup(nm, 11, seed=9, inc=0.04)
+    spos = synthetic.source_positions(3, [-0.15, -0.15, 0.6], [0.15, 0.15, 0.6], 10)
+    freqs = np.array([700.0, 5000.0, 14000.0])
+    csm = synthetic.analytic_csm(mics.pos, spos, synthetic.level_powers(3, 6.0, 11), freqs, dtype=np.complex128)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer)
+    norm = B.sig_loss_norm(r_diag, 0.0, nm)
+    got, gnorm = aa.beamform_f64_device(_lib.to_device(csm), freqs, np.arange(3), sv, steer, r_diag, norm,
+                                        phase_mode, clip=r_diag, out_norm=True)
+    want = B.beamformer_map(csm, freqs, grid.pos, mics.pos, steer, r_diag, phase_mode=phase_mode)
+    assert got.dtype == _lib.torch().float64
+    assert peak_rel_err(got.cpu().numpy(), want) <= 1e-9
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    wn = B.freq_beamformer_full_csm(csm[0], r0, rm, 2 * np.pi * freqs[0] / 343.0, norm, r_diag, steer)[1]
+    np.testing.assert_allclose(gnorm.cpu().numpy()[0], wn, rtol=1e-12)
+
+
 @pytest.mark.parametrize('steer', ['true level', 'classic'])
 @pytest.mark.parametrize('r_diag', [True, False])
 def test_beamformer_damas_matches_oracle(gpu, steer, r_diag):
@@ -119,12 +144,12 @@ def test_beamformer_damas_matches_oracle(gpu, steer, r_diag):
     csm = synthetic.analytic_csm(mics.pos, spos, synthetic.level_powers(2, 6.0, 8), freqs, dtype=np.complex128)
     ps = aa.PowerSpectraImport(csm=csm, frequencies=freqs)
     sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer)
-    bf = aa.BeamformerDamas(freq_data=ps, steer=sv, r_diag=r_diag, n_iter=50, form='plain_f32')
+    bf = aa.BeamformerDamas(freq_data=ps, steer=sv, r_diag=r_diag, n_iter=100)
     got = bf.result[:]
     y = B.beamformer_map(csm, freqs, grid.pos, mics.pos, steer, r_diag)
     r0 = B.steer_r0(grid.pos, np.zeros(3))
     rm = B.dist_mat(grid.pos, mics.pos)
     A = np.stack([D.psf(steer, r0, rm, 2 * np.pi * fr / 343.0, np.arange(grid.size)) for fr in freqs])
     want = y.copy()
-    cref.damas_gauss_seidel(A, y, want, 50, 1.0)
+    cref.damas_gauss_seidel(A, y, want, 100, 1.0)
     assert peak_rel_err(got, want) <= 1e-4
