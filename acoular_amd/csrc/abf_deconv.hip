@@ -11,12 +11,13 @@
 //
 // DAMAS (damasSolverGaussSeidel, acoular/fastFuncs.py:820-855): one 1024-thread workgroup per
 // frequency runs the relaxed Gauss-Seidel sweeps with the solution vector in LDS.  Rows go in
-// blocks of 64: all 16 waves first form, for the 64 rows of a block, the part of each row's
-// sum over columns outside the block (those columns hold this sweep's values before the block
-// and the last sweep's after it, exactly what the sequential sweep sees); one wave then walks
-// the 64 rows in order, adding the in-block columns with the values it has just updated, and
-// applies the relaxation and the positivity clip.  The update order is the reference's; only
-// the float64 summation order differs.
+// blocks of 64: all 16 waves first form, for the 64 rows of a block, each row's sum over the
+// columns that keep their current value while the block is updated (everything except the
+// in-block columns up to the diagonal: columns before the block already hold this sweep's
+// values, columns after it the last sweep's, exactly what the sequential sweep sees); one wave
+// then walks the 64 rows in order, lane r adding A[r][l] x_l for every row l < r as soon as x_l
+// is updated, and applies the relaxation and the positivity clip.  The update order is the
+// reference's; only the float64 summation order differs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -241,9 +242,9 @@ __device__ __forceinline__ double load_a(const void *A, int64_t idx)
 template <bool F64>
 __global__ __launch_bounds__(1024) void damas_kernel(DamasArgs a)
 {
-    extern __shared__ double s_x[];                 // [G]
-    __shared__ double s_blk[kDamasRB][kDamasRB + 1]; // in-block coefficients of the 64 rows
-    __shared__ double s_part[kDamasRB];             // out-of-block sums of the 64 rows
+    extern __shared__ double s_x[];                  // [G]
+    __shared__ double s_blk[kDamasRB][kDamasRB + 1]; // in-block coefficients (row r, column l)
+    __shared__ double s_part[kDamasRB];              // per row: sum over columns not yet updated
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t G = a.G;
     const int f = blockIdx.x;
@@ -257,23 +258,34 @@ __global__ __launch_bounds__(1024) void damas_kernel(DamasArgs a)
     for (int it = 0; it < a.n_iter; ++it) {
         for (int64_t i0 = 0; i0 < G; i0 += kDamasRB) {
             const int nb = (int)min<int64_t>(kDamasRB, G - i0);
-            const int64_t blk = i0 / 64;
-            // out-of-block sums: wave w takes rows w, w + 16, w + 32, w + 48 of the block
+            // 1. for rows r = wave + 16 q: the sum over every column except the in-block ones at
+            //    or left of the diagonal (those get this sweep's values in step 2), and the
+            //    in-block coefficients; 4 column chunks of 64 per step keep 16 loads in flight
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int64_t t = 0; t < nchunks; ++t) {
-                const int64_t j = t * 64 + lane;
-                const bool jok = j < G;
-                const double xj = jok ? s_x[j] : 0.0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = wave + kDamasWaves * q;
-                    if (r >= nb) continue;
-                    const double aij = jok ? load_a<F64>(a.A, abase + (i0 + r) * G + j) : 0.0;
-                    if (t == blk)
-                        s_blk[r][lane] = aij;
-                    else
-                        acc[q] = fma(aij, xj, acc[q]);
+            for (int q = 0; q < 4; ++q) {
+                const int r = wave + kDamasWaves * q;
+                const int64_t j = i0 + lane;
+                if (r < nb) s_blk[r][lane] = j < G ? load_a<F64>(a.A, abase + (i0 + r) * G + j) : 0.0;
+            }
+            for (int64_t t0 = 0; t0 < nchunks; t0 += 4) {
+                double av[4][4], xv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t j = (t0 + u) * 64 + lane;
+                    const bool jok = j < G;
+                    xv[u] = jok ? s_x[j] : 0.0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int r = wave + kDamasWaves * q;
+                        const bool ok = jok && r < nb && !(j >= i0 && j <= i0 + r);
+                        av[q][u] = ok ? load_a<F64>(a.A, abase + (i0 + r) * G + j) : 0.0;
+                    }
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = fma(av[q][u], xv[u], acc[q]);
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -282,17 +294,23 @@ __global__ __launch_bounds__(1024) void damas_kernel(DamasArgs a)
                 if (r < nb && lane == 0) s_part[r] = sacc;
             }
             __syncthreads();
+            // 2. one wave walks the rows in order; lane r keeps row r's running sum and adds
+            //    A[r][l] x_l as soon as row l < r has its new value, so no step needs a reduction
             if (wave == 0) {
-                double xb = lane < nb ? s_x[i0 + lane] : 0.0;
-                for (int r = 0; r < nb; ++r) {
-                    const double p = (lane != r && lane < nb) ? s_blk[r][lane] * xb : 0.0;
-                    const double s = s_part[r] + wave_sum(p);
-                    const double xi = __shfl(xb, r, 64);
-                    double v = (1.0 - w) * xi + w * (y[i0 + r] - s);
-                    v = v > 0.0 ? v : 0.0;
-                    if (lane == r) xb = v;
+                const bool mine = lane < nb;
+                double run = mine ? s_part[lane] : 0.0;
+                const double xo = mine ? s_x[i0 + lane] : 0.0;
+                const double yl = mine ? y[i0 + lane] : 0.0;
+                double xn = xo;
+                for (int l = 0; l < nb; ++l) {
+                    if (lane == l) {
+                        const double v = (1.0 - w) * xo + w * (yl - run);
+                        xn = v > 0.0 ? v : 0.0;
+                    }
+                    const double xl = __shfl(xn, l, 64);
+                    if (lane > l && mine) run = fma(s_blk[lane][l], xl, run);
                 }
-                if (lane < nb) s_x[i0 + lane] = xb;
+                if (mine) s_x[i0 + lane] = xn;
             }
             __syncthreads();
         }
