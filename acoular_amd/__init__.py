@@ -28,7 +28,8 @@ from .fbeamform import (
 )
 from .grids import Grid, ImportGrid, RectGrid, RectGrid3D
 from .microphones import MicGeom
-from .sources import TimeSamples
+from .signals import NoiseGenerator, SignalGenerator, SineGenerator, WNoiseGenerator
+from .sources import PointSource, SourceMixer, TimeSamples, UncorrelatedNoiseSource
 from .spectra import BaseSpectra, PowerSpectra, PowerSpectraImport
 
 __all__ = [
@@ -45,12 +46,19 @@ __all__ = [
     'LazyBfResult',
     'MicGeom',
     'PowerSpectra',
+    'NoiseGenerator',
+    'PointSource',
     'PointSpreadFunction',
     'PowerSpectraImport',
     'RectGrid',
     'RectGrid3D',
+    'SignalGenerator',
+    'SineGenerator',
+    'SourceMixer',
     'SteeringVector',
     'TimeSamples',
+    'UncorrelatedNoiseSource',
+    'WNoiseGenerator',
     'beamform_device',
     'beamform_eig_device',
     'beamform_f64_device',

@@ -159,23 +159,25 @@ class PowerSpectra(BaseSpectra):
 
     def _calc_csm_device(self):
         torch = _lib.torch()
-        data = self._time_data()
         bs = self.block_size
-        nch = data.shape[1]
-        starts = self.block_starts()
-        nb = int(starts.shape[0])
+        if hasattr(self.source, 'result_device'):
+            # sources synthesised on the GPU hand their samples over in HBM
+            tdev = self.source.result_device(bs).to(torch.float32).contiguous()
+        else:
+            tdev = _lib.to_device(np.ascontiguousarray(self._time_data(), dtype=np.float32))
+        n, nch = int(tdev.shape[0]), int(tdev.shape[1])
+        nb = max(0, (n - bs) // (bs // self.overlap_) + 1) if n >= bs else 0
         wind = self.window_(bs)
         weight = float(np.dot(wind, wind))
         scale = 2.0 / self.block_size / weight / self.num_blocks  # ZeroDivisionError as the reference
         nf = bs // 2 + 1
         csm = torch.empty((nf, nch, nch), dtype=torch.complex64, device=_lib.device())
-        tdev = _lib.to_device(np.ascontiguousarray(data, dtype=np.float32))
         wdev = _lib.to_device(wind.astype(np.float32))
         bpp = int(max(1, min(max(nb, 1), _PASS_BYTES // max(1, nf * nch * 8))))
         lib = _lib.lib()
         ws = _lib.workspace(lib.abf_welch_csm_workspace_size(nch, bs, bpp))
         _lib.check(
-            lib.abf_welch_csm(_lib.ptr(tdev), data.shape[0], nch, bs, bs // self.overlap_, nb, _lib.ptr(wdev),
+            lib.abf_welch_csm(_lib.ptr(tdev), n, nch, bs, bs // self.overlap_, nb, _lib.ptr(wdev),
                               float(scale), _lib.ptr(csm), bpp, _lib.ptr(ws), ws.numel(), _lib.stream_handle()),
             'PowerSpectra.csm')
         return csm

@@ -199,6 +199,29 @@ int abf_psf(const abf_geometry *geom, const double *wavenumber, int32_t num_freq
 int abf_damas(const void *psf, int32_t psf_f64, const double *dirty, double *x, int64_t num_grid, int32_t num_freqs,
               int32_t n_iter, double relax, void *stream);
 
+/* Microphone signals of a stationary point source (PointSource.result, sources.py:774-842, and
+ * _fill_mic_signal_block, sources.py:77-91): out[b][m] = signal[int(0.5 + ind_m(b))] / rm[m] with
+ * ind_m(0) = ind0[m] and ind_m(b+1) = ind_m(b) + up accumulated in float64 sample by sample as the
+ * reference does; negative indices wrap as in numpy (or give 0 where ind < 0 with
+ * zeros_prepadding); an index outside [-signal_len, signal_len) yields NaN.  signal is the
+ * upsampled source signal (float64), out [num_samples][num_mics] float64. */
+size_t abf_point_source_workspace_size(int32_t num_mics, int64_t num_samples);
+int abf_point_source(const double *signal, int64_t signal_len, const double *ind0, const double *rm, int32_t num_mics,
+                     int32_t up, int32_t zeros_prepadding, int64_t num_samples, double *out, void *workspace,
+                     size_t workspace_bytes, void *stream);
+
+/* Independent white noise per channel (UncorrelatedNoiseSource.result, sources.py:1911-1967, with
+ * WNoiseGenerator.signal, signals.py:255-269): channel c is rms * numpy's legacy
+ * RandomState(seeds[c]).standard_normal(num_samples) (MT19937 and the polar method), written to
+ * out [num_samples][num_channels] float64; identical to numpy's values except for about 1 % that
+ * differ by one ulp (the last bit of the float64 log). */
+int abf_white_noise(const uint32_t *seeds, int32_t num_channels, int64_t num_samples, double rms, double *out,
+                    void *stream);
+
+/* One step of SourceMixer.result (sources.py:2139-2185): acc = weight * src (first != 0) or
+ * acc += weight * src, n float64 values, each product and sum rounded separately. */
+int abf_mix(const double *src, double weight, int64_t n, int32_t first, double *acc, void *stream);
+
 /* _freqBeamformer_SpecificSteerVec_FullCSM / _CsmRemovedDiag (fastFuncs.py:414-476): explicit
  * steering vectors steer [G][M] complex64. out = q * norm, out_norm = sum |h|^2. */
 int abf_beamform_custom(const void *csm, const void *steer, int64_t num_grid, int32_t num_mics, int32_t r_diag,
