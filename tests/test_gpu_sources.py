@@ -71,7 +71,8 @@ def test_white_noise_matches_numpy(gpu, n):
     # the device float64 log and glibc's differ in the last bit for about 1 % of the pairs; those
     # normals move by one ulp, the others are identical
     np.testing.assert_allclose(got, want, rtol=0, atol=1.5e-15)
-    assert np.mean(got == want) >= 0.97
+    if n >= 1000:
+        assert np.mean(got == want) >= 0.97
     if n >= 6:
         np.testing.assert_allclose(got[:6, 0] / 0.7, [1.764052345967664, 0.4001572083672233, 0.9787379841057392,
                                                       2.240893199201458, 1.8675579901499675, -0.977277879876411],
