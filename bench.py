@@ -613,6 +613,102 @@ def damas_bench(args):
     }
 
 
+def _c1_scene(seed=1001):
+    """The 64-mic configuration's sources: 64 mics in a 1 m disk; 3 unit-variance white-noise
+    sources at uniform [-0.2, 0.2]^2, z = 0.3; independent noise of rms 0.1 at every microphone;
+    51.2 kHz, 1 s."""
+    import acoular_amd as aa
+    from acoular_amd import synthetic
+
+    fs, n, nm = 51200.0, 51200, 64
+    mics = aa.MicGeom(pos_total=synthetic.disk_array(nm, 1.0, seed))
+    spos = synthetic.source_positions(3, [-0.2, -0.2, 0.3], [0.2, 0.2, 0.3], seed + 1)
+    srcs = [aa.PointSource(signal=aa.WNoiseGenerator(sample_freq=fs, num_samples=n, seed=seed + 10 + i),
+                           loc=spos[:, i], mics=mics) for i in range(3)]
+    noise = aa.UncorrelatedNoiseSource(signal=aa.WNoiseGenerator(sample_freq=fs, num_samples=n, seed=seed + 100,
+                                                                 rms=0.1), mics=mics)
+    mixer = aa.SourceMixer(sources=[*srcs, noise])
+    grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.3, increment=0.01)
+    return mics, srcs, noise, mixer, grid
+
+
+def c1_bench(args):
+    """The reference's own CPU-runnable configuration end to end on the GPU: source synthesis
+    (three point sources + microphone noise, mixed), Welch CSM (block size 128, Hanning, 50 %:
+    799 blocks, 65 bins) and true-level r_diag maps on a 41 x 41 grid for every bin."""
+    import torch
+
+    import acoular_amd as aa
+    from oracle import beamform as B
+    from oracle import cref
+    from oracle import sources as OS
+    from oracle import spectra as OSp
+
+    mics, srcs, noise, mixer, grid = _c1_scene()
+    box = {}
+
+    def step():
+        ps = aa.PowerSpectra(source=mixer, block_size=128, window='Hanning', overlap='50%', precision='complex64')
+        bf = aa.BeamformerBase(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type='true level'),
+                               r_diag=True, precision='float32')
+        box['maps'] = bf.result[:]
+        box['csm'] = ps.csm_device
+
+    step()
+    torch.cuda.synchronize()
+    t_step = time_steps(step, max(1, args.steps), 1)
+    bins = 65 * grid.size
+    log(f'C1 end to end: {t_step * 1e3:.2f} ms -> {bins / t_step:.4g} map-bins/s')
+    # parts on the device
+    dev_data = mixer.result_device(128)
+    t_syn = event_time(lambda: mixer.result_device(128), 3)
+    log(f'  synthesis {t_syn * 1e3:.2f} ms')
+    err = None
+    if not args.no_compare:
+        data = dev_data.cpu().numpy()
+        csm = OSp.welch_csm(data, 128, 'Hanning', '50%')
+        freqs = np.arange(65) * 51200.0 / 128
+        sample = [10, 40]
+        ref = B.beamformer_map(csm, freqs, grid.pos, mics.pos, 'true level', True, ind=sample)
+        got = box['maps'][sample]
+        err = max(float(np.abs(got[i] - ref[i]).max() / np.abs(ref[i]).max()) for i in range(len(sample)))
+        log(f'max relative map error on bins {sample} (oracle Welch + beamformer on the same samples): {err:.2e}')
+    cpu = None
+    if not args.no_cpu:
+        # the oracle pipeline on the host: sources (numpy), Welch (numpy), beamformer (C + OpenMP)
+        t0 = time.perf_counter()
+        parts = []
+        for s in srcs:
+            rm = s.env.apparent_r(s.loc, s.mics.pos).reshape(1, -1)
+            ind0 = (-rm / s.env.c - s.start_t + s.start) * s.sample_freq * s.up
+            parts.append(OS.point_source(s.signal.usignal(s.up), rm, ind0, s.up, s.prepadding, s.num_samples, 128,
+                                         float(s.num_samples)))
+        parts.append(OS.white_noise(np.arange(64) + noise.signal.seed, noise.num_samples, noise.signal.rms))
+        data = OS.mix(parts, [1.0] * 4)
+        csm = OSp.welch_csm(data, 128, 'Hanning', '50%')
+        freqs = np.arange(65) * 51200.0 / 128
+        r0 = B.steer_r0(grid.pos, np.zeros(3))
+        rm = cref.dist_mat(grid.pos, mics.pos)
+        for b in range(65):
+            cref.beamformer_full_csm(csm[b], r0, rm, 2 * np.pi * freqs[b] / 343.0, 64 / 63.0, True, 3)
+        dt = time.perf_counter() - t0
+        cpu = {'value': bins / dt, 'unit': 'map-bins/s', 'cores': cref.num_threads(), 'kind': 'port',
+               'sample': f'the whole configuration once (synthesis and Welch in numpy, beamformer in C + OpenMP), '
+                         f'{dt:.1f} s'}
+        log(f"CPU pipeline: {dt:.2f} s -> {cpu['value']:.4g} map-bins/s")
+    return {
+        'metric': 'map-bins/s (grid pts x freqs) at <=1e-4 rel err vs FP64', 'value': bins / t_step,
+        'unit': 'map-bins/s', 'n_gpus': 1, 'ms_per_step': t_step * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64 synthesis, f32 CSM, bf16x3 maps',
+        'data': 'synthetic (seeded sources synthesised on the GPU)',
+        'config': {'workload': '64 mics (1 m disk), 3 white-noise point sources + mic noise rms 0.1, 51.2 kHz, 1 s; '
+                               'Welch bs 128 Hanning 50 % (799 blocks, 65 bins); RectGrid 41x41 z=0.3; true level, '
+                               'r_diag; synthesis + CSM + maps', 'num_mics': 64, 'num_grid': grid.size,
+                   'num_freqs': 65, 'parallelism': 'single GPU'},
+        'synthesis_ms': t_syn * 1e3, 'max_rel_err': err, 'cpu_baseline': cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -625,7 +721,7 @@ def main():
     ap.add_argument('--tune', action='store_true', help='tune the launch shape now even if the table has it')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig',
-                                                          'damas'],
+                                                          'damas', 'c1'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
                          "'serving': the mixed-size request loop (Poisson arrivals) against the unbatched "
                          "plain kernel; 'fused': Welch CSM + beamforming of 512 mics, 10 s at 51.2 kHz, "
@@ -667,6 +763,9 @@ def main():
         return
     if args.mode == 'damas':
         print(json.dumps(damas_bench(args)), flush=True)
+        return
+    if args.mode == 'c1':
+        print(json.dumps(c1_bench(args)), flush=True)
         return
     log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
     wl = build_workload(dev)
