@@ -703,15 +703,25 @@ int abf_beamform_eig_grid(const abf_geometry *geom, const double *wavenumber, in
     if (G == 0 || num_freqs == 0) return ABF_OK;
     if (!geom->gpos || !geom->mpos || !wavenumber || !out || !eigval || !eigvec)
         return fail(ABF_ERR_INVALID, "null pointer");
+    // many eigenvectors without r_diag: the fp32 MFMA kernel, 16-point column tiles, the
+    // steering tile [Mp][P] within 144 KiB
+    const int Mp4 = (M + 3) & ~3;
+    const bool mfma = !r_diag && num_eig >= 16 && (size_t)Mp4 * 16 * sizeof(float2) <= 147456;
     // grid points per workgroup: the steering tile [M][P] stays within 64 KiB where it can
     // (two or more workgroups per CU), within 144 KiB otherwise
     int P = 64;
-    while (P > 4 && (size_t)M * P * sizeof(float2) > 65536) P /= 2;
-    const size_t lds = (size_t)M * P * sizeof(float2);
+    if (mfma) {
+        // two workgroups per CU where the tile allows (64 KiB), one otherwise
+        while (P > 16 && (size_t)Mp4 * P * sizeof(float2) > 65536) P /= 2;
+    } else {
+        while (P > 4 && (size_t)M * P * sizeof(float2) > 65536) P /= 2;
+    }
+    const size_t lds = (size_t)(mfma ? Mp4 : M) * P * sizeof(float2);
     if (lds > 147456) return fail(ABF_ERR_UNSUPPORTED, "too many microphones for the eigenvalue form");
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)bf_eig_grid_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 147456);
+        (void)hipFuncSetAttribute((const void *)bf_eig_mfma_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 147456);
         attr = true;
     }
     EigGridArgs a{};
@@ -735,8 +745,12 @@ int abf_beamform_eig_grid(const abf_geometry *geom, const double *wavenumber, in
     a.phase_mode = phase_mode & kPhaseMask;
     a.fast_trig = (phase_mode & kFastTrigFlag) != 0;
     a.clip = clip_negative;
-    hipLaunchKernelGGL(bf_eig_grid_kernel, dim3((unsigned)cdiv(G, P), (unsigned)num_freqs), dim3(256), lds, S(stream),
-                       a);
+    if (mfma)
+        hipLaunchKernelGGL(bf_eig_mfma_kernel, dim3((unsigned)cdiv(G, P), (unsigned)num_freqs), dim3(256), lds,
+                           S(stream), a);
+    else
+        hipLaunchKernelGGL(bf_eig_grid_kernel, dim3((unsigned)cdiv(G, P), (unsigned)num_freqs), dim3(256), lds,
+                           S(stream), a);
     return hip_status("eigenvalue grid beamformer launch");
 }
 

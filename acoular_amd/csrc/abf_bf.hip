@@ -500,6 +500,113 @@ __global__ __launch_bounds__(256) void bf_eig_grid_kernel(EigGridArgs a)
     }
 }
 
+// The same eigenvalue form on fp32 MFMA when many eigenvectors are summed (BeamformerMusic,
+// BeamformerFunctional): Y = V_K^H H is a complex GEMM (K x M times M x P) and
+// q = sum_k l_k |Y_kg|^2.  One workgroup per (P grid points, frequency) generates H into LDS as
+// bf_eig_grid_kernel does; each wave then takes (16-eigenvector, 16-point) tiles, streams the
+// eigenvector fragments from L2 (lane l: eigenvector l % 16, microphone l / 16 of a 4-microphone
+// step, 64 contiguous bytes per 16 lanes) and the steering fragments from LDS, accumulates
+// Re Y and Im Y with four v_mfma_f32_16x16x4_f32 per step, and folds l_k |Y|^2 of its 16 rows
+// into per-point sums.  float32 products and sums as in the VALU kernel; without r_diag only
+// (the r_diag correction needs per-microphone weights of the whole range; that case stays on
+// the VALU kernel).
+__global__ __launch_bounds__(256) void bf_eig_mfma_kernel(EigGridArgs a)
+{
+    extern __shared__ float2 hx[]; // [Mp][P], microphones padded to a multiple of 4
+    __shared__ double s_red[256];
+    __shared__ double s_part[4][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int P = a.P;
+    const int Mp = (a.M + 3) & ~3;
+    const int f = blockIdx.y;
+    const int p = tid & (P - 1);
+    const int slot = tid / P, nslot = 256 / P;
+    const int64_t g = (int64_t)blockIdx.x * P + p;
+    const bool valid = g < a.G;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (valid) {
+        gx = a.gpos[g];
+        gy = a.gpos[a.G + g];
+        gz = a.gpos[2 * a.G + g];
+    }
+    const double k = a.wavenumber[f];
+    double help = 0.0;
+    for (int m = slot; m < Mp; m += nslot) {
+        float2 v = make_float2(0.f, 0.f);
+        if (valid && m < a.M) {
+            const double r = dist3(gx, gy, gz, a.mpos[m], a.mpos[a.M + m], a.mpos[2 * a.M + m]);
+            float c, s, w;
+            double h;
+            phasor_rev(k * r, a.phase_mode, a.fast_trig, c, s);
+            steer_weight(a.steer_type, r, w, h);
+            help += h;
+            v = make_float2(w * c, -w * s);
+        }
+        hx[m * P + p] = v;
+    }
+    s_red[tid] = help;
+    if (tid < 4 * 64) s_part[tid >> 6][tid & 63] = 0.0;
+    __syncthreads();
+    double hs = 0.0;
+    if (tid < P)
+        for (int i = 0; i < nslot; ++i) hs += s_red[tid + P * i];
+
+    const float2 *__restrict__ V = a.eigvec + (int64_t)f * a.M * a.M;
+    const double *__restrict__ lam = a.eigval + (int64_t)f * a.M;
+    const int nrt = (a.K + 15) / 16, nct = P / 16;
+    const int row = lane & 15, grp = lane >> 4;
+    for (int t = wave; t < nrt * nct; t += 4) {
+        const int rt = t / nct, ct = t % nct;
+        const int e = a.e0 + rt * 16 + row; // this lane's eigenvector in the A fragment
+        const bool eok = e < a.e0 + a.K;
+        const int ec = eok ? e : a.e0;
+        f4 accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
+        // four 4-microphone steps per pass: all loads of the pass are issued before its MFMAs
+        for (int m0 = 0; m0 < Mp; m0 += 16) {
+            float2 ev[4], hv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int m = m0 + 4 * u + grp;
+                ev[u] = make_float2(0.f, 0.f);
+                hv[u] = make_float2(0.f, 0.f);
+                if (m < Mp) {
+                    if (m < a.M && eok) ev[u] = V[(int64_t)m * a.M + ec];
+                    hv[u] = hx[m * P + ct * 16 + row];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                // Y = conj(V)^T H: Re = Vr Hr + Vi Hi, Im = Vr Hi - Vi Hr
+                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].x, hv[u].x, accr, 0, 0, 0);
+                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].y, hv[u].y, accr, 0, 0, 0);
+                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].x, hv[u].y, acci, 0, 0, 0);
+                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(-ev[u].y, hv[u].x, acci, 0, 0, 0);
+            }
+        }
+        // lane (col = l % 16, group) holds rows 4 group + i of the tile
+        double sq = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int er = a.e0 + rt * 16 + 4 * grp + i;
+            if (er < a.e0 + a.K) sq += ((double)accr[i] * accr[i] + (double)acci[i] * acci[i]) * lam[er];
+        }
+        sq += __shfl_xor(sq, 16, 64);
+        sq += __shfl_xor(sq, 32, 64);
+        if (grp == 0) s_part[wave][ct * 16 + row] += sq;
+    }
+    __syncthreads();
+    if (tid < P && valid) {
+        const double qs = s_part[0][tid] + s_part[1][tid] + s_part[2][tid] + s_part[3][tid];
+        const double r0 = a.ref_dist > 0.0 ? a.ref_dist : dist3(gx, gy, gz, a.ref[0], a.ref[1], a.ref[2]);
+        double res, nst;
+        normalise(a.steer_type, qs, r0, hs, (double)a.M, a.norm, res, nst);
+        float rf = (float)res;
+        if (a.clip && rf < 0.f) rf = 0.f;
+        a.out[(int64_t)f * a.G + g] = rf;
+        if (a.out_norm) a.out_norm[(int64_t)f * a.G + g] = (float)nst;
+    }
+}
+
 __global__ __launch_bounds__(256) void bf_custom_kernel(CustomArgs a)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

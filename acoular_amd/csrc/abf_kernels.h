@@ -107,6 +107,7 @@ __global__ void bf_mfma_kernel(const MfJob *jobs, int njobs);
 template <bool BF16X3, int VAR> __global__ void bf_mfma_ws_kernel(const MfJob *jobs, int njobs);
 __global__ void bf_eig_kernel(EigArgs a);
 __global__ void bf_eig_grid_kernel(EigGridArgs a);
+__global__ void bf_eig_mfma_kernel(EigGridArgs a);
 __global__ void bf_custom_kernel(CustomArgs a);
 
 } // namespace abf

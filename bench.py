@@ -469,6 +469,7 @@ def eig_bench(args):
         outs[name] = box['out']
         log(f"{name} (eigenvectors {e0}..{e0 + k - 1}): {t * 1e3:.2f} ms -> {bins / t:.4g} map-bins/s")
     err = None
+    music_errs = []
     sample_bins = [40, 300]
     if not args.no_compare:
         r0 = B.steer_r0(gpos, np.zeros(3))
@@ -481,8 +482,14 @@ def eig_bench(args):
             ref = np.where(ref < 0, 0.0, ref)
             got = outs['eig'][b].double().cpu().numpy()
             errs.append(float(np.abs(got - ref).max() / np.abs(ref).max()))
+            # the 255 smallest eigenpairs without r_diag (the fp32 MFMA kernel)
+            refm = B.freq_beamformer_eig(lam[:-1], V[:, :-1], r0, rm, 2 * np.pi * freqs[b] / 343.0, norm, False,
+                                         'classic')[0]
+            gotm = outs['music'][b].double().cpu().numpy()
+            music_errs.append(float(np.abs(gotm - refm).max() / np.abs(refm).max()))
         err = max(errs)
-        log(f'BeamformerEig max relative map error on bins {sample_bins}: {err:.2e}')
+        log(f'BeamformerEig max relative map error on bins {sample_bins}: {err:.2e}; '
+            f'255 smallest eigenpairs: {max(music_errs):.2e}')
     cpu = None
     if not args.no_cpu:
         # the oracle's eigenvalue form (numpy, one thread) on whole maps of the same workload
@@ -511,6 +518,7 @@ def eig_bench(args):
                    'num_mics': nm, 'num_grid': int(gpos.shape[1]), 'num_freqs': len(freqs),
                    'parallelism': 'single GPU'},
         'eigh_ms': t_eig * 1e3, 'forms': res, 'max_rel_err': err, 'max_rel_err_bins': sample_bins,
+        'max_rel_err_music': max(music_errs) if music_errs else None,
         'cpu_baseline': cpu,
     }
 
