@@ -15,6 +15,7 @@ from .fbeamform import (
     BeamformerEig,
     BeamformerFunctional,
     BeamformerMusic,
+    BeamformerOrth,
     LazyBfResult,
     L_p,
     PointSpreadFunction,
@@ -39,6 +40,7 @@ __all__ = [
     'BeamformerEig',
     'BeamformerFunctional',
     'BeamformerMusic',
+    'BeamformerOrth',
     'Environment',
     'Grid',
     'ImportGrid',

@@ -427,6 +427,64 @@ class BeamformerFunctional(BeamformerBase):
         self._fr[ind] = 1
 
 
+class BeamformerOrth(BeamformerBase):
+    """Orthogonal beamforming (``fbeamform.py:1334-1400``): every selected eigenvector of the CSM
+    is beamformed on its own (eigenvalue 1) and its eigenvalue / M is placed at the grid point
+    where that map peaks.  ``eva_list`` holds the eigenvector indices (numpy indexing, default
+    the largest); setting ``n`` selects the ``n`` largest.  The maps of one eigenvector for all
+    requested bins come from one ``abf_beamform_eig_grid`` launch.
+    """
+
+    _params = BeamformerBase._params + ('eva_list', 'n')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'eva_list', np.array([-1]))
+        object.__setattr__(self, 'n', 1)
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name == 'eva_list':
+            value = np.atleast_1d(np.asarray(value, dtype=int))
+        if name == 'n':
+            value = int(value)
+            super().__setattr__('n', value)
+            # the observer of the reference (fbeamform.py:1364-1367)
+            super().__setattr__('eva_list', np.arange(-1, -1 - value, -1))
+            return
+        super().__setattr__(name, value)
+
+    def _digest_items(self):
+        # n acts only through eva_list (fbeamform.py:1356-1358)
+        return tuple(getattr(self, p) for p in self._params if p != 'n')
+
+    def _calc(self, ind):
+        torch = _lib.torch()
+        ind = np.asarray(ind, dtype=np.int64)
+        nm = self.freq_data.num_channels
+        normfactor = self.sig_loss_norm()
+        eigvals = self.freq_data.eigvals
+        if type(self.steer) is not SteeringVector:
+            _, steer_func = self._beamformer_params()
+            for i in ind:
+                eigvecs = np.array(self.freq_data.eigvecs[i], dtype='complex128')
+                for n in self.eva_list:
+                    out = beamformerFreq('custom', self.r_diag, normfactor, steer_func(self._f[i]),
+                                         (np.ones(1), eigvecs[:, n].reshape((-1, 1))))[0]
+                    self._ac[i, out.argmax()] += np.float64(eigvals[i][n]) / nm
+                self._fr[i] = 1
+            return
+        _, evec = self.freq_data.eig_device()
+        ones = torch.ones((evec.shape[0], nm), dtype=torch.float64, device=evec.device)
+        for n in self.eva_list:
+            e = int(np.arange(nm)[n])  # numpy indexing: negative counts from the end, IndexError beyond
+            maps = beamform_eig_device(ones, evec, self._f[ind], ind, self.steer, self.steer.steer_type, self.r_diag,
+                                       normfactor, e, 1, self.phase_mode).cpu().numpy()
+            peak = maps.argmax(axis=1)
+            for row, i in enumerate(ind):
+                self._ac[i, peak[row]] += np.float64(eigvals[i][n]) / nm
+        self._fr[ind] = 1
+
+
 class PointSpreadFunction(HasParams):
     """The point spread function of a grid and microphone array (``fbeamform.py:948-1124``).
 

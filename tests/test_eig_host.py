@@ -54,3 +54,15 @@ def test_import_precision_parameter():
     assert aa.PowerSpectraImport(csm=csm, frequencies=1.0, precision='complex64').precision == 'complex64'
     with pytest.raises(ValueError):
         aa.PowerSpectraImport(csm=csm, frequencies=1.0, precision='float32')
+
+
+def test_orth_eigenvalue_selection():
+    bf = aa.BeamformerOrth(steer=_steer())
+    np.testing.assert_array_equal(bf.eva_list, [-1])
+    bf.n = 3
+    np.testing.assert_array_equal(bf.eva_list, [-1, -2, -3])
+    d = bf.digest
+    bf.eva_list = [0, 5]
+    np.testing.assert_array_equal(bf.eva_list, [0, 5])
+    assert bf.digest != d
+    np.testing.assert_array_equal(aa.BeamformerOrth(steer=_steer(), n=2).eva_list, [-1, -2])

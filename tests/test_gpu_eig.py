@@ -204,3 +204,26 @@ def test_beamformer_functional_matches_oracle(gpu, steer, gamma):
         base = aa.BeamformerBase(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type=steer),
                                  r_diag=False, form='plain_f32')
         assert peak_rel_err(got, base.result[:]) <= TOL
+
+
+@pytest.mark.parametrize('eva', [None, [-1, -3], [0, -2]])
+@pytest.mark.parametrize('r_diag', [True, False])
+def test_beamformer_orth_matches_oracle(gpu, eva, r_diag):
+    freqs, mics, grid, csm, ps = _bf_setup()
+    kw = {'n': 2} if eva is None else {'eva_list': eva}
+    bf = aa.BeamformerOrth(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type='true level'),
+                           r_diag=r_diag, **kw)
+    got = bf.result[:]
+    nm = mics.num_mics
+    norm = B.sig_loss_norm(r_diag, 0.0, nm)
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    want = np.zeros_like(got)
+    for f, fr in enumerate(freqs):
+        lam, V = np.linalg.eigh(csm[f])
+        for n in bf.eva_list:
+            m = B.freq_beamformer_eig(np.ones(1), V[:, n:n + 1] if n != -1 else V[:, -1:], r0, rm,
+                                      2 * np.pi * fr / 343.0, norm, r_diag, 'true level')[0]
+            want[f, m.argmax()] += lam[n] / nm
+    np.testing.assert_array_equal(got != 0, want != 0)
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=0)
