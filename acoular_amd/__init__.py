@@ -11,6 +11,7 @@ from .environments import Environment, dist_mat
 from .fastFuncs import beamformerFreq, calcCSM, calcTransfer
 from .fbeamform import (
     BeamformerBase,
+    BeamformerCapon,
     BeamformerDamas,
     BeamformerEig,
     BeamformerFunctional,
@@ -36,6 +37,7 @@ from .spectra import BaseSpectra, PowerSpectra, PowerSpectraImport
 __all__ = [
     'BaseSpectra',
     'BeamformerBase',
+    'BeamformerCapon',
     'BeamformerDamas',
     'BeamformerEig',
     'BeamformerFunctional',

@@ -303,6 +303,36 @@ class BeamformerBase(HasParams):
         return h
 
 
+class BeamformerCapon(BeamformerBase):
+    """Capon (minimum variance) beamforming (``fbeamform.py:777-818``): ``1 / (h^H C^-1 h)`` with
+    normalisation M^2.  The reference inverts every CSM slice and runs the full-CSM form on the
+    inverse; here the inverse enters through the resident eigendecomposition, h^H C^-1 h =
+    sum_k |v_k^H h|^2 / l_k, one ``abf_beamform_eig_grid`` launch over all eigenpairs (fp32 MFMA)
+    for all requested bins.  Only defined for the full CSM.
+    """
+
+    _fixed = {'r_diag': False, 'r_diag_norm': 1.0}
+
+    def _calc(self, ind):
+        ind = np.asarray(ind, dtype=np.int64)
+        nm = self.freq_data.num_channels
+        normfactor = self.sig_loss_norm() * nm**2
+        if type(self.steer) is not SteeringVector:
+            import scipy.linalg as spla
+
+            _, steer_func = self._beamformer_params()
+            for i in ind:
+                csm = np.array(spla.inv(np.array(self.freq_data.csm[i], dtype='complex128')), order='C')
+                self._ac[i] = 1.0 / beamformerFreq('custom', self.r_diag, normfactor, steer_func(self._f[i]), csm)[0]
+                self._fr[i] = 1
+            return
+        ev, evec = self.freq_data.eig_device()
+        maps = beamform_eig_device(1.0 / ev, evec, self._f[ind], ind, self.steer, self.steer.steer_type, False,
+                                   normfactor, 0, nm, self.phase_mode)
+        self._ac[ind] = (1.0 / maps.double()).cpu().numpy().astype(self.precision)
+        self._fr[ind] = 1
+
+
 class BeamformerEig(BeamformerBase):
     """Beamforming with one eigenvalue/eigenvector pair of the CSM (``fbeamform.py:821-888``):
     ``n`` selects the pair (negative values count from the largest eigenvalue, default -1).

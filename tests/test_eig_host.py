@@ -28,7 +28,8 @@ def test_defaults_and_fixed_parameters():
     assert aa.BeamformerMusic(steer=_steer(), n=3).n == 3
     func = aa.BeamformerFunctional(steer=_steer())
     assert func.gamma == 1.0 and func.r_diag is False and func.r_diag_norm == 1.0
-    for cls in (aa.BeamformerMusic, aa.BeamformerFunctional):
+    assert aa.BeamformerCapon(steer=_steer()).r_diag is False
+    for cls in (aa.BeamformerMusic, aa.BeamformerFunctional, aa.BeamformerCapon):
         with pytest.raises(ValueError):
             cls(steer=_steer(), r_diag=True)
         with pytest.raises(ValueError):

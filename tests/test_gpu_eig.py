@@ -227,3 +227,18 @@ def test_beamformer_orth_matches_oracle(gpu, eva, r_diag):
             want[f, m.argmax()] += lam[n] / nm
     np.testing.assert_array_equal(got != 0, want != 0)
     np.testing.assert_allclose(got, want, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize('steer', STEERS)
+def test_beamformer_capon_matches_oracle(gpu, steer):
+    """The reference inverts the CSM and runs the full-CSM form on the inverse; the GPU path goes
+    through the eigendecomposition.  Both give 1 / (h^H C^-1 h) for the well-conditioned CSM."""
+    freqs, mics, grid, csm, ps = _bf_setup()
+    bf = aa.BeamformerCapon(freq_data=ps, steer=aa.SteeringVector(grid=grid, mics=mics, steer_type=steer))
+    got = bf.result[:]
+    nm = mics.num_mics
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    want = np.array([1.0 / B.freq_beamformer_full_csm(np.linalg.inv(csm[f]), r0, rm, 2 * np.pi * fr / 343.0,
+                                                      float(nm) ** 2, False, steer)[0] for f, fr in enumerate(freqs)])
+    assert peak_rel_err(got, want) <= TOL
