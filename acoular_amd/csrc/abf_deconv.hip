@@ -223,11 +223,16 @@ struct DamasArgs {
     double relax;
 };
 
-__device__ __forceinline__ double wave_sum(double v)
+// wave-wide sum through the device library's DPP reduction (no LDS round trips)
+extern "C" __device__ double __ockl_wfred_add_f64(double);
+__device__ __forceinline__ double wave_sum(double v) { return __ockl_wfred_add_f64(v); }
+
+// lane l's value broadcast to the wave (l wave-uniform): two v_readlane, no LDS round trip
+__device__ __forceinline__ double lane_value(double v, int l)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l), hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 template <bool F64>
@@ -260,32 +265,46 @@ __global__ __launch_bounds__(1024) void damas_kernel(DamasArgs a)
             const int nb = (int)min<int64_t>(kDamasRB, G - i0);
             // 1. for rows r = wave + 16 q: the sum over every column except the in-block ones at
             //    or left of the diagonal (those get this sweep's values in step 2), and the
-            //    in-block coefficients; 4 column chunks of 64 per step keep 16 loads in flight
+            //    in-block coefficients; 4 column chunks of 64 per step keep 16 loads in flight.
+            //    Loads are unconditional (row and column indices clamped into the matrix) and
+            //    their values masked afterwards, so all of them are in flight before the first wait
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            {
+                const int64_t j = i0 + lane, jc = min(j, G - 1);
+                double bv[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = wave + kDamasWaves * q;
-                const int64_t j = i0 + lane;
-                if (r < nb) s_blk[r][lane] = j < G ? load_a<F64>(a.A, abase + (i0 + r) * G + j) : 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    const int rr = min(wave + kDamasWaves * q, nb - 1);
+                    bv[q] = load_a<F64>(a.A, abase + (i0 + rr) * G + jc);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = wave + kDamasWaves * q;
+                    if (r < nb) s_blk[r][lane] = j < G ? bv[q] : 0.0;
+                }
             }
             for (int64_t t0 = 0; t0 < nchunks; t0 += 4) {
                 double av[4][4], xv[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int64_t j = (t0 + u) * 64 + lane;
-                    const bool jok = j < G;
-                    xv[u] = jok ? s_x[j] : 0.0;
+                    const int64_t j = (t0 + u) * 64 + lane, jc = min(j, G - 1);
+                    xv[u] = s_x[jc];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const int r = wave + kDamasWaves * q;
-                        const bool ok = jok && r < nb && !(j >= i0 && j <= i0 + r);
-                        av[q][u] = ok ? load_a<F64>(a.A, abase + (i0 + r) * G + j) : 0.0;
+                        const int rr = min(wave + kDamasWaves * q, nb - 1);
+                        av[q][u] = load_a<F64>(a.A, abase + (i0 + rr) * G + jc);
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t j = (t0 + u) * 64 + lane;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[q] = fma(av[q][u], xv[u], acc[q]);
+                    for (int q = 0; q < 4; ++q) {
+                        const int r = wave + kDamasWaves * q;
+                        const bool ok = j < G && r < nb && !(j >= i0 && j <= i0 + r);
+                        acc[q] = fma(ok ? av[q][u] : 0.0, xv[u], acc[q]);
+                    }
+                }
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -302,13 +321,23 @@ __global__ __launch_bounds__(1024) void damas_kernel(DamasArgs a)
                 const double xo = mine ? s_x[i0 + lane] : 0.0;
                 const double yl = mine ? y[i0 + lane] : 0.0;
                 double xn = xo;
-                for (int l = 0; l < nb; ++l) {
-                    if (lane == l) {
-                        const double v = (1.0 - w) * xo + w * (yl - run);
-                        xn = v > 0.0 ? v : 0.0;
+                for (int l0 = 0; l0 < nb; l0 += 16) {
+                    // the row's next 16 coefficients, read from LDS ahead of the dependent steps
+                    double cf[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) cf[u] = s_blk[lane][l0 + u];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        const int l = l0 + u;
+                        if (l < nb) {
+                            if (lane == l) {
+                                const double v = (1.0 - w) * xo + w * (yl - run);
+                                xn = v > 0.0 ? v : 0.0;
+                            }
+                            const double xl = lane_value(xn, l);
+                            if (lane > l && mine) run = fma(cf[u], xl, run);
+                        }
                     }
-                    const double xl = __shfl(xn, l, 64);
-                    if (lane > l && mine) run = fma(s_blk[lane][l], xl, run);
                 }
                 if (mine) s_x[i0 + lane] = xn;
             }
