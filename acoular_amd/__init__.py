@@ -12,6 +12,7 @@ from .fastFuncs import beamformerFreq, calcCSM, calcTransfer
 from .fbeamform import (
     BeamformerBase,
     BeamformerCapon,
+    BeamformerCleansc,
     BeamformerDamas,
     BeamformerEig,
     BeamformerFunctional,
@@ -24,6 +25,7 @@ from .fbeamform import (
     beamform_device,
     beamform_eig_device,
     beamform_f64_device,
+    cleansc_device,
     damas_device,
     integrate,
     psf_device,
@@ -38,6 +40,7 @@ __all__ = [
     'BaseSpectra',
     'BeamformerBase',
     'BeamformerCapon',
+    'BeamformerCleansc',
     'BeamformerDamas',
     'BeamformerEig',
     'BeamformerFunctional',
@@ -66,6 +69,7 @@ __all__ = [
     'beamform_device',
     'beamform_eig_device',
     'beamform_f64_device',
+    'cleansc_device',
     'damas_device',
     'beamformerFreq',
     'calcCSM',

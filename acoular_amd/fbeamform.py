@@ -515,6 +515,44 @@ class BeamformerOrth(BeamformerBase):
         self._fr[ind] = 1
 
 
+class BeamformerCleansc(BeamformerBase):
+    """CLEAN-SC deconvolution (``fbeamform.py:1403-1492``): up to ``n_iter`` (0: 2 M) removals of
+    the coherent source component at the map's peak, damped by ``damp``, stopping when the peak
+    rises against the one ``stopn`` iterations earlier.  All requested bins iterate together on the
+    device (``abf_beamform_f64`` for the dirty maps, ``abf_cleansc`` for the iterations) in
+    float64, as the reference computes.
+    """
+
+    _params = BeamformerBase._params + ('n_iter', 'damp', 'stopn')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'n_iter', 0)
+        object.__setattr__(self, 'damp', 0.6)
+        object.__setattr__(self, 'stopn', 3)
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name in ('n_iter', 'stopn'):
+            value = int(value)
+        if name == 'damp':
+            value = float(value)
+            if not 0.01 <= value <= 1.0:
+                raise ValueError('damp must be in [0.01, 1.0]')
+        super().__setattr__(name, value)
+
+    def _calc(self, ind):
+        ind = np.asarray(ind, dtype=np.int64)
+        if type(self.steer) is not SteeringVector:
+            msg = 'BeamformerCleansc on the device needs a plain SteeringVector'
+            raise NotImplementedError(msg)
+        normfactor = self.sig_loss_norm()
+        csm = self.freq_data.csm_device_f64()
+        out = cleansc_device(csm, self._f[ind], ind, self.steer, self.steer.steer_type, self.r_diag, normfactor,
+                             self.n_iter, self.damp, self.stopn, self.phase_mode)
+        self._ac[ind] = out.cpu().numpy().astype(self.precision)
+        self._fr[ind] = 1
+
+
 class PointSpreadFunction(HasParams):
     """The point spread function of a grid and microphone array (``fbeamform.py:948-1124``).
 
@@ -745,6 +783,32 @@ def damas_device(psf, dirty, n_iter: int, damp: float):
     _lib.check(_lib.lib().abf_damas(_lib.ptr(psf), int(psf.dtype == torch.float64), _lib.ptr(dirty), _lib.ptr(x),
                                     G, F, int(n_iter), float(damp), _lib.stream_handle()), 'DAMAS')
     return x
+
+
+def cleansc_device(csm, freqs, ind, steer: SteeringVector, steer_type: str, r_diag: bool, norm: float, n_iter: int,
+                   damp: float, stopn: int, phase_mode: str = 'reference'):
+    """CLEAN-SC of the complex128 device CSM slices ``ind`` at the frequencies ``freqs``: the
+    float64 dirty maps (``abf_beamform_f64``) and the iterations of every bin together
+    (``abf_cleansc``, which works on a copy of the slices).  Returns the float32 result maps
+    ``[len(ind), G]`` as a device tensor."""
+    torch = _lib.torch()
+    ind = np.asarray(ind, dtype=np.int64)
+    dev = _lib.device()
+    if csm.dtype != torch.complex128:
+        raise TypeError('csm must be a complex128 device tensor')
+    work = csm.index_select(0, torch.from_numpy(ind).to(dev)).contiguous()
+    h = beamform_f64_device(work, freqs, np.arange(len(ind)), steer, steer_type, r_diag, norm, phase_mode)
+    gpos, mpos = steer.device_geometry()
+    F, M = len(ind), mpos.shape[1]
+    k = torch.from_numpy(2 * np.pi * np.asarray(freqs, dtype=np.float64) / steer.env.c).to(dev)
+    out = torch.empty((F, gpos.shape[1]), dtype=torch.float32, device=dev)
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.abf_cleansc_workspace_size(M, F, int(n_iter)))
+    _lib.check(lib.abf_cleansc(_geometry(gpos, mpos, steer.ref), _lib.ptr(k), F, _lib.ptr(work), _lib.ptr(h),
+                               _lib.STEER_CODES[steer_type], int(bool(r_diag)), float(norm),
+                               _lib.PHASE_MODES[phase_mode], int(n_iter), float(damp), int(stopn), _lib.ptr(out),
+                               _lib.ptr(ws), ws.numel(), _lib.stream_handle()), 'CLEAN-SC')
+    return out
 
 
 def _geometry(gpos, mpos, ref) -> _lib.AbfGeometry:

@@ -8,6 +8,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
   per-formulation weights and normalisations (classic ``:657-677``, inverse ``:693-717``, true
   level ``:733-759``, true location ``:775-801``).  Vectorised over grid points; one source
   column at a time.
+* ``clean_sc``: ``BeamformerCleansc._calc`` (``fbeamform.py:1449-1492``), the reference's
+  iteration restated on the oracle's beamformer forms.
 * ``damas_gauss_seidel``: ``damasSolverGaussSeidel`` (``fastFuncs.py:820-855``), the
   row-by-row relaxed Gauss-Seidel sweep with the positivity clip, one scalar loop over rows
   per iteration (the dot product of a row with the current solution is the only vectorised
@@ -66,3 +68,51 @@ def damas_gauss_seidel(A, dirty, n_iter, relax, x) -> np.ndarray:
             s = (1 - relax) * x[i] + relax * (y[i] - s)
             x[i] = s if s > 0.0 else 0.0
     return x
+
+
+def clean_sc(csm, freqs, gpos, mpos, steer_type='true level', r_diag=True, r_diag_norm=0.0, n_iter=0, damp=0.6,
+             stopn=3, c=343.0, ref=(0.0, 0.0, 0.0), phase_mode='reference'):
+    """``BeamformerCleansc._calc`` (``fbeamform.py:1449-1492``) over every slice of ``csm``:
+    float32 result maps ``[F, G]``.  The dirty map and the maps of the removed coherent
+    components use the oracle's full-CSM and eigenvalue forms (``fastFuncs.py:238-311,
+    335-411``); the steering vector at the peak is the transfer function with the steering
+    normalisation of ``SteeringVector.steer_vector`` (``fbeamform.py:145-153, 237-258``)."""
+    from .beamform import (calc_transfer, dist_mat, freq_beamformer_eig, freq_beamformer_full_csm, sig_loss_norm,
+                           steer_r0, steer_vector)
+
+    csm = np.asarray(csm)
+    nm = csm.shape[1]
+    r0 = steer_r0(gpos, ref if np.isscalar(ref) else np.asarray(ref, dtype=float))
+    rm = dist_mat(gpos, mpos)
+    normfactor = sig_loss_norm(r_diag, r_diag_norm, nm)
+    J = nm * 2 if not n_iter else n_iter
+    out = np.zeros((csm.shape[0], rm.shape[0]), dtype=np.float32)
+    for i, fr in enumerate(freqs):
+        k = 2 * np.pi * fr / c
+        C = np.array(csm[i], dtype='complex128', copy=True)
+        h = freq_beamformer_full_csm(C, r0, rm, k, normfactor, r_diag, steer_type, phase_mode)[0]
+        result = np.zeros(rm.shape[0], 'f')
+        powers = np.zeros(J, 'd')
+        for j in range(J):
+            xi_max = h.argmax()
+            powers[j] = hmax = h[xi_max]
+            result[xi_max] += damp * hmax
+            if j > stopn and hmax > powers[j - stopn]:
+                break
+            wmax = steer_vector(calc_transfer(r0[xi_max:xi_max + 1], rm[xi_max:xi_max + 1], k, phase_mode),
+                                steer_type) * np.sqrt(normfactor)
+            wmax = wmax[0].conj()
+            hh = wmax.copy()
+            D1 = np.dot(C.T - np.diag(np.diag(C)), wmax) / hmax
+            ww = wmax.conj() * wmax
+            for _ in range(20):
+                H = hh.conj() * hh
+                hh = (D1 + H * wmax) / np.sqrt(1 + np.dot(ww, H))
+            hh = hh[:, np.newaxis]
+            csm1 = hmax * (hh * hh.conj().T)
+            h1 = freq_beamformer_eig(np.array((hmax,)), hh.conj(), r0, rm, k, normfactor, r_diag, steer_type,
+                                     phase_mode)[0]
+            h -= damp * h1
+            C -= damp * csm1.T
+        out[i] = result
+    return out

@@ -1,10 +1,11 @@
-"""Host logic of PointSpreadFunction and BeamformerDamas (no GPU): parameters, digests and the
-index check that runs before any device work (fbeamform.py:948-1208)."""
+"""Host logic of PointSpreadFunction, BeamformerDamas and BeamformerCleansc (no GPU): parameters,
+digests, workspace sizes and the index check that runs before any device work (fbeamform.py:948-1492)."""
 
 import numpy as np
 import pytest
 
 import acoular_amd as aa
+from acoular_amd import _lib
 
 
 def _steer(nm=6):
@@ -56,3 +57,25 @@ def test_damas_parameters():
     assert bf.digest == d  # calcmode does not change the result
     bf.n_iter = 50
     assert bf.digest != d
+
+
+def test_cleansc_parameters():
+    bf = aa.BeamformerCleansc(steer=_steer())
+    assert bf.n_iter == 0 and bf.damp == 0.6 and bf.stopn == 3 and bf.r_diag is True
+    with pytest.raises(ValueError):
+        bf.damp = 0.0
+    with pytest.raises(ValueError):
+        bf.damp = 1.5
+    d = bf.digest
+    bf.stopn = 4
+    assert bf.digest != d
+    d = bf.digest
+    bf.damp = 0.5
+    assert bf.digest != d
+
+
+def test_cleansc_workspace_and_errors():
+    lib = _lib.lib()
+    assert lib.abf_cleansc_workspace_size(64, 10, 0) == 10 * 128 * 8 + 10 * 64 * 16 + 10 * 8 + 10 * 4 + 1024
+    assert lib.abf_cleansc_workspace_size(64, 10, 7) == 10 * 7 * 8 + 10 * 64 * 16 + 10 * 8 + 10 * 4 + 1024
+    assert lib.abf_cleansc_workspace_size(0, 10, 0) == 0

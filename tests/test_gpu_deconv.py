@@ -153,3 +153,45 @@ def test_beamformer_damas_matches_oracle(gpu, steer, r_diag):
     want = y.copy()
     cref.damas_gauss_seidel(A, y, want, 100, 1.0)
     assert peak_rel_err(got, want) <= 1e-4
+
+
+@pytest.mark.parametrize('steer', STEERS)
+@pytest.mark.parametrize('r_diag', [True, False])
+@pytest.mark.parametrize('n_iter', [0, 25])
+def test_beamformer_cleansc_matches_oracle(gpu, steer, r_diag, n_iter):
+    """CLEAN-SC (fbeamform.py:1449-1492) on three correlated-free sources plus uncorrelated noise:
+    same peaks in the same order (identical support) and values to 1e-5 of the map peak (float64
+    arithmetic on both sides, only the summation order differs)."""
+    mics, grid = _setup(40, 15, seed=9, inc=0.04)
+    spos = synthetic.source_positions(3, [-0.25, -0.25, 0.6], [0.25, 0.25, 0.6], 4)
+    freqs = np.array([2500.0, 5000.0, 11000.0])
+    csm = synthetic.analytic_csm(mics.pos, spos, synthetic.level_powers(3, 6.0, 5), freqs, dtype=np.complex128)
+    csm += 0.02 * np.eye(40)[None]
+    ps = aa.PowerSpectraImport(csm=csm, frequencies=freqs)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer)
+    bf = aa.BeamformerCleansc(freq_data=ps, steer=sv, r_diag=r_diag, n_iter=n_iter)
+    got = bf.result[:]
+    want = D.clean_sc(csm, freqs, grid.pos, mics.pos, steer, r_diag=r_diag, r_diag_norm=0.0, n_iter=n_iter)
+    assert got.shape == want.shape
+    np.testing.assert_array_equal(got != 0, want != 0)
+    assert peak_rel_err(got, want) <= 1e-5
+
+
+def test_cleansc_stop_rule_and_damping(gpu):
+    """Two sources on grid points with uncorrelated noise, several dampings and stop distances:
+    the automatic iteration count (2 M) runs unless the stop rule fires, and both sides stop at
+    the same iteration (the scene's peaks are separated well beyond rounding: perturbing the CSM
+    by 1e-11 leaves the oracle's support unchanged)."""
+    mics, grid = _setup(16, 9, seed=4, inc=0.05)
+    freqs = np.array([4000.0, 7000.0])
+    csm = synthetic.analytic_csm(mics.pos, grid.pos[:, [31, 57]], np.array([1.0, 0.3]), freqs, dtype=np.complex128)
+    csm += 0.05 * np.eye(16)[None]
+    ps = aa.PowerSpectraImport(csm=csm, frequencies=freqs)
+    sv = aa.SteeringVector(grid=grid, mics=mics)
+    for r_diag in (True, False):
+        for damp, stopn in ((0.1, 3), (0.9, 1), (0.6, 8)):
+            bf = aa.BeamformerCleansc(freq_data=ps, steer=sv, damp=damp, stopn=stopn, r_diag=r_diag)
+            want = D.clean_sc(csm, freqs, grid.pos, mics.pos, 'true level', r_diag=r_diag, damp=damp, stopn=stopn)
+            got = bf.result[:]
+            np.testing.assert_array_equal(got != 0, want != 0)
+            assert peak_rel_err(got, want) <= 1e-5

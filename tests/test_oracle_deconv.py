@@ -123,3 +123,28 @@ def test_damas_c_loop_equals_numpy():
         xn = y[f].copy()
         D.damas_gauss_seidel(A[f], y[f], 25, 0.9, xn)
         np.testing.assert_allclose(xc[f], xn, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize('steer_type', ['classic', 'inverse', 'true level', 'true location'])
+def test_clean_sc_single_source_is_removed_exactly(steer_type):
+    """One coherent monopole exactly on a grid point, beamformed with the diagonal removed: the
+    source component that CLEAN-SC solves for (``fbeamform.py:1474-1480``, Sijtsma 2007) is then
+    exactly the source's cross-spectral matrix, so each iteration finds the same peak and removes
+    the fraction ``damp`` of it; after n iterations the map holds h0 (1 - (1 - damp)^n) at the
+    source (h0 the dirty map's peak) and nothing elsewhere.  A transposed or unconjugated CSM in
+    the component estimate, or a missing normalisation, leaves a residue of several per cent."""
+    from acoular_amd import synthetic
+    from oracle.beamform import dist_mat, freq_beamformer_full_csm, sig_loss_norm, steer_r0
+
+    mpos = synthetic.disk_array(24, 1.0, 3)
+    x = np.linspace(-0.3, 0.3, 13)
+    gpos = np.stack([np.repeat(x, 13), np.tile(x, 13), np.full(169, 0.5)])
+    freqs = np.array([5000.0])
+    csm = synthetic.analytic_csm(mpos, gpos[:, 60:61], np.array([2.0]), freqs, dtype=np.complex128)
+    h0 = freq_beamformer_full_csm(csm[0], steer_r0(gpos, np.zeros(3)), dist_mat(gpos, mpos), 2 * np.pi * 5000.0 / 343.0,
+                                  sig_loss_norm(True, 0.0, 24), True, steer_type, 'fp64')[0]
+    assert np.argmax(h0) == 60
+    for n in (1, 2, 7, 30):
+        out = D.clean_sc(csm, freqs, gpos, mpos, steer_type, r_diag=True, n_iter=n, damp=0.6, phase_mode='fp64')[0]
+        np.testing.assert_allclose(out[60], h0[60] * (1 - 0.4**n), rtol=2e-6)
+        assert np.count_nonzero(out) == 1
