@@ -97,7 +97,7 @@ _SIGS = {
                                         ctypes.c_int32, ctypes.c_int64, _c_dp, _c_dp, ctypes.c_size_t, _c_dp]),
     'abf_white_noise': (ctypes.c_int, [_c_dp, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, _c_dp, _c_dp]),
     'abf_mix': (ctypes.c_int, [_c_dp, ctypes.c_double, ctypes.c_int64, ctypes.c_int32, _c_dp, _c_dp]),
-    'abf_cleansc_workspace_size': (ctypes.c_size_t, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    'abf_cleansc_workspace_size': (ctypes.c_size_t, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]),
     'abf_cleansc': (ctypes.c_int, [ctypes.POINTER(AbfGeometry), _c_dp, ctypes.c_int32, _c_dp, _c_dp, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                    ctypes.c_int32, _c_dp, _c_dp, ctypes.c_size_t, _c_dp]),

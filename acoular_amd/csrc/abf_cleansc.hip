@@ -24,6 +24,9 @@
 namespace abf {
 
 constexpr int kCscMaxM = 1024;
+constexpr int kCscThreads = 256;
+constexpr int kCscPer = kCscMaxM / kCscThreads; // microphones per thread in the selection kernel
+constexpr int kCscBatch = 8;                     // loads issued together before their first use
 
 struct CscArgs {
     const double *gpos;       // [3][G]
@@ -36,6 +39,8 @@ struct CscArgs {
     double2 *hh;              // [F][M]
     double *hmax;             // [F]
     int *stopped;             // [F]
+    double2 *rw;              // [M][G]: distance grid point -> mic, steering weight
+    double *help;             // [G]: the steer type's sum over the mics (fastFuncs.py:361-372)
     double ref[3];
     double ref_dist;
     double norm, damp;
@@ -49,17 +54,19 @@ __device__ __forceinline__ double csc_r0(const CscArgs &a, double gx, double gy,
     return a.ref_dist > 0.0 ? a.ref_dist : dist3(gx, gy, gz, a.ref[0], a.ref[1], a.ref[2]);
 }
 
-__device__ double block_sum(double v, double *red)
+extern "C" __device__ double __ockl_wfred_add_f64(double);
+
+// Sum over the 256 threads of a workgroup with one barrier: wave sums, then the four partials in
+// wave order.  red holds two sets of four slots used alternately (par flips on every call), so a
+// call never overwrites slots that a thread may still be reading from the previous call.
+__device__ __forceinline__ double block_sum(double v, double *red, int &par)
 {
-    const int tid = threadIdx.x;
+    v = __ockl_wfred_add_f64(v);
+    double *r = red + 4 * par;
+    par ^= 1;
+    if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = v;
     __syncthreads();
-    red[tid] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
-    }
-    return red[0];
+    return ((r[0] + r[1]) + r[2]) + r[3];
 }
 
 __global__ __launch_bounds__(256) void fill_zero_f32_kernel(float *p, int64_t n)
@@ -74,44 +81,86 @@ __global__ __launch_bounds__(256) void fill_zero_i32_kernel(int *p, int64_t n)
     if (i < n) p[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void csc_select_kernel(CscArgs a, int j)
+// Per call: distances and steering weights of every (mic, grid point) and the per-point sums of
+// the steer normalisation, in the microphone order of the reference's loops.
+__global__ __launch_bounds__(256) void csc_table_kernel(CscArgs a)
 {
-    __shared__ double s_val[256];
-    __shared__ int64_t s_idx[256];
-    __shared__ double s_red[256];
-    __shared__ double2 s_w[kCscMaxM], s_d1[kCscMaxM], s_hh[kCscMaxM];
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.G) return;
+    const double gx = a.gpos[g], gy = a.gpos[a.G + g], gz = a.gpos[2 * a.G + g];
+    const int M = a.M;
+    double help = 0.0;
+    for (int m = 0; m < M; ++m) {
+        const double r = dist3(gx, gy, gz, a.mpos[m], a.mpos[M + m], a.mpos[2 * M + m]);
+        double w = 1.0;
+        if (a.steer_type == ABF_STEER_INVERSE) {
+            w = r;
+            help += r * r;
+        } else if (a.steer_type != ABF_STEER_CLASSIC) {
+            w = 1.0 / r;
+            help += 1.0 / (r * r);
+        }
+        a.rw[(int64_t)m * a.G + g] = make_double2(r, w);
+    }
+    a.help[g] = help;
+}
+
+__global__ __launch_bounds__(kCscThreads) void csc_select_kernel(CscArgs a, int j)
+{
+    __shared__ double s_pv[4];
+    __shared__ int64_t s_pi[4];
+    __shared__ double s_red[8];
+    __shared__ double2 s_w[kCscMaxM], s_hh[kCscMaxM], s_part[kCscThreads];
     __shared__ int s_stop;
-    const int f = blockIdx.x, tid = threadIdx.x;
+    const int f = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int M = a.M;
     const int64_t G = a.G;
     if (a.stopped[f]) return;
-    // 1. the peak of h: largest value, first index on ties
+    int par = 0;
+    // 1. the peak of h: largest value, first index on ties (numpy's argmax)
     const double *h = a.h + (int64_t)f * G;
     double best = -INFINITY;
     int64_t bi = G;
-    for (int64_t g = tid; g < G; g += blockDim.x) {
-        const double v = h[g];
-        if (v > best) {
-            best = v;
-            bi = g;
+    for (int64_t base = tid; base < G; base += kCscBatch * kCscThreads) {
+        double v[kCscBatch];
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int64_t g = base + (int64_t)u * kCscThreads;
+            v[u] = h[g < G ? g : G - 1];
         }
-    }
-    s_val[tid] = best;
-    s_idx[tid] = bi;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            const double v = s_val[tid + s];
-            const int64_t i = s_idx[tid + s];
-            if (v > s_val[tid] || (v == s_val[tid] && i < s_idx[tid])) {
-                s_val[tid] = v;
-                s_idx[tid] = i;
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int64_t g = base + (int64_t)u * kCscThreads;
+            if (g < G && v[u] > best) {
+                best = v[u];
+                bi = g;
             }
         }
-        __syncthreads();
     }
-    const double hmax = s_val[0];
-    const int64_t xi = s_idx[0] < G ? s_idx[0] : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double v = __shfl_xor(best, off);
+        const int64_t i = __shfl_xor(bi, off);
+        if (v > best || (v == best && i < bi)) {
+            best = v;
+            bi = i;
+        }
+    }
+    if (lane == 0) {
+        s_pv[wv] = best;
+        s_pi[wv] = bi;
+    }
+    __syncthreads();
+    double hmax = s_pv[0];
+    int64_t xi = s_pi[0];
+    for (int q = 1; q < 4; ++q) {
+        const double v = s_pv[q];
+        const int64_t i = s_pi[q];
+        if (v > hmax || (v == hmax && i < xi)) {
+            hmax = v;
+            xi = i;
+        }
+    }
+    if (xi >= G) xi = 0;
     // 2. bookkeeping and the stop test (the break comes after the result update)
     if (tid == 0) {
         a.powers[(int64_t)f * a.J + j] = hmax;
@@ -126,91 +175,176 @@ __global__ __launch_bounds__(256) void csc_select_kernel(CscArgs a, int j)
     const double k = a.wavenumber[f];
     const double gx = a.gpos[xi], gy = a.gpos[G + xi], gz = a.gpos[2 * G + xi];
     const double r0 = csc_r0(a, gx, gy, gz);
+    double2 wr[kCscPer];
     double sq = 0.0;
-    for (int m = tid; m < M; m += blockDim.x) {
-        const double rm = dist3(gx, gy, gz, a.mpos[m], a.mpos[M + m], a.mpos[2 * M + m]);
-        float c, s;
-        phasor(k * (rm - r0), a.phase_mode, c, s);
-        const double ar = (double)c * r0 / rm, ai = -(double)s * r0 / rm;
-        s_w[m] = make_double2(ar, ai);
-        sq += ar * ar + ai * ai;
+#pragma unroll
+    for (int q = 0; q < kCscPer; ++q) {
+        const int m = tid + q * kCscThreads;
+        wr[q] = make_double2(0.0, 0.0);
+        if (m < M) {
+            const double rm = dist3(gx, gy, gz, a.mpos[m], a.mpos[M + m], a.mpos[2 * M + m]);
+            float c, s;
+            phasor(k * (rm - r0), a.phase_mode, c, s);
+            const double ar = (double)c * r0 / rm, ai = -(double)s * r0 / rm;
+            wr[q] = make_double2(ar, ai);
+            sq += ar * ar + ai * ai;
+        }
     }
-    const double S = block_sum(sq, s_red); // sum |a|^2 (true level / location)
+    const double S = block_sum(sq, s_red, par); // sum |a|^2 (true level / location)
     const double sn = sqrt(a.norm);
-    for (int m = tid; m < M; m += blockDim.x) {
-        const double2 x = s_w[m];
-        double wr, wi;
+#pragma unroll
+    for (int q = 0; q < kCscPer; ++q) {
+        const int m = tid + q * kCscThreads;
+        if (m >= M) continue;
+        const double2 x = wr[q];
+        double vr, vi;
         switch (a.steer_type) {
         case ABF_STEER_CLASSIC: {
             const double ab = hypot(x.x, x.y);
-            wr = x.x / ab / M;
-            wi = x.y / ab / M;
+            vr = x.x / ab / M;
+            vi = x.y / ab / M;
             break;
         }
         case ABF_STEER_INVERSE: { // 1 / conj(x) / M
             const double d = x.x * x.x + x.y * x.y;
-            wr = x.x / d / M;
-            wi = x.y / d / M;
+            vr = x.x / d / M;
+            vi = x.y / d / M;
             break;
         }
         case ABF_STEER_TRUE_LEVEL:
-            wr = x.x / S;
-            wi = x.y / S;
+            vr = x.x / S;
+            vi = x.y / S;
             break;
         default: {
             const double d = sqrt(S * M);
-            wr = x.x / d;
-            wi = x.y / d;
+            vr = x.x / d;
+            vi = x.y / d;
             break;
         }
         }
         // wmax = conj(steer * sqrt(normfactor))
-        s_w[m] = make_double2(wr * sn, -(wi * sn));
+        wr[q] = make_double2(vr * sn, -(vi * sn));
+        s_w[m] = wr[q];
     }
     __syncthreads();
-    // 4. D1 = (C^T - diag C) w / hmax: D1_a = sum_{b != a} C[b][a] w_b / hmax
+    // 4. D1 = (C^T - diag C) w / hmax: D1_a = sum_{b != a} C[b][a] w_b / hmax.  Up to 256
+    //    microphones the rows are split over kCscThreads / M groups of threads, whose partial
+    //    sums are added in group order; the loads go out kCscBatch at a time.
     const double2 *C = a.csm + (int64_t)f * M * M;
-    for (int am = tid; am < M; am += blockDim.x) {
-        double dr = 0.0, di = 0.0;
-        for (int b = 0; b < M; ++b) {
-            if (b == am) continue;
-            const double2 c = C[(int64_t)b * M + am];
-            const double2 w = s_w[b];
-            dr += c.x * w.x - c.y * w.y;
-            di += c.x * w.y + c.y * w.x;
-        }
-        s_d1[am] = make_double2(dr / hmax, di / hmax);
-        s_hh[am] = s_w[am];
+    double2 d1[kCscPer], hr[kCscPer];
+#pragma unroll
+    for (int q = 0; q < kCscPer; ++q) {
+        d1[q] = make_double2(0.0, 0.0);
+        hr[q] = wr[q];
     }
-    // 5. hh <- (D1 + |hh|^2 w) / sqrt(1 + sum |w|^2 |hh|^2), 20 times
+    if (M <= kCscThreads) {
+        const int groups = kCscThreads / M, am = tid % M, grp = tid / M;
+        double dr = 0.0, di = 0.0;
+        if (grp < groups) {
+            for (int b0 = grp; b0 < M; b0 += kCscBatch * groups) {
+                double2 c[kCscBatch];
+#pragma unroll
+                for (int u = 0; u < kCscBatch; ++u) {
+                    const int b = b0 + u * groups;
+                    c[u] = C[(int64_t)(b < M ? b : M - 1) * M + am];
+                }
+#pragma unroll
+                for (int u = 0; u < kCscBatch; ++u) {
+                    const int b = b0 + u * groups;
+                    if (b < M && b != am) {
+                        const double2 w = s_w[b];
+                        dr += c[u].x * w.x - c[u].y * w.y;
+                        di += c[u].x * w.y + c[u].y * w.x;
+                    }
+                }
+            }
+            s_part[tid] = make_double2(dr, di);
+        }
+        __syncthreads();
+        if (tid < M) {
+            double sr = 0.0, si = 0.0;
+            for (int q = 0; q < groups; ++q) {
+                const double2 p = s_part[q * M + tid];
+                sr += p.x;
+                si += p.y;
+            }
+            d1[0] = make_double2(sr / hmax, si / hmax);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kCscPer; ++q) {
+            const int am = tid + q * kCscThreads;
+            if (am >= M) continue;
+            double dr = 0.0, di = 0.0;
+            for (int b0 = 0; b0 < M; b0 += kCscBatch) {
+                double2 c[kCscBatch];
+#pragma unroll
+                for (int u = 0; u < kCscBatch; ++u) {
+                    const int b = b0 + u;
+                    c[u] = C[(int64_t)(b < M ? b : M - 1) * M + am];
+                }
+#pragma unroll
+                for (int u = 0; u < kCscBatch; ++u) {
+                    const int b = b0 + u;
+                    if (b < M && b != am) {
+                        const double2 w = s_w[b];
+                        dr += c[u].x * w.x - c[u].y * w.y;
+                        di += c[u].x * w.y + c[u].y * w.x;
+                    }
+                }
+            }
+            d1[q] = make_double2(dr / hmax, di / hmax);
+        }
+    }
+    // 5. hh <- (D1 + |hh|^2 w) / sqrt(1 + sum |w|^2 |hh|^2), 20 times; each thread keeps its own
+    //    microphones in registers, only the sum is shared
     for (int it = 0; it < 20; ++it) {
         double part = 0.0;
-        __syncthreads();
-        for (int m = tid; m < M; m += blockDim.x) {
-            const double2 w = s_w[m], q = s_hh[m];
-            part += (w.x * w.x + w.y * w.y) * (q.x * q.x + q.y * q.y);
+#pragma unroll
+        for (int q = 0; q < kCscPer; ++q) {
+            const double2 w = wr[q], x = hr[q];
+            part += (w.x * w.x + w.y * w.y) * (x.x * x.x + x.y * x.y);
         }
-        const double den = sqrt(1.0 + block_sum(part, s_red));
-        for (int m = tid; m < M; m += blockDim.x) {
-            const double2 w = s_w[m], q = s_hh[m], d = s_d1[m];
-            const double H = q.x * q.x + q.y * q.y;
-            s_hh[m] = make_double2((d.x + H * w.x) / den, (d.y + H * w.y) / den);
+        const double den = sqrt(1.0 + block_sum(part, s_red, par));
+#pragma unroll
+        for (int q = 0; q < kCscPer; ++q) {
+            const double2 w = wr[q], x = hr[q], d = d1[q];
+            const double H = x.x * x.x + x.y * x.y;
+            hr[q] = make_double2((d.x + H * w.x) / den, (d.y + H * w.y) / den);
         }
     }
-    __syncthreads();
     // 6. hand hh and hmax to the map kernel; C -= damp * (hmax hh hh^H)^T
-    for (int m = tid; m < M; m += blockDim.x) a.hh[(int64_t)f * M + m] = s_hh[m];
+#pragma unroll
+    for (int q = 0; q < kCscPer; ++q) {
+        const int m = tid + q * kCscThreads;
+        if (m < M) {
+            s_hh[m] = hr[q];
+            a.hh[(int64_t)f * M + m] = hr[q];
+        }
+    }
     if (tid == 0) a.hmax[f] = hmax;
+    __syncthreads();
     double2 *Cw = a.csm + (int64_t)f * M * M;
-    for (int64_t idx = tid; idx < (int64_t)M * M; idx += blockDim.x) {
-        const int r = (int)(idx / M), cidx = (int)(idx % M);
-        const double2 hb = s_hh[cidx], ha = s_hh[r];
-        // (hmax hh hh^H)[b][a] = hmax hh_b conj(hh_a)
-        const double pr = hb.x * ha.x + hb.y * ha.y, pi = hb.y * ha.x - hb.x * ha.y;
-        double2 v = Cw[idx];
-        v.x -= a.damp * (hmax * pr);
-        v.y -= a.damp * (hmax * pi);
-        Cw[idx] = v;
+    const int64_t MM = (int64_t)M * M;
+    for (int64_t base = tid; base < MM; base += kCscBatch * kCscThreads) {
+        double2 v[kCscBatch];
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int64_t idx = base + (int64_t)u * kCscThreads;
+            v[u] = Cw[idx < MM ? idx : MM - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int64_t idx = base + (int64_t)u * kCscThreads;
+            if (idx >= MM) continue;
+            const int r = (int)(idx / M), cidx = (int)(idx % M);
+            const double2 hb = s_hh[cidx], ha = s_hh[r];
+            // (hmax hh hh^H)[b][a] = hmax hh_b conj(hh_a)
+            const double pr = hb.x * ha.x + hb.y * ha.y, pi = hb.y * ha.x - hb.x * ha.y;
+            v[u].x -= a.damp * (hmax * pr);
+            v[u].y -= a.damp * (hmax * pi);
+            Cw[idx] = v[u];
+        }
     }
 }
 
@@ -224,35 +358,57 @@ __global__ __launch_bounds__(256) void csc_map_kernel(CscArgs a)
     __syncthreads();
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.G) return;
-    const double gx = a.gpos[g], gy = a.gpos[a.G + g], gz = a.gpos[2 * a.G + g];
     const double k = a.wavenumber[f];
-    double tr = 0.0, ti = 0.0, ds = 0.0, help = 0.0;
-    for (int m = 0; m < M; ++m) {
-        const double r = dist3(gx, gy, gz, a.mpos[m], a.mpos[M + m], a.mpos[2 * M + m]);
-        float c, s;
-        phasor(k * r, a.phase_mode, c, s);
-        double w = 1.0;
-        if (a.steer_type == ABF_STEER_INVERSE) {
-            w = r;
-            help += r * r;
-        } else if (a.steer_type != ABF_STEER_CLASSIC) {
-            w = 1.0 / r;
-            help += 1.0 / (r * r);
+    double tr = 0.0, ti = 0.0, ds = 0.0;
+    const double2 *rw = a.rw + g;
+    for (int m0 = 0; m0 < M; m0 += kCscBatch) {
+        double2 t[kCscBatch];
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int m = m0 + u;
+            t[u] = rw[(int64_t)(m < M ? m : M - 1) * a.G];
         }
-        const double vr = w * c, vi = -w * s;
-        // conj(E_m) v_m with E = conj(hh): hh_m v_m
-        const double2 q = s_hh[m];
-        const double pr = q.x * vr - q.y * vi, pi = q.x * vi + q.y * vr;
-        tr += pr;
-        ti += pi;
-        ds += (q.x * q.x + q.y * q.y) * (vr * vr + vi * vi);
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int m = m0 + u; // uniform over the workgroup
+            if (m >= M) break;
+            float c, s;
+            phasor(k * t[u].x, a.phase_mode, c, s);
+            const double vr = t[u].y * c, vi = -t[u].y * s;
+            // conj(E_m) v_m with E = conj(hh): hh_m v_m
+            const double2 q = s_hh[m];
+            const double pr = q.x * vr - q.y * vi, pi = q.x * vi + q.y * vr;
+            tr += pr;
+            ti += pi;
+            ds += (q.x * q.x + q.y * q.y) * (vr * vr + vi * vi);
+        }
     }
     double full = tr * tr + ti * ti;
     if (a.r_diag) full -= ds;
     const double q = full * a.hmax[f];
+    const double gx = a.gpos[g], gy = a.gpos[a.G + g], gz = a.gpos[2 * a.G + g];
     double res, nst;
-    normalise(a.steer_type, q, csc_r0(a, gx, gy, gz), help, (double)M, a.norm, res, nst);
+    normalise(a.steer_type, q, csc_r0(a, gx, gy, gz), a.help[g], (double)M, a.norm, res, nst);
     a.h[(int64_t)f * a.G + g] -= a.damp * res;
+}
+
+// Workspace layout: every array starts on a 256-byte boundary.
+struct CscLayout {
+    size_t powers, hh, hmax, stopped, rw, help, total;
+};
+
+inline CscLayout csc_layout(int M, int64_t G, int F, int J)
+{
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    CscLayout l{};
+    l.powers = 0;
+    l.hh = up(l.powers + (size_t)F * J * sizeof(double));
+    l.hmax = up(l.hh + (size_t)F * M * sizeof(double2));
+    l.stopped = up(l.hmax + (size_t)F * sizeof(double));
+    l.rw = up(l.stopped + (size_t)F * sizeof(int));
+    l.help = up(l.rw + (size_t)M * G * sizeof(double2));
+    l.total = up(l.help + (size_t)G * sizeof(double));
+    return l;
 }
 
 } // namespace abf
@@ -261,12 +417,10 @@ using namespace abf;
 
 extern "C" {
 
-size_t abf_cleansc_workspace_size(int32_t num_mics, int32_t num_freqs, int32_t n_iter)
+size_t abf_cleansc_workspace_size(int32_t num_mics, int64_t num_grid, int32_t num_freqs, int32_t n_iter)
 {
-    if (num_mics < 1 || num_freqs < 0 || n_iter < 0) return 0;
-    const int J = n_iter ? n_iter : 2 * num_mics;
-    const size_t F = (size_t)num_freqs;
-    return F * J * sizeof(double) + F * num_mics * sizeof(double2) + F * sizeof(double) + F * sizeof(int) + 1024;
+    if (num_mics < 1 || num_grid < 0 || num_freqs < 0 || n_iter < 0) return 0;
+    return csc_layout(num_mics, num_grid, num_freqs, n_iter ? n_iter : 2 * num_mics).total;
 }
 
 int abf_cleansc(const abf_geometry *geom, const double *wavenumber, int32_t num_freqs, void *csm, double *h,
@@ -286,7 +440,7 @@ int abf_cleansc(const abf_geometry *geom, const double *wavenumber, int32_t num_
     if (G == 0 || num_freqs == 0) return ABF_OK;
     if (!geom->gpos || !geom->mpos || !wavenumber || !csm || !h || !result)
         return set_error(ABF_ERR_INVALID, "null pointer");
-    const size_t need = abf_cleansc_workspace_size(M, num_freqs, n_iter);
+    const size_t need = abf_cleansc_workspace_size(M, G, num_freqs, n_iter);
     if (!workspace || workspace_bytes < need) return set_error(ABF_ERR_WORKSPACE, "workspace too small");
     const int J = n_iter ? n_iter : 2 * M;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -298,10 +452,13 @@ int abf_cleansc(const abf_geometry *geom, const double *wavenumber, int32_t num_
     a.csm = static_cast<double2 *>(csm);
     a.h = h;
     a.result = result;
-    a.powers = reinterpret_cast<double *>(ws);
-    a.hh = reinterpret_cast<double2 *>(ws + (size_t)num_freqs * J * sizeof(double));
-    a.hmax = reinterpret_cast<double *>(reinterpret_cast<char *>(a.hh) + (size_t)num_freqs * M * sizeof(double2));
-    a.stopped = reinterpret_cast<int *>(reinterpret_cast<char *>(a.hmax) + (size_t)num_freqs * sizeof(double));
+    const CscLayout lay = csc_layout(M, G, num_freqs, J);
+    a.powers = reinterpret_cast<double *>(ws + lay.powers);
+    a.hh = reinterpret_cast<double2 *>(ws + lay.hh);
+    a.hmax = reinterpret_cast<double *>(ws + lay.hmax);
+    a.stopped = reinterpret_cast<int *>(ws + lay.stopped);
+    a.rw = reinterpret_cast<double2 *>(ws + lay.rw);
+    a.help = reinterpret_cast<double *>(ws + lay.help);
     for (int i = 0; i < 3; ++i) a.ref[i] = geom->ref[i];
     a.ref_dist = geom->ref_dist;
     a.norm = norm_factor;
@@ -319,8 +476,9 @@ int abf_cleansc(const abf_geometry *geom, const double *wavenumber, int32_t num_
                        G * num_freqs);
     hipLaunchKernelGGL(fill_zero_i32_kernel, dim3((unsigned)((num_freqs + 255) / 256)), dim3(256), 0, s, a.stopped,
                        (int64_t)num_freqs);
+    hipLaunchKernelGGL(csc_table_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, a);
     for (int j = 0; j < J; ++j) {
-        hipLaunchKernelGGL(csc_select_kernel, dim3((unsigned)num_freqs), dim3(256), 0, s, a, j);
+        hipLaunchKernelGGL(csc_select_kernel, dim3((unsigned)num_freqs), dim3(kCscThreads), 0, s, a, j);
         hipLaunchKernelGGL(csc_map_kernel, dim3((unsigned)((G + 255) / 256), (unsigned)num_freqs), dim3(256), 0, s, a);
     }
     const hipError_t e = hipGetLastError();

@@ -803,7 +803,7 @@ def cleansc_device(csm, freqs, ind, steer: SteeringVector, steer_type: str, r_di
     k = torch.from_numpy(2 * np.pi * np.asarray(freqs, dtype=np.float64) / steer.env.c).to(dev)
     out = torch.empty((F, gpos.shape[1]), dtype=torch.float32, device=dev)
     lib = _lib.lib()
-    ws = _lib.workspace(lib.abf_cleansc_workspace_size(M, F, int(n_iter)))
+    ws = _lib.workspace(lib.abf_cleansc_workspace_size(M, gpos.shape[1], F, int(n_iter)))
     _lib.check(lib.abf_cleansc(_geometry(gpos, mpos, steer.ref), _lib.ptr(k), F, _lib.ptr(work), _lib.ptr(h),
                                _lib.STEER_CODES[steer_type], int(bool(r_diag)), float(norm),
                                _lib.PHASE_MODES[phase_mode], int(n_iter), float(damp), int(stopn), _lib.ptr(out),

@@ -227,7 +227,7 @@ int abf_mix(const double *src, double weight, int64_t n, int32_t first, double *
  * it), h [F][G] holds the dirty maps (abf_beamform_f64 with the same steering, r_diag and
  * norm_factor) and is updated in place; result [F][G] float32 receives the CLEAN-SC maps.
  * n_iter 0 means 2 * num_mics iterations; damp and stopn as the reference's traits. */
-size_t abf_cleansc_workspace_size(int32_t num_mics, int32_t num_freqs, int32_t n_iter);
+size_t abf_cleansc_workspace_size(int32_t num_mics, int64_t num_grid, int32_t num_freqs, int32_t n_iter);
 int abf_cleansc(const abf_geometry *geom, const double *wavenumber, int32_t num_freqs, void *csm, double *h,
                 int32_t steer_type, int32_t r_diag, double norm_factor, int32_t phase_mode, int32_t n_iter,
                 double damp, int32_t stopn, float *result, void *workspace, size_t workspace_bytes, void *stream);

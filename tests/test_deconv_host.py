@@ -75,7 +75,22 @@ def test_cleansc_parameters():
 
 
 def test_cleansc_workspace_and_errors():
+    """Workspace: powers [F][J] f64, hh [F][M] c128, hmax [F] f64, stop flags [F] i32, the distance
+    and weight table [M][G] (2 x f64) and the per-point sums [G] f64, each on a 256-byte boundary."""
     lib = _lib.lib()
-    assert lib.abf_cleansc_workspace_size(64, 10, 0) == 10 * 128 * 8 + 10 * 64 * 16 + 10 * 8 + 10 * 4 + 1024
-    assert lib.abf_cleansc_workspace_size(64, 10, 7) == 10 * 7 * 8 + 10 * 64 * 16 + 10 * 8 + 10 * 4 + 1024
-    assert lib.abf_cleansc_workspace_size(0, 10, 0) == 0
+
+    def up(x):
+        return (x + 255) // 256 * 256
+
+    def want(M, G, F, J):
+        n = up(F * J * 8)
+        n = up(n + F * M * 16)
+        n = up(n + F * 8)
+        n = up(n + F * 4)
+        n = up(n + M * G * 16)
+        return up(n + G * 8)
+
+    assert lib.abf_cleansc_workspace_size(64, 1681, 10, 0) == want(64, 1681, 10, 128)
+    assert lib.abf_cleansc_workspace_size(64, 1681, 3, 25) == want(64, 1681, 3, 25)
+    assert lib.abf_cleansc_workspace_size(0, 10, 10, 0) == 0
+    assert lib.abf_cleansc_workspace_size(4, -1, 10, 0) == 0
