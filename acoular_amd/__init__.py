@@ -12,6 +12,7 @@ from .fastFuncs import beamformerFreq, calcCSM, calcTransfer
 from .fbeamform import (
     BeamformerBase,
     BeamformerCapon,
+    BeamformerClean,
     BeamformerCleansc,
     BeamformerDamas,
     BeamformerEig,
@@ -25,6 +26,7 @@ from .fbeamform import (
     beamform_device,
     beamform_eig_device,
     beamform_f64_device,
+    clean_device,
     cleansc_device,
     damas_device,
     integrate,
@@ -40,6 +42,7 @@ __all__ = [
     'BaseSpectra',
     'BeamformerBase',
     'BeamformerCapon',
+    'BeamformerClean',
     'BeamformerCleansc',
     'BeamformerDamas',
     'BeamformerEig',
@@ -69,6 +72,7 @@ __all__ = [
     'beamform_device',
     'beamform_eig_device',
     'beamform_f64_device',
+    'clean_device',
     'cleansc_device',
     'damas_device',
     'beamformerFreq',

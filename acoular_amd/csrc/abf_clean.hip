@@ -1,9 +1,10 @@
-// CLEAN-SC deconvolution (BeamformerCleansc._calc, acoular/fbeamform.py:1449-1492) for gfx950.
+// CLEAN-SC (BeamformerCleansc._calc, acoular/fbeamform.py:1449-1492) and CLEAN
+// (BeamformerClean._calc, fbeamform.py:1536-1598) for gfx950.
 //
-// The reference runs, per frequency, up to J iterations of: peak of the map h, coherent source
-// component hh of the CSM at the peak (a 20-step fixed point on the steering vector), removal of
-// hh's map from h and of hmax hh hh^H from the CSM.  Here every frequency of a request advances
-// together, two launches per iteration:
+// CLEAN-SC: the reference runs, per frequency, up to J iterations of: peak of the map h, coherent
+// source component hh of the CSM at the peak (a 20-step fixed point on the steering vector),
+// removal of hh's map from h and of hmax hh hh^H from the CSM.  Here every frequency of a request
+// advances together, two launches per iteration:
 //  * csc_select_kernel, one workgroup per frequency: the peak (first index of the maximum, as
 //    numpy's argmax), the float32 result update, the stop test, the steering vector at the peak
 //    (transfer function, SteeringVector.steer_vector normalisation, fbeamform.py:145-153,
@@ -13,6 +14,18 @@
 // A frequency whose stop test fires is skipped by all later launches, so the result equals the
 // reference's early break.  Arithmetic is float64 except the reference's float32 phases and
 // float32 cos/sin (fastFuncs.py:259-261, 896-897).
+//
+// CLEAN (Hoegbom): per frequency, the peak of the dirty map, its amplitude damp * max added to
+// the clean map, and the PSF column of the peak (calcPointSpreadFunction, fastFuncs.py:563-801,
+// float32 phase differences) times that amplitude subtracted from the dirty map, until an
+// iteration no longer lowers the sum of absolute values, n_iter iterations have run or no
+// positive value is left.  Again two launches per iteration for all frequencies:
+// clean_select_kernel (sum, peak, stop test, amplitude; one workgroup per frequency) and
+// clean_update_kernel (the PSF column term by term and the subtraction; one thread per grid
+// point and frequency).
+//
+// Both methods read the distances grid point -> microphone and the steering weights from a
+// per-call table ([M][G], csc_table_kernel) instead of recomputing them in every iteration.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -82,27 +95,28 @@ __global__ __launch_bounds__(256) void fill_zero_i32_kernel(int *p, int64_t n)
 }
 
 // Per call: distances and steering weights of every (mic, grid point) and the per-point sums of
-// the steer normalisation, in the microphone order of the reference's loops.
-__global__ __launch_bounds__(256) void csc_table_kernel(CscArgs a)
+// the steer normalisation (sum r^2 for inverse, sum 1/r^2 for true level and true location), in
+// the microphone order of the reference's loops.
+__global__ __launch_bounds__(256) void csc_table_kernel(const double *gpos, const double *mpos, double2 *rw,
+                                                        double *help_out, int64_t G, int M, int steer_type)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.G) return;
-    const double gx = a.gpos[g], gy = a.gpos[a.G + g], gz = a.gpos[2 * a.G + g];
-    const int M = a.M;
+    if (g >= G) return;
+    const double gx = gpos[g], gy = gpos[G + g], gz = gpos[2 * G + g];
     double help = 0.0;
     for (int m = 0; m < M; ++m) {
-        const double r = dist3(gx, gy, gz, a.mpos[m], a.mpos[M + m], a.mpos[2 * M + m]);
+        const double r = dist3(gx, gy, gz, mpos[m], mpos[M + m], mpos[2 * M + m]);
         double w = 1.0;
-        if (a.steer_type == ABF_STEER_INVERSE) {
+        if (steer_type == ABF_STEER_INVERSE) {
             w = r;
             help += r * r;
-        } else if (a.steer_type != ABF_STEER_CLASSIC) {
+        } else if (steer_type != ABF_STEER_CLASSIC) {
             w = 1.0 / r;
             help += 1.0 / (r * r);
         }
-        a.rw[(int64_t)m * a.G + g] = make_double2(r, w);
+        rw[(int64_t)m * G + g] = make_double2(r, w);
     }
-    a.help[g] = help;
+    help_out[g] = help;
 }
 
 __global__ __launch_bounds__(kCscThreads) void csc_select_kernel(CscArgs a, int j)
@@ -392,6 +406,169 @@ __global__ __launch_bounds__(256) void csc_map_kernel(CscArgs a)
     a.h[(int64_t)f * a.G + g] -= a.damp * res;
 }
 
+struct CleanArgs {
+    const double *gpos;       // [3][G]
+    const double *mpos;       // [3][M]
+    const double *wavenumber; // [F]
+    double *dirty;            // [F][G] dirty maps, updated
+    double *clean;            // [F][G] clean maps
+    double *prev_sum;         // [F] sum |dirty| at the start of the last iteration
+    double *amp;              // [F] amplitude of the last iteration
+    int64_t *peak;            // [F] peak of the last iteration
+    int *stopped;             // [F]
+    const double2 *rw;        // [M][G] distance grid point -> mic, steering weight
+    const double *help;       // [G] sum over the mics of 1 / r^2
+    double ref[3];
+    double ref_dist;
+    double damp;
+    int64_t G;
+    int M, F, steer_type, phase_mode, psf_f32;
+};
+
+__device__ __forceinline__ double clean_r0(const CleanArgs &a, int64_t g)
+{
+    return a.ref_dist > 0.0 ? a.ref_dist
+                            : dist3(a.gpos[g], a.gpos[a.G + g], a.gpos[2 * a.G + g], a.ref[0], a.ref[1], a.ref[2]);
+}
+
+__global__ __launch_bounds__(kCscThreads) void clean_select_kernel(CleanArgs a, int j)
+{
+    __shared__ double s_pv[4], s_ps[4];
+    __shared__ int64_t s_pi[4];
+    const int f = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t G = a.G;
+    if (a.stopped[f]) return;
+    // sum of absolute values and the peak (first index of the maximum) of the dirty map
+    const double *d = a.dirty + (int64_t)f * G;
+    double best = -INFINITY, asum = 0.0;
+    int64_t bi = G;
+    for (int64_t base = tid; base < G; base += kCscBatch * kCscThreads) {
+        double v[kCscBatch];
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int64_t g = base + (int64_t)u * kCscThreads;
+            v[u] = d[g < G ? g : G - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int64_t g = base + (int64_t)u * kCscThreads;
+            if (g < G) {
+                asum += fabs(v[u]);
+                if (v[u] > best) {
+                    best = v[u];
+                    bi = g;
+                }
+            }
+        }
+    }
+    asum = __ockl_wfred_add_f64(asum);
+    for (int off = 32; off > 0; off >>= 1) {
+        const double v = __shfl_xor(best, off);
+        const int64_t i = __shfl_xor(bi, off);
+        if (v > best || (v == best && i < bi)) {
+            best = v;
+            bi = i;
+        }
+    }
+    if (lane == 0) {
+        s_pv[wv] = best;
+        s_pi[wv] = bi;
+        s_ps[wv] = asum;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    double dmax = s_pv[0];
+    int64_t xi = s_pi[0];
+    for (int q = 1; q < 4; ++q) {
+        if (s_pv[q] > dmax || (s_pv[q] == dmax && s_pi[q] < xi)) {
+            dmax = s_pv[q];
+            xi = s_pi[q];
+        }
+    }
+    if (xi >= G) xi = 0;
+    const double S = ((s_ps[0] + s_ps[1]) + s_ps[2]) + s_ps[3];
+    // the reference tests, after each iteration, that it lowered sum |dirty| and left a positive
+    // value; the first iteration always runs
+    if (j > 0 && !(a.prev_sum[f] > S && dmax > 0.0)) {
+        a.stopped[f] = 1;
+        return;
+    }
+    const double amp = a.damp * d[xi];
+    a.clean[(int64_t)f * G + xi] += amp;
+    a.amp[f] = amp;
+    a.peak[f] = xi;
+    a.prev_sum[f] = S;
+}
+
+__global__ __launch_bounds__(256) void clean_update_kernel(CleanArgs a)
+{
+    const int f = blockIdx.y;
+    if (a.stopped[f]) return;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.G) return;
+    const int M = a.M;
+    const int64_t s = a.peak[f];
+    const double k = a.wavenumber[f];
+    double sr = 0.0, si = 0.0;
+    const double2 *rg = a.rw + g;
+    const double2 *rsrc = a.rw + s;
+    for (int m0 = 0; m0 < M; m0 += kCscBatch) {
+        double2 t[kCscBatch];
+        double rs[kCscBatch];
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            const int64_t m = m0 + u < M ? m0 + u : M - 1;
+            t[u] = rg[m * a.G];
+            rs[u] = rsrc[m * a.G].x;
+        }
+#pragma unroll
+        for (int u = 0; u < kCscBatch; ++u) {
+            if (m0 + u >= M) break; // uniform over the workgroup
+            float c, sn;
+            phasor(k * (t[u].x - rs[u]), a.phase_mode, c, sn);
+            // (cos - 1j sin) / r_s, times r_g (inverse) or 1 / r_g (true level, true location)
+            double er = (double)c / rs[u], ei = -(double)sn / rs[u];
+            if (a.steer_type != ABF_STEER_CLASSIC) {
+                er *= t[u].y;
+                ei *= t[u].y;
+            }
+            sr += er;
+            si += ei;
+        }
+    }
+    const double sq = sr * sr + si * si;
+    const double r0s = clean_r0(a, s);
+    double res;
+    switch (a.steer_type) {
+    case ABF_STEER_CLASSIC: {
+        const double nf = r0s / M;
+        res = sq * (nf * nf);
+        break;
+    }
+    case ABF_STEER_INVERSE: {
+        const double nf = r0s / clean_r0(a, g) / M;
+        res = sq * (nf * nf);
+        break;
+    }
+    case ABF_STEER_TRUE_LEVEL: {
+        const double nf = r0s / clean_r0(a, g) / a.help[g];
+        res = sq * (nf * nf);
+        break;
+    }
+    default:
+        res = sq * (r0s * r0s) / M / a.help[g];
+        break;
+    }
+    if (a.psf_f32) res = (double)(float)res;
+    a.dirty[(int64_t)f * a.G + g] -= res * a.amp[f];
+}
+
+__global__ __launch_bounds__(256) void fill_zero_f64_kernel(double *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
 // Workspace layout: every array starts on a 256-byte boundary.
 struct CscLayout {
     size_t powers, hh, hmax, stopped, rw, help, total;
@@ -405,6 +582,24 @@ inline CscLayout csc_layout(int M, int64_t G, int F, int J)
     l.hh = up(l.powers + (size_t)F * J * sizeof(double));
     l.hmax = up(l.hh + (size_t)F * M * sizeof(double2));
     l.stopped = up(l.hmax + (size_t)F * sizeof(double));
+    l.rw = up(l.stopped + (size_t)F * sizeof(int));
+    l.help = up(l.rw + (size_t)M * G * sizeof(double2));
+    l.total = up(l.help + (size_t)G * sizeof(double));
+    return l;
+}
+
+struct CleanLayout {
+    size_t prev_sum, amp, peak, stopped, rw, help, total;
+};
+
+inline CleanLayout clean_layout(int M, int64_t G, int F)
+{
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    CleanLayout l{};
+    l.prev_sum = 0;
+    l.amp = up(l.prev_sum + (size_t)F * sizeof(double));
+    l.peak = up(l.amp + (size_t)F * sizeof(double));
+    l.stopped = up(l.peak + (size_t)F * sizeof(int64_t));
     l.rw = up(l.stopped + (size_t)F * sizeof(int));
     l.help = up(l.rw + (size_t)M * G * sizeof(double2));
     l.total = up(l.help + (size_t)G * sizeof(double));
@@ -476,10 +671,81 @@ int abf_cleansc(const abf_geometry *geom, const double *wavenumber, int32_t num_
                        G * num_freqs);
     hipLaunchKernelGGL(fill_zero_i32_kernel, dim3((unsigned)((num_freqs + 255) / 256)), dim3(256), 0, s, a.stopped,
                        (int64_t)num_freqs);
-    hipLaunchKernelGGL(csc_table_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(csc_table_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, a.gpos, a.mpos, a.rw,
+                       a.help, G, M, steer_type);
     for (int j = 0; j < J; ++j) {
         hipLaunchKernelGGL(csc_select_kernel, dim3((unsigned)num_freqs), dim3(kCscThreads), 0, s, a, j);
         hipLaunchKernelGGL(csc_map_kernel, dim3((unsigned)((G + 255) / 256), (unsigned)num_freqs), dim3(256), 0, s, a);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(ABF_ERR_HIP, hipGetErrorString(e));
+    return ABF_OK;
+}
+
+size_t abf_clean_workspace_size(int32_t num_mics, int64_t num_grid, int32_t num_freqs)
+{
+    if (num_mics < 1 || num_grid < 0 || num_freqs < 0) return 0;
+    return clean_layout(num_mics, num_grid, num_freqs).total;
+}
+
+int abf_clean(const abf_geometry *geom, const double *wavenumber, int32_t num_freqs, double *dirty,
+              int32_t steer_type, int32_t phase_mode, int32_t psf_f32, int32_t n_iter, double damp, double *clean,
+              void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (!geom) return set_error(ABF_ERR_INVALID, "null geometry");
+    if (steer_type < ABF_STEER_CLASSIC || steer_type > ABF_STEER_TRUE_LOCATION)
+        return set_error(ABF_ERR_INVALID, "unknown steer_type");
+    const int pm = phase_mode & 0xff;
+    if (pm != ABF_PHASE_REFERENCE && pm != ABF_PHASE_FP64) return set_error(ABF_ERR_INVALID, "unknown phase_mode");
+    const int M = geom->num_mics;
+    const int64_t G = geom->num_grid;
+    if (M < 1 || G < 0 || num_freqs < 0) return set_error(ABF_ERR_INVALID, "bad sizes");
+    if (num_freqs > 65535) return set_error(ABF_ERR_INVALID, "at most 65535 frequencies per call");
+    if (G == 0 || num_freqs == 0) return ABF_OK;
+    if (!geom->gpos || !geom->mpos || !wavenumber || !dirty || !clean)
+        return set_error(ABF_ERR_INVALID, "null pointer");
+    const size_t need = abf_clean_workspace_size(M, G, num_freqs);
+    if (!workspace || workspace_bytes < need) return set_error(ABF_ERR_WORKSPACE, "workspace too small");
+    // the reference's loop runs its first iteration unconditionally
+    const int J = n_iter > 1 ? n_iter : 1;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    char *ws = static_cast<char *>(workspace);
+    const CleanLayout lay = clean_layout(M, G, num_freqs);
+    CleanArgs a{};
+    a.gpos = geom->gpos;
+    a.mpos = geom->mpos;
+    a.wavenumber = wavenumber;
+    a.dirty = dirty;
+    a.clean = clean;
+    a.prev_sum = reinterpret_cast<double *>(ws + lay.prev_sum);
+    a.amp = reinterpret_cast<double *>(ws + lay.amp);
+    a.peak = reinterpret_cast<int64_t *>(ws + lay.peak);
+    a.stopped = reinterpret_cast<int *>(ws + lay.stopped);
+    double2 *rw = reinterpret_cast<double2 *>(ws + lay.rw);
+    double *help = reinterpret_cast<double *>(ws + lay.help);
+    a.rw = rw;
+    a.help = help;
+    for (int i = 0; i < 3; ++i) a.ref[i] = geom->ref[i];
+    a.ref_dist = geom->ref_dist;
+    a.damp = damp;
+    a.G = G;
+    a.M = M;
+    a.F = num_freqs;
+    a.steer_type = steer_type;
+    a.phase_mode = pm;
+    a.psf_f32 = psf_f32 ? 1 : 0;
+    // the clean maps and the stop flags start at zero (fill kernels, no memset nodes)
+    hipLaunchKernelGGL(fill_zero_f64_kernel, dim3((unsigned)((G * num_freqs + 255) / 256)), dim3(256), 0, s, clean,
+                       G * num_freqs);
+    hipLaunchKernelGGL(fill_zero_i32_kernel, dim3((unsigned)((num_freqs + 255) / 256)), dim3(256), 0, s, a.stopped,
+                       (int64_t)num_freqs);
+    // CLEAN's PSF weights are those of the beamformer: r (inverse), 1/r (true level, location)
+    hipLaunchKernelGGL(csc_table_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, a.gpos, a.mpos, rw,
+                       help, G, M, steer_type);
+    for (int j = 0; j < J; ++j) {
+        hipLaunchKernelGGL(clean_select_kernel, dim3((unsigned)num_freqs), dim3(kCscThreads), 0, s, a, j);
+        hipLaunchKernelGGL(clean_update_kernel, dim3((unsigned)((G + 255) / 256), (unsigned)num_freqs), dim3(256), 0,
+                           s, a);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(ABF_ERR_HIP, hipGetErrorString(e));

@@ -553,6 +553,50 @@ class BeamformerCleansc(BeamformerBase):
         self._fr[ind] = 1
 
 
+class BeamformerClean(BeamformerBase):
+    """Hoegbom CLEAN (``fbeamform.py:1495-1598``): from the float64 dirty map (clipped at zero with
+    ``r_diag``), repeatedly move ``damp`` times the peak into the clean map and subtract that
+    amplitude times the peak's PSF column, until an iteration no longer lowers the sum of absolute
+    values, ``n_iter`` iterations have run or no positive value is left.  All requested bins
+    iterate together on the device (``abf_beamform_f64``, then ``abf_clean``); the PSF columns are
+    formed per iteration on the device, so ``calcmode`` only keeps the reference's interface."""
+
+    _params = BeamformerBase._params + ('n_iter', 'damp', 'psf_precision')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'psf_precision', 'float64')
+        object.__setattr__(self, 'damp', 0.6)
+        object.__setattr__(self, 'n_iter', 100)
+        object.__setattr__(self, 'calcmode', 'block')
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name == 'psf_precision' and value not in ('float64', 'float32'):
+            raise ValueError("psf_precision must be 'float64' or 'float32'")
+        if name == 'calcmode' and value not in ('block', 'full', 'single', 'readonly'):
+            raise ValueError("calcmode must be 'block', 'full', 'single' or 'readonly'")
+        if name == 'n_iter':
+            value = int(value)
+        if name == 'damp':
+            value = float(value)
+            if not 0.01 <= value <= 1.0:
+                raise ValueError('damp must be in [0.01, 1.0]')
+        super().__setattr__(name, value)
+
+    def _calc(self, ind):
+        ind = np.asarray(ind, dtype=np.int64)
+        if type(self.steer) is not SteeringVector:
+            msg = 'BeamformerClean on the device needs a plain SteeringVector'
+            raise NotImplementedError(msg)
+        normfactor = self.sig_loss_norm()
+        dirty = beamform_f64_device(self.freq_data.csm_device_f64(), self._f[ind], ind, self.steer,
+                                    self.steer.steer_type, self.r_diag, normfactor, self.phase_mode, clip=self.r_diag)
+        out = clean_device(dirty, self._f[ind], self.steer, self.n_iter, self.damp, self.psf_precision,
+                           self.phase_mode)
+        self._ac[ind] = out.cpu().numpy().astype(self.precision)
+        self._fr[ind] = 1
+
+
 class PointSpreadFunction(HasParams):
     """The point spread function of a grid and microphone array (``fbeamform.py:948-1124``).
 
@@ -808,6 +852,34 @@ def cleansc_device(csm, freqs, ind, steer: SteeringVector, steer_type: str, r_di
                                _lib.STEER_CODES[steer_type], int(bool(r_diag)), float(norm),
                                _lib.PHASE_MODES[phase_mode], int(n_iter), float(damp), int(stopn), _lib.ptr(out),
                                _lib.ptr(ws), ws.numel(), _lib.stream_handle()), 'CLEAN-SC')
+    return out
+
+
+def clean_device(dirty, freqs, steer: SteeringVector, n_iter: int, damp: float, psf_precision: str = 'float64',
+                 phase_mode: str = 'reference'):
+    """One ``abf_clean`` call: CLEAN of the float64 device dirty maps ``[F, G]`` (updated in place)
+    at the frequencies ``freqs``.  Returns the float64 clean maps ``[F, G]`` as a device tensor."""
+    torch = _lib.torch()
+    if dirty.dtype != torch.float64:
+        raise TypeError('dirty must be a float64 device tensor')
+    if psf_precision not in ('float64', 'float32'):
+        raise ValueError("psf_precision must be 'float64' or 'float32'")
+    dirty = dirty.contiguous()
+    gpos, mpos = steer.device_geometry()
+    F, G = dirty.shape
+    freqs = np.asarray(freqs, dtype=np.float64)
+    if G != gpos.shape[1] or len(freqs) != F:
+        raise ValueError(f'dirty maps of shape {tuple(dirty.shape)} do not fit {gpos.shape[1]:d} grid points and '
+                         f'{len(freqs):d} frequencies')
+    dev = dirty.device
+    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    out = torch.empty((F, G), dtype=torch.float64, device=dev)
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.abf_clean_workspace_size(mpos.shape[1], G, F))
+    _lib.check(lib.abf_clean(_geometry(gpos, mpos, steer.ref), _lib.ptr(k), F, _lib.ptr(dirty),
+                             _lib.STEER_CODES[steer.steer_type], _lib.PHASE_MODES[phase_mode],
+                             int(psf_precision == 'float32'), int(n_iter), float(damp), _lib.ptr(out), _lib.ptr(ws),
+                             ws.numel(), _lib.stream_handle()), 'CLEAN')
     return out
 
 

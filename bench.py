@@ -621,6 +621,83 @@ def damas_bench(args):
     }
 
 
+def clean_bench(args):
+    """CLEAN on the 64-mic workload of the DAMAS mode (64 mics, RectGrid 41 x 41 at z = 0.3, 65
+    bins of block size 128 at 51.2 kHz, analytic complex128 CSM of 3 seeded sources), true level,
+    r_diag, the reference's defaults (up to 100 iterations, damp 0.6, float64 PSF): the float64
+    dirty maps (abf_beamform_f64, clipped) and the iterations of all bins together (abf_clean)."""
+    import torch
+
+    import acoular_amd as aa
+    from acoular_amd import synthetic
+    from oracle import beamform as B
+    from oracle import deconv as D
+
+    dev = torch.device('cuda', 0)
+    nm, seed = 64, 1001
+    mpos = synthetic.disk_array(nm, 1.0, seed)
+    grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.3, increment=0.01)
+    G = grid.size
+    spos = synthetic.source_positions(3, [-0.2, -0.2, 0.3], [0.2, 0.2, 0.3], seed + 1)
+    freqs = np.arange(65) * 51200.0 / 128
+    csm = synthetic.analytic_csm(mpos, spos, synthetic.level_powers(3, 10.0, seed + 2), freqs, dtype=np.complex128)
+    mics = aa.MicGeom(pos_total=mpos)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='true level')
+    dcsm = torch.from_numpy(csm).to(dev)
+    ind = np.arange(len(freqs))
+    norm = nm / (nm - 1.0)
+    box = {}
+
+    def step():
+        y = aa.beamform_f64_device(dcsm, freqs, ind, sv, 'true level', True, norm, clip=True)
+        box['x'] = aa.clean_device(y, freqs, sv, 100, 0.6)
+
+    step()
+    t_step = time_steps(step, max(1, args.steps), 1)
+    bins = len(freqs) * G
+    log(f'CLEAN step {t_step * 1e3:.2f} ms -> {bins / t_step:.4g} map-bins/s')
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mpos)
+    err = None
+    if not args.no_compare:
+        sample = [5, 33, 64]
+        y = B.beamformer_map(csm, freqs, grid.pos, mpos, 'true level', True, ind=sample)
+        y[y < 0] = 0.0
+        want = np.stack([D.clean(y[i], 'true level', r0, rm, 2 * np.pi * freqs[b] / 343.0) for i, b in
+                         enumerate(sample)])
+        got = box['x'][sample].cpu().numpy()
+        err = max(float(np.abs(got[i] - want[i]).max() / np.abs(want[i]).max()) for i in range(len(sample)))
+        same = bool(np.array_equal(got != 0, want != 0))
+        log(f'CLEAN max relative map error on bins {sample}: {err:.2e}, same support: {same}')
+        if not same:
+            err = float('inf')
+    cpu = None
+    if not args.no_cpu:
+        # the oracle (numpy) on whole bins of the same workload: dirty map and iterations
+        done, t0 = 0, time.perf_counter()
+        for b in range(3, len(freqs), 5):
+            y = B.beamformer_map(csm, freqs, grid.pos, mpos, 'true level', True, ind=[b])[0]
+            y[y < 0] = 0.0
+            D.clean(y, 'true level', r0, rm, 2 * np.pi * freqs[b] / 343.0)
+            done += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {'value': G * done / dt, 'unit': 'map-bins/s', 'cores': 1, 'kind': 'port',
+               'sample': f'{done} bins of the same workload (dirty map + iterations, numpy), {dt:.1f} s'}
+        log(f"CPU: {cpu['value']:.4g} map-bins/s")
+    return {
+        'metric': 'map-bins/s (grid pts x freqs) at <=1e-4 rel err vs FP64', 'value': bins / t_step,
+        'unit': 'map-bins/s', 'n_gpus': 1, 'ms_per_step': t_step * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic (seeded analytic CSM)',
+        'config': {'workload': 'CLEAN: 64 mics (1 m disk), RectGrid 41x41 z=0.3, 65 bins (bs 128, 51.2 kHz), '
+                               '3 sources, true level, r_diag, n_iter 100, damp 0.6, float64 PSF', 'num_mics': nm,
+                   'num_grid': G, 'num_freqs': len(freqs), 'parallelism': 'single GPU'},
+        'max_rel_err': err, 'cpu_baseline': cpu,
+    }
+
+
 def cleansc_bench(args):
     """CLEAN-SC on the 64-mic workload of the DAMAS mode (64 mics, RectGrid 41 x 41 at z = 0.3,
     65 bins of block size 128 at 51.2 kHz, analytic complex128 CSM of 3 seeded sources), true
@@ -798,7 +875,7 @@ def main():
     ap.add_argument('--tune', action='store_true', help='tune the launch shape now even if the table has it')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig',
-                                                          'damas', 'cleansc', 'c1'],
+                                                          'damas', 'cleansc', 'clean', 'c1'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
                          "'serving': the mixed-size request loop (Poisson arrivals) against the unbatched "
                          "plain kernel; 'fused': Welch CSM + beamforming of 512 mics, 10 s at 51.2 kHz, "
@@ -840,6 +917,9 @@ def main():
         return
     if args.mode == 'damas':
         print(json.dumps(damas_bench(args)), flush=True)
+        return
+    if args.mode == 'clean':
+        print(json.dumps(clean_bench(args)), flush=True)
         return
     if args.mode == 'cleansc':
         print(json.dumps(cleansc_bench(args)), flush=True)

@@ -232,6 +232,16 @@ int abf_cleansc(const abf_geometry *geom, const double *wavenumber, int32_t num_
                 int32_t steer_type, int32_t r_diag, double norm_factor, int32_t phase_mode, int32_t n_iter,
                 double damp, int32_t stopn, float *result, void *workspace, size_t workspace_bytes, void *stream);
 
+/* CLEAN (BeamformerClean._calc, fbeamform.py:1536-1598) for every frequency at once: dirty [F][G]
+ * float64 holds the dirty maps (abf_beamform_f64, clipped at zero when r_diag) and is updated in
+ * place; clean [F][G] float64 receives the clean maps.  The PSF columns of the peaks are formed
+ * per iteration as calcPointSpreadFunction (fastFuncs.py:563-801) forms them, rounded to float32
+ * when psf_f32 (psf_precision 'float32').  At least one iteration runs, at most max(n_iter, 1). */
+size_t abf_clean_workspace_size(int32_t num_mics, int64_t num_grid, int32_t num_freqs);
+int abf_clean(const abf_geometry *geom, const double *wavenumber, int32_t num_freqs, double *dirty,
+              int32_t steer_type, int32_t phase_mode, int32_t psf_f32, int32_t n_iter, double damp, double *clean,
+              void *workspace, size_t workspace_bytes, void *stream);
+
 /* _freqBeamformer_SpecificSteerVec_FullCSM / _CsmRemovedDiag (fastFuncs.py:414-476): explicit
  * steering vectors steer [G][M] complex64. out = q * norm, out_norm = sum |h|^2. */
 int abf_beamform_custom(const void *csm, const void *steer, int64_t num_grid, int32_t num_mics, int32_t r_diag,

@@ -10,6 +10,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
   column at a time.
 * ``clean_sc``: ``BeamformerCleansc._calc`` (``fbeamform.py:1449-1492``), the reference's
   iteration restated on the oracle's beamformer forms.
+* ``clean``: the iteration of ``BeamformerClean._calc`` (``fbeamform.py:1578-1598``) on one
+  dirty map, with PSF columns from ``psf``.
 * ``damas_gauss_seidel``: ``damasSolverGaussSeidel`` (``fastFuncs.py:820-855``), the
   row-by-row relaxed Gauss-Seidel sweep with the positivity clip, one scalar loop over rows
   per iteration (the dot product of a row with the current solution is the only vectorised
@@ -116,3 +118,25 @@ def clean_sc(csm, freqs, gpos, mpos, steer_type='true level', r_diag=True, r_dia
             C -= damp * csm1.T
         out[i] = result
     return out
+
+
+def clean(dirty, steer_type, r0, rm, k, n_iter=100, damp=0.6, psf_precision='float64', phase_mode='reference'):
+    """The Hoegbom CLEAN loop of ``BeamformerClean._calc`` (``fbeamform.py:1578-1598``) on one
+    float64 dirty map (already clipped when r_diag): returns the float64 clean map.  At least one
+    iteration always runs; the loop stops after an iteration that does not lower the sum of
+    absolute values, after ``n_iter`` iterations, or when the map has no positive value left."""
+    dirty = np.array(dirty, dtype=np.float64, copy=True)
+    gs = dirty.shape[0]
+    cl = np.zeros(gs, dtype=dirty.dtype)
+    i_iter = 0
+    flag = True
+    while flag:
+        dirty_sum = abs(dirty).sum(0)
+        next_max = dirty.argmax(0)
+        col = psf(steer_type, r0, rm, k, [next_max], phase_mode, dtype=psf_precision).reshape(gs)
+        new_amp = damp * dirty[next_max]
+        cl[next_max] += new_amp
+        dirty -= col * new_amp
+        i_iter += 1
+        flag = dirty_sum > abs(dirty).sum(0) and i_iter < n_iter and max(dirty) > 0
+    return cl

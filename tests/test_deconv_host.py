@@ -94,3 +94,24 @@ def test_cleansc_workspace_and_errors():
     assert lib.abf_cleansc_workspace_size(64, 1681, 3, 25) == want(64, 1681, 3, 25)
     assert lib.abf_cleansc_workspace_size(0, 10, 10, 0) == 0
     assert lib.abf_cleansc_workspace_size(4, -1, 10, 0) == 0
+
+
+def test_clean_parameters():
+    bf = aa.BeamformerClean(steer=_steer())
+    assert bf.n_iter == 100 and bf.damp == 0.6 and bf.psf_precision == 'float64' and bf.calcmode == 'block'
+    with pytest.raises(ValueError):
+        bf.psf_precision = 'float16'
+    with pytest.raises(ValueError):
+        bf.calcmode = 'all'
+    with pytest.raises(ValueError):
+        bf.damp = 0.001
+    d = bf.digest
+    bf.calcmode = 'single'
+    assert bf.digest == d  # calcmode does not change the result
+    bf.n_iter = 10
+    assert bf.digest != d
+    lib = _lib.lib()
+    G, M, F = 1681, 64, 10
+    n = 4 * 256 + (M * G * 16 + 255) // 256 * 256 + (G * 8 + 255) // 256 * 256
+    assert lib.abf_clean_workspace_size(M, G, F) == n
+    assert lib.abf_clean_workspace_size(0, G, F) == 0

@@ -195,3 +195,47 @@ def test_cleansc_stop_rule_and_damping(gpu):
             got = bf.result[:]
             np.testing.assert_array_equal(got != 0, want != 0)
             assert peak_rel_err(got, want) <= 1e-5
+
+
+@pytest.mark.parametrize('steer', STEERS)
+@pytest.mark.parametrize('r_diag', [True, False])
+@pytest.mark.parametrize('psf_precision', ['float64', 'float32'])
+def test_beamformer_clean_matches_oracle(gpu, steer, r_diag, psf_precision):
+    """CLEAN (fbeamform.py:1536-1598) on three sources: the same peaks in the same order
+    (identical support) and values to 1e-6 of the map peak (float64 on both sides; float32 PSF
+    columns rounded identically)."""
+    mics, grid = _setup(40, 15, seed=9, inc=0.04)
+    spos = synthetic.source_positions(3, [-0.25, -0.25, 0.6], [0.25, 0.25, 0.6], 4)
+    freqs = np.array([2500.0, 5000.0, 11000.0])
+    csm = synthetic.analytic_csm(mics.pos, spos, synthetic.level_powers(3, 6.0, 5), freqs, dtype=np.complex128)
+    ps = aa.PowerSpectraImport(csm=csm, frequencies=freqs)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer)
+    bf = aa.BeamformerClean(freq_data=ps, steer=sv, r_diag=r_diag, psf_precision=psf_precision, n_iter=60)
+    got = bf.result[:]
+    dirty = B.beamformer_map(csm, freqs, grid.pos, mics.pos, steer, r_diag)
+    if r_diag:
+        dirty[dirty < 0] = 0.0
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    want = np.stack([D.clean(dirty[i], steer, r0, rm, 2 * np.pi * fr / 343.0, n_iter=60, damp=0.6,
+                             psf_precision=psf_precision) for i, fr in enumerate(freqs)])
+    np.testing.assert_array_equal(got != 0, want != 0)
+    assert peak_rel_err(got, want) <= 1e-6
+
+
+def test_clean_device_stop_rules(gpu):
+    """Per-bin stop rules on the device: a bin without positive values runs one iteration, a bin
+    with a PSF-shaped map runs all n_iter, and n_iter 0 runs one."""
+    mics, grid = _setup(24, 9, seed=3, inc=0.05)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='classic')
+    freqs = np.array([3000.0, 3000.0])
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    k = 2 * np.pi * 3000.0 / 343.0
+    col = D.psf('classic', r0, rm, k, [40])[:, 0]
+    dirty = np.stack([-np.linspace(1.0, 2.0, grid.size), 2.0 * col])
+    for n_iter in (0, 1, 25):
+        got = aa.clean_device(_lib.to_device(dirty.copy()), freqs, sv, n_iter, 0.6).cpu().numpy()
+        want = np.stack([D.clean(dirty[i], 'classic', r0, rm, k, n_iter=n_iter, damp=0.6) for i in range(2)])
+        np.testing.assert_array_equal(got != 0, want != 0)
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12)

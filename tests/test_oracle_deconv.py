@@ -148,3 +148,43 @@ def test_clean_sc_single_source_is_removed_exactly(steer_type):
         out = D.clean_sc(csm, freqs, gpos, mpos, steer_type, r_diag=True, n_iter=n, damp=0.6, phase_mode='fp64')[0]
         np.testing.assert_allclose(out[60], h0[60] * (1 - 0.4**n), rtol=2e-6)
         assert np.count_nonzero(out) == 1
+
+
+@pytest.mark.parametrize('n_iter', [1, 3, 40])
+def test_clean_on_a_psf_shaped_map(n_iter):
+    """Classic steering, a dirty map that is p times the PSF column of grid point s (the map of a
+    monopole at s, whose peak is at s by the triangle inequality): every iteration picks s and
+    scales the map by (1 - damp c), c = psf(s, s), so after n iterations the clean map holds
+    p (1 - (1 - damp c)^n) at s and nothing elsewhere.  A PSF row taken for a column, or an
+    amplitude not taken from the current map, breaks the geometric series."""
+    from acoular_amd import synthetic
+
+    mpos = synthetic.disk_array(32, 1.0, 8)
+    x = np.linspace(-0.3, 0.3, 11)
+    gpos = np.stack([np.repeat(x, 11), np.tile(x, 11), np.full(121, 0.6)])
+    r0 = B.steer_r0(gpos, np.zeros(3))
+    rm = B.dist_mat(gpos, mpos)
+    k = 2 * np.pi * 4000.0 / 343.0
+    s, p = 47, 3.0
+    col = D.psf('classic', r0, rm, k, [s])[:, 0]
+    assert np.argmax(col) == s
+    c = col[s]
+    out = D.clean(p * col, 'classic', r0, rm, k, n_iter=n_iter, damp=0.6)
+    np.testing.assert_allclose(out[s], p * (1 - (1 - 0.6 * c) ** n_iter), rtol=1e-12)
+    assert np.count_nonzero(out) == 1
+
+
+def test_clean_stop_rules():
+    """A map with no positive value runs exactly one iteration (the loop's first pass is
+    unconditional); n_iter 0 behaves as 1."""
+    from acoular_amd import synthetic
+
+    mpos = synthetic.disk_array(8, 1.0, 8)
+    gpos = np.stack([np.linspace(-0.2, 0.2, 5), np.zeros(5), np.full(5, 0.5)])
+    r0 = B.steer_r0(gpos, np.zeros(3))
+    rm = B.dist_mat(gpos, mpos)
+    dirty = -np.arange(1.0, 6.0)
+    out = D.clean(dirty, 'true level', r0, rm, 1.0, n_iter=50, damp=0.5)
+    assert np.count_nonzero(out) == 1 and out[0] == 0.5 * dirty[0]
+    out = D.clean(np.arange(1.0, 6.0), 'classic', r0, rm, 1.0, n_iter=0, damp=0.5)
+    assert np.count_nonzero(out) == 1 and out[4] == 2.5
