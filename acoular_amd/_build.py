@@ -9,7 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libabf.so')
-SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f32.hip', 'abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip',
+SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f16.hip', 'abf_bf_mfma_f32.hip', 'abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip',
            'abf_eig.hip', 'abf_deconv.hip', 'abf_sources.hip', 'abf_clean.hip',
            'abf_api.hip']
 # the batched Hermitian eigensolver of the eigenvalue path

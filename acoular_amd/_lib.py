@@ -21,7 +21,7 @@ ABF_ERR_UNSUPPORTED = -2
 ABF_ERR_HIP = -3
 ABF_ERR_WORKSPACE = -4
 
-FORMS = {'plain_f32': 0, 'mfma_bf16x3': 1, 'mfma_f32': 2}
+FORMS = {'plain_f32': 0, 'mfma_bf16x3': 1, 'mfma_f32': 2, 'mfma_f16x2': 3}
 PHASE_MODES = {'reference': 0, 'fp64': 1}
 FAST_TRIG = 0x100
 STEER_CODES = {'classic': 1, 'inverse': 2, 'true level': 3, 'true location': 4}

@@ -190,10 +190,11 @@ def tune_job(job: _lib.AbfGridJob, reps: int = 3, kernels=KERNELS, tile_cols=TIL
 
 
 # (form, mics, grid points, bins) of the classes the shipped table covers: the 256-mic 201 x 201
-# plain-vs-MFMA map, the fused 512-mic 128 x 128 map of 1025 bins, the 1024-mic 64 x 64 x 32 map
+# plain-vs-MFMA map (all three MFMA forms), the fused 512-mic 128 x 128 map of 1025 bins, the 1024-mic 64 x 64 x 32 map
 # and the serving mix (mic counts 32..512, small and large grids, narrow and wide bands)
 STANDARD_CLASSES = (
-    [('mfma_bf16x3', 256, 40401, 513), ('mfma_f32', 256, 40401, 513), ('mfma_bf16x3', 512, 16384, 1025),
+    [('mfma_bf16x3', 256, 40401, 513), ('mfma_f32', 256, 40401, 513), ('mfma_f16x2', 256, 40401, 513),
+     ('mfma_bf16x3', 512, 16384, 1025),
      ('mfma_bf16x3', 1024, 131072, 256)]
     + [('mfma_bf16x3', m, g, f) for m in (32, 64, 128, 256, 512) for g in (4096, 65536) for f in (4, 32)]
 )

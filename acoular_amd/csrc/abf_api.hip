@@ -63,15 +63,16 @@ struct MfPlan {
     bool ws; // warp-specialised kernel (double-buffered steering matrix)
     int64_t ntiles, wgs;
     size_t prep_bytes, lds;
-    bool bf16;
+    size_t uscale_off; // f16x2: offset of the per-frequency U scales in the job's workspace
+    int prec;          // kPrF32, kPrBF16X3 or kPrF16X2
 };
 
-size_t mfma_lds(bool bf16, int BN, int Mp)
+size_t mfma_lds(bool h16, int BN, int Mp)
 {
-    // X planes of the tile, r0 + normalisation sums, per-wave partial results
-    const size_t KS = bf16 ? Mp + 8 : Mp + 4;
-    const size_t planes = bf16 ? 4 * (size_t)BN * KS * 2 : 2 * (size_t)BN * KS * 4;
-    return planes + 16 * (size_t)BN + 4 * 8 * (size_t)BN;
+    // X planes of the tile, r0 + normalisation sums, per-wave partial results, weight scales
+    const size_t KS = h16 ? Mp + 8 : Mp + 4;
+    const size_t planes = h16 ? 4 * (size_t)BN * KS * 2 : 2 * (size_t)BN * KS * 4;
+    return planes + 16 * (size_t)BN + 4 * 8 * (size_t)BN + 8 * (size_t)BN;
 }
 
 size_t mfma_ws_lds(bool bf16, int BN, int Mp)
@@ -90,13 +91,16 @@ int env_int(const char *name, int dflt)
 
 int plan_mfma(const abf_grid_job *j, MfPlan &p)
 {
-    p.bf16 = j->form == ABF_FORM_MFMA_BF16X3;
+    p.prec = j->form == ABF_FORM_MFMA_BF16X3 ? kPrBF16X3 : (j->form == ABF_FORM_MFMA_F16X2 ? kPrF16X2 : kPrF32);
+    const bool h16 = p.prec != kPrF32;
     p.Mp = round_up(j->geom.num_mics, 32);
     p.nK = p.Mp / 32;
     p.nUp = p.nK * (p.nK + 1) / 2;
     const int fb = j->tile_freqs > 0 ? j->tile_freqs : env_int("ABF_MFMA_FB", 8);
     p.FB = std::max(1, std::min(fb, j->num_freqs));
     p.prep_bytes = (size_t)j->num_freqs * p.nUp * 8192;
+    p.uscale_off = p.prep_bytes;
+    if (p.prec == kPrF16X2) p.prep_bytes += align256((size_t)j->num_freqs * sizeof(double));
     // grid points needed by this job, as a power-of-two count of 16-point tiles
     int ct_cap = 16;
     const int64_t tiles16 = cdiv(j->geom.num_grid, 16);
@@ -106,7 +110,13 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     // default (the faster one where both apply, measured on the 256-mic workload); the
     // warp-specialised kernel on request (tile_kernel, or ABF_MFMA_KERNEL=2) when its two
     // steering buffers fit LDS with 32 or 64 grid points per tile.
-    const int kreq = j->tile_kernel > 0 ? j->tile_kernel : env_int("ABF_MFMA_KERNEL", ABF_MFMA_KERNEL_AUTO);
+    int kreq = j->tile_kernel > 0 ? j->tile_kernel : env_int("ABF_MFMA_KERNEL", ABF_MFMA_KERNEL_AUTO);
+    // the f16x2 form exists in the alternating kernels only: the warp-specialised and pipelined
+    // requests map to their alternating counterparts
+    if (p.prec == kPrF16X2) {
+        if (kreq == ABF_MFMA_KERNEL_SPECIALISED || kreq == ABF_MFMA_KERNEL_PIPELINED) kreq = ABF_MFMA_KERNEL_ALTERNATING;
+        if (kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS) kreq = ABF_MFMA_KERNEL_ROW_PAIRS;
+    }
     p.CT = 0;
     p.ws = false;
     if (kreq == ABF_MFMA_KERNEL_SPECIALISED) {
@@ -114,7 +124,7 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
             if (ct > ct_cap || (ct_req > 0 && ct > ct_req)) continue;
             const int BN = 16 * ct;
             if (p.Mp / 8 > (256 / BN) * 5) continue; // 4 steering waves, at most 5 groups of 8 per thread
-            if (mfma_ws_lds(p.bf16, BN, p.Mp) > kLdsMax) continue;
+            if (mfma_ws_lds(h16, BN, p.Mp) > kLdsMax) continue;
             // the per-wave step table of one frequency lives in the 64 lanes of a register
             if (cdiv(2 * p.nK * (ct / 2), 4) * p.nK > 64) continue;
             p.CT = ct;
@@ -129,8 +139,8 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
             const int TPC = 512 / BN;
             // steering prologue: at most 5 groups of 8 per thread (4 for the pipelined kernels)
             const bool pf_req = kreq == ABF_MFMA_KERNEL_PIPELINED || kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS;
-            if (p.Mp / 8 > TPC * (pf_req && p.bf16 ? 4 : 5)) continue;
-            if (mfma_lds(p.bf16, BN, p.Mp) > kLdsMax) continue;
+            if (p.Mp / 8 > TPC * (pf_req && p.prec == kPrBF16X3 ? 4 : 5)) continue;
+            if (mfma_lds(h16, BN, p.Mp) > kLdsMax) continue;
             p.CT = ct;
             break;
         }
@@ -138,14 +148,15 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     if (p.CT == 0)
         return fail(ABF_ERR_UNSUPPORTED, "MFMA form supports at most ~1150 microphones; use the plain form");
     p.TT = (kreq == ABF_MFMA_KERNEL_ROW_PAIRS || kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS) ? 2 : 1;
-    p.pf = !p.ws && p.bf16 && (kreq == ABF_MFMA_KERNEL_PIPELINED || kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS);
+    p.pf = !p.ws && p.prec == kPrBF16X3 &&
+           (kreq == ABF_MFMA_KERNEL_PIPELINED || kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS);
     if (p.ws) p.CW = 2;
     else if (kreq == ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS) p.CW = 1;
     else if (p.TT == 2 || kreq == ABF_MFMA_KERNEL_PIPELINED) p.CW = std::min(p.CT, 2);
     else p.CW = std::min(p.CT, 4);
     p.ntiles = cdiv(j->geom.num_grid, 16 * p.CT);
     p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
-    p.lds = p.ws ? mfma_ws_lds(p.bf16, 16 * p.CT, p.Mp) : mfma_lds(p.bf16, 16 * p.CT, p.Mp);
+    p.lds = p.ws ? mfma_ws_lds(h16, 16 * p.CT, p.Mp) : mfma_lds(h16, 16 * p.CT, p.Mp);
     return ABF_OK;
 }
 
@@ -159,7 +170,7 @@ int validate_job(const abf_grid_job *j)
         return fail(ABF_ERR_INVALID, "bad job sizes");
     if (int r = check_steer(j->steer_type)) return r;
     if (int r = check_phase(j->phase_mode)) return r;
-    if (j->form < ABF_FORM_PLAIN_F32 || j->form > ABF_FORM_MFMA_F32) return fail(ABF_ERR_INVALID, "unknown form");
+    if (j->form < ABF_FORM_PLAIN_F32 || j->form > ABF_FORM_MFMA_F16X2) return fail(ABF_ERR_INVALID, "unknown form");
     if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS)
         return fail(ABF_ERR_INVALID, "unknown MFMA kernel variant");
     if (j->tile_cols < 0 || j->tile_freqs < 0) return fail(ABF_ERR_INVALID, "negative tile size");
@@ -216,7 +227,7 @@ __global__ void twiddle_kernel(float2 *tw, int bs)
     tw[j] = make_float2((float)c, (float)-s);
 }
 
-template <bool B, int CW, int V, int TT, bool PF>
+template <int B, int CW, int V, int TT, bool PF>
 void launch_mfma(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
     static bool attr = false;
@@ -228,7 +239,7 @@ void launch_mfma(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s
     hipLaunchKernelGGL((bf_mfma_kernel<B, CW, V, TT, PF>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
 }
 
-template <bool B, int V> void launch_mfma_ws(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+template <int B, int V> void launch_mfma_ws(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
     static bool attr = false;
     if (!attr) {
@@ -239,12 +250,14 @@ template <bool B, int V> void launch_mfma_ws(const MfJob *dev, int n, int64_t wg
     hipLaunchKernelGGL((bf_mfma_ws_kernel<B, V>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
 }
 
-template <bool B, int V>
+template <int B, int V>
 void launch_mfma_form(int CW, int TT, bool pf, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds,
                       hipStream_t s)
 {
-    if (ws) launch_mfma_ws<B, V>(dev, n, wgs, lds, s);
-    else if constexpr (B) {
+    if constexpr (B != kPrF16X2) {
+        if (ws) launch_mfma_ws<B, V>(dev, n, wgs, lds, s);
+    }
+    if constexpr (B == kPrBF16X3) {
         if (pf) {
             if (TT == 2) launch_mfma<B, 1, V, 2, true>(dev, n, wgs, lds, s);
             else if (CW == 1) launch_mfma<B, 1, V, 1, true>(dev, n, wgs, lds, s);
@@ -266,7 +279,7 @@ int mfma_variant(const abf_grid_job *j)
     return ((j->phase_mode & kFastTrigFlag) ? 1 : 0) | (weight_kind(j->steer_type) << 1);
 }
 
-template <bool B>
+template <int B>
 void launch_mfma_var(int var, int CW, int TT, bool pf, bool ws, const MfJob *dev, int n, int64_t wgs, size_t lds,
                      hipStream_t s)
 {
@@ -280,11 +293,12 @@ void launch_mfma_var(int var, int CW, int TT, bool pf, bool ws, const MfJob *dev
     }
 }
 
-void launch_mfma_group(bool bf16, int CW, int TT, bool pf, bool ws, int var, const MfJob *dev, int n, int64_t wgs,
+void launch_mfma_group(int prec, int CW, int TT, bool pf, bool ws, int var, const MfJob *dev, int n, int64_t wgs,
                        size_t lds, hipStream_t s)
 {
-    if (bf16) launch_mfma_var<true>(var, CW, TT, pf, ws, dev, n, wgs, lds, s);
-    else launch_mfma_var<false>(var, CW, TT, pf, ws, dev, n, wgs, lds, s);
+    if (prec == kPrBF16X3) launch_mfma_var<kPrBF16X3>(var, CW, TT, pf, ws, dev, n, wgs, lds, s);
+    else if (prec == kPrF16X2) launch_mfma_var<kPrF16X2>(var, CW, TT, pf, ws, dev, n, wgs, lds, s);
+    else launch_mfma_var<kPrF32>(var, CW, TT, pf, ws, dev, n, wgs, lds, s);
 }
 
 void prep_plain(const abf_grid_job *j, void *ws, hipStream_t s)
@@ -336,6 +350,8 @@ MfJob make_mfjob(const abf_grid_job *j, const MfPlan &p, const void *prep, int64
 {
     MfJob d{};
     d.prep = prep;
+    d.uscale = p.prec == kPrF16X2 ? reinterpret_cast<const double *>(static_cast<const char *>(prep) + p.uscale_off)
+                                  : nullptr;
     d.gpos = j->geom.gpos;
     d.mpos = j->geom.mpos;
     d.wavenumber = j->wavenumber;
@@ -361,8 +377,14 @@ MfJob make_mfjob(const abf_grid_job *j, const MfPlan &p, const void *prep, int64
 
 void launch_prep_mfma(const abf_grid_job *j, const MfPlan &p, void *prep, hipStream_t s)
 {
+    double *uscale = nullptr;
+    if (p.prec == kPrF16X2) {
+        uscale = reinterpret_cast<double *>(static_cast<char *>(prep) + p.uscale_off);
+        hipLaunchKernelGGL(uscale_kernel, dim3(j->num_freqs), dim3(256), 0, s,
+                           reinterpret_cast<const float2 *>(j->csm), j->geom.num_mics, j->r_diag, uscale);
+    }
     hipLaunchKernelGGL(prep_mfma_kernel, dim3(p.nK * p.nK, 2, j->num_freqs), dim3(64), 0, s,
-                       reinterpret_cast<const float2 *>(j->csm), j->geom.num_mics, p.Mp, j->r_diag, p.bf16 ? 1 : 0,
+                       reinterpret_cast<const float2 *>(j->csm), j->geom.num_mics, p.Mp, j->r_diag, p.prec, uscale,
                        reinterpret_cast<char *>(prep));
 }
 
@@ -438,7 +460,7 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
     // plain jobs run one after the other; MFMA jobs are grouped per (form, kernel, CW, steering
     // variant) into one launch
     struct Group {
-        bool bf16;
+        int prec;
         int CW, TT;
         bool pf, ws;
         int var;
@@ -461,13 +483,13 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         if (!(mode & kRun)) continue;
         auto it = std::find_if(groups.begin(), groups.end(),
                                [&](const Group &g) {
-                                   return g.bf16 == plans[i].bf16 && g.CW == plans[i].CW && g.TT == plans[i].TT &&
+                                   return g.prec == plans[i].prec && g.CW == plans[i].CW && g.TT == plans[i].TT &&
                                           g.pf == plans[i].pf && g.ws == plans[i].ws &&
                                           g.var == mfma_variant(&jobs[i]);
                                });
         if (it == groups.end()) {
             groups.push_back(
-                {plans[i].bf16, plans[i].CW, plans[i].TT, plans[i].pf, plans[i].ws, mfma_variant(&jobs[i]), {}});
+                {plans[i].prec, plans[i].CW, plans[i].TT, plans[i].pf, plans[i].ws, mfma_variant(&jobs[i]), {}});
             it = groups.end() - 1;
         }
         it->idx.push_back(i);
@@ -489,7 +511,7 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
             for (int q = 0; q < pk.n; ++q) pk.jobs[q] = descs[b + q];
             hipLaunchKernelGGL(write_jobs_kernel, dim3(1), dim3(64), 0, s, pk, dev_jobs + desc_base + b);
         }
-        launch_mfma_group(gr.bf16, gr.CW, gr.TT, gr.pf, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+        launch_mfma_group(gr.prec, gr.CW, gr.TT, gr.pf, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
         if (int r = hip_status("MFMA beamformer launch")) return r;
         desc_base += (int)descs.size();
     }

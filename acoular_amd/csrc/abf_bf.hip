@@ -309,12 +309,44 @@ __global__ __launch_bounds__(256) void bf_plain64_kernel(Plain64Args a)
 // Fragment layout of U for one frequency: upper 32x32 blocks (I <= J) in row-major upper
 // order; per block two 16-row tiles; per tile 4 KiB:
 //   bf16x3: parts {Ur_hi, Ur_lo, Ui_hi, Ui_lo}, each [64 lanes][8 bf16]
+//   f16x2:  parts {Ur, -, Ui, -}, each [64 lanes][8 fp16], U scaled by 1 / uscale[f]
 //   f32:    parts {Ur, Ui}, each [64 lanes][8 f32]
 // Lane l of a tile holds row (l & 15) and columns 8*(l >> 4) + 0..7 of the block, the same
 // k-set the B operand (X) uses, so each MFMA pairs matching k.
 
+// f16x2 form: per frequency, the power of two s that maps the largest |Re U_ij|, |Im U_ij| of the
+// upper triangle into [2^14, 2^15) (fp16 holds up to 65504); uscale[f] = 1 / s undoes it on q.
+__global__ __launch_bounds__(256) void uscale_kernel(const float2 *__restrict__ csm, int M, int r_diag,
+                                                     double *__restrict__ uscale)
+{
+    __shared__ float s_m[4];
+    const int f = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    float mx = 0.f;
+    const float2 *C = csm + (int64_t)f * M * M;
+    // each wave walks rows i = wave, wave + 4, ...: the upper part j >= i of a row, lane by lane
+    for (int i = tid >> 6; i < M; i += 4) {
+        const float2 *row = C + (int64_t)i * M;
+        for (int j = i + lane; j < M; j += 64) {
+            if (j == i && r_diag) continue;
+            const float2 c = row[j];
+            const float a = j > i ? 2.f * fmaxf(fabsf(c.x), fabsf(c.y)) : fabsf(c.x);
+            mx = fmaxf(mx, a); // fmaxf drops NaN: a NaN entry stays NaN in U and in the map
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if ((tid & 63) == 0) s_m[tid >> 6] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        const float m = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
+        int e = 0;
+        if (m > 0.f && m < INFINITY) e = 14 - ilogbf(m); // m 2^e in [2^14, 2^15)
+        uscale[f] = ldexp(1.0, -e);
+    }
+}
+
 __global__ __launch_bounds__(64) void prep_mfma_kernel(const float2 *__restrict__ csm, int M, int Mp, int r_diag,
-                                                       int bf16x3, char *__restrict__ prep)
+                                                       int prec, const double *__restrict__ uscale,
+                                                       char *__restrict__ prep)
 {
     const int nK = Mp / 32;
     const int I = blockIdx.x / nK, J = blockIdx.x % nK;
@@ -341,7 +373,19 @@ __global__ __launch_bounds__(64) void prep_mfma_kernel(const float2 *__restrict_
     }
     const int nUp = nK * (nK + 1) / 2;
     char *dst = prep + (((int64_t)f * nUp + upper_index(I, J, nK)) * 2 + t) * 4096;
-    if (bf16x3) {
+    if (prec == kPrF16X2) {
+        // U scaled by a power of two (exact) and rounded once to fp16: re into part 0, im into
+        // part 2 of the fragment (parts 1 and 3 are not read by this form)
+        const double sc = 1.0 / uscale[f];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            ur[e] = (float)((double)ur[e] * sc);
+            ui[e] = (float)((double)ui[e] * sc);
+        }
+        u4 *d = reinterpret_cast<u4 *>(dst);
+        d[0 * 64 + l] = pack8_h(ur);
+        d[2 * 64 + l] = pack8_h(ui);
+    } else if (prec == kPrBF16X3) {
         u4 rh, rl, ih, il;
         split8(ur, rh, rl);
         split8(ui, ih, il);

@@ -5,59 +5,59 @@
 
 namespace abf {
 
-template __global__ void bf_mfma_kernel<true, 1, 0, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 0, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 4, 0, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 0, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 0, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 0, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 0, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 0, 2, true>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<true, 0>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 1, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 1, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 4, 1, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 1, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 1, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 1, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 1, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 1, 2, true>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<true, 1>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 2, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 2, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 4, 2, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 2, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 2, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 2, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 2, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 2, 2, true>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<true, 2>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 3, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 3, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 4, 3, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 3, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 3, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 3, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 3, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 3, 2, true>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<true, 3>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 4, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 4, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 4, 4, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 4, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 4, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 4, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 4, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 4, 2, true>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<true, 4>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 5, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 5, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 4, 5, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 5, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 5, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 5, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 2, 5, 1, true>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<true, 1, 5, 2, true>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<true, 5>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 0, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 0, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 4, 0, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 0, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 0, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 0, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 0, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 0, 2, true>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrBF16X3, 0>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 1, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 1, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 4, 1, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 1, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 1, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 1, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 1, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 1, 2, true>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrBF16X3, 1>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 2, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 2, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 4, 2, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 2, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 2, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 2, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 2, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 2, 2, true>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrBF16X3, 2>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 3, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 3, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 4, 3, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 3, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 3, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 3, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 3, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 3, 2, true>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrBF16X3, 3>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 4, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 4, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 4, 4, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 4, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 4, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 4, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 4, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 4, 2, true>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrBF16X3, 4>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 5, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 5, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 4, 5, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 5, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 5, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 5, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 2, 5, 1, true>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrBF16X3, 1, 5, 2, true>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrBF16X3, 5>(const MfJob *, int);
 
 } // namespace abf

@@ -5,41 +5,41 @@
 
 namespace abf {
 
-template __global__ void bf_mfma_kernel<false, 1, 0, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 0, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 4, 0, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 0, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 0, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<false, 0>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 1, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 1, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 4, 1, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 1, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 1, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<false, 1>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 2, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 2, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 4, 2, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 2, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 2, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<false, 2>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 3, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 3, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 4, 3, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 3, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 3, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<false, 3>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 4, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 4, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 4, 4, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 4, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 4, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<false, 4>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 5, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 5, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 4, 5, 1, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 1, 5, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_kernel<false, 2, 5, 2, false>(const MfJob *, int);
-template __global__ void bf_mfma_ws_kernel<false, 5>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 0, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 0, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 4, 0, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 0, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 0, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrF32, 0>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 1, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 1, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 4, 1, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 1, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 1, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrF32, 1>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 2, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 2, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 4, 2, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 2, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 2, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrF32, 2>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 3, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 3, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 4, 3, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 3, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 3, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrF32, 3>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 4, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 4, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 4, 4, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 4, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 4, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrF32, 4>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 5, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 5, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 4, 5, 1, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 1, 5, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_kernel<kPrF32, 2, 5, 2, false>(const MfJob *, int);
+template __global__ void bf_mfma_ws_kernel<kPrF32, 5>(const MfJob *, int);
 
 } // namespace abf

@@ -40,8 +40,14 @@ struct Plain64Args {
 };
 
 // one beamforming job of an MFMA launch
+// Operand precision of the MFMA forms (template parameter PR of the kernels)
+constexpr int kPrF32 = 0;    // fp32 MFMA, U and X in float32
+constexpr int kPrBF16X3 = 1; // bf16 MFMA, U and X split hi + lo, 3 products per real product
+constexpr int kPrF16X2 = 2;  // fp16 MFMA, U rounded once (scaled per frequency), X split hi + lo: 2 products
+
 struct MfJob {
     const void *prep;          // U in fragment order, [F][nUpper][2][4 KiB]
+    const double *uscale;      // [F] power-of-two factor undoing the fp16 scaling of U (f16x2 form), or null
     const double *gpos;        // [3][G]
     const double *mpos;        // [3][M]
     const double *wavenumber;  // [F]
@@ -98,13 +104,15 @@ __global__ void prep_plain_kernel(const float2 *csm, int M, int Mp, int r_diag, 
 __global__ void bf_plain_kernel(PlainArgs a);
 __global__ void prep_plain64_kernel(const double2 *csm, int M, int Mp, int r_diag, double2 *ut);
 __global__ void bf_plain64_kernel(Plain64Args a);
-__global__ void prep_mfma_kernel(const float2 *csm, int M, int Mp, int r_diag, int bf16x3, char *prep);
+__global__ void prep_mfma_kernel(const float2 *csm, int M, int Mp, int r_diag, int prec, const double *uscale,
+                                 char *prep);
+__global__ void uscale_kernel(const float2 *csm, int M, int r_diag, double *uscale);
 // VAR = fast trig (bit 0) | steering weight kind (0 none, 1 r, 2 1/r) << 1
 // TT = row tiles (of 16) per work unit: 1, or 2 (both halves of a 32-row block); PF = software-
 // pipelined X/U fragment loads (split-bf16 only)
-template <bool BF16X3, int CW, int VAR, int TT, bool PF>
+template <int PR, int CW, int VAR, int TT, bool PF>
 __global__ void bf_mfma_kernel(const MfJob *jobs, int njobs);
-template <bool BF16X3, int VAR> __global__ void bf_mfma_ws_kernel(const MfJob *jobs, int njobs);
+template <int PR, int VAR> __global__ void bf_mfma_ws_kernel(const MfJob *jobs, int njobs);
 __global__ void bf_eig_kernel(EigArgs a);
 __global__ void bf_eig_grid_kernel(EigGridArgs a);
 __global__ void bf_eig_mfma_kernel(EigGridArgs a);

@@ -7,6 +7,8 @@ namespace abf {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
@@ -43,6 +45,32 @@ __device__ __forceinline__ void split8(const float *x, u4 &hi, u4 &lo)
         lo[d] = __builtin_bit_cast(uint32_t, l);
     }
 }
+
+// x = hi + lo with hi, lo fp16 (round to nearest even), 8 values
+__device__ __forceinline__ void split8_h(const float *x, u4 &hi, u4 &lo)
+{
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const f16x2 h = {(_Float16)x[2 * d], (_Float16)x[2 * d + 1]};
+        const f16x2 l = {(_Float16)(x[2 * d] - (float)h[0]), (_Float16)(x[2 * d + 1] - (float)h[1])};
+        hi[d] = __builtin_bit_cast(uint32_t, h);
+        lo[d] = __builtin_bit_cast(uint32_t, l);
+    }
+}
+
+// 8 values rounded to fp16 (round to nearest even)
+__device__ __forceinline__ u4 pack8_h(const float *x)
+{
+    u4 v;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const f16x2 h = {(_Float16)x[2 * d], (_Float16)x[2 * d + 1]};
+        v[d] = __builtin_bit_cast(uint32_t, h);
+    }
+    return v;
+}
+
+__device__ __forceinline__ float f16_to_float(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
 
 // frequency-independent steering normalisation sum of one microphone (fastFuncs.py:262-273)
 __device__ __forceinline__ double help_term(int steer_type, double r)

@@ -1,5 +1,6 @@
 // MFMA beamformer kernels (device code), included by the translation units that instantiate
-// them (abf_bf_mfma_bf16.hip, abf_bf_mfma_f32.hip).  VAR = fast trig (bit 0) | weight kind << 1
+// them (abf_bf_mfma_bf16.hip, abf_bf_mfma_f16.hip, abf_bf_mfma_f32.hip).  PR selects the operand
+// precision (kPrF32, kPrBF16X3, kPrF16X2).  VAR = fast trig (bit 0) | weight kind << 1
 // selects the steering-generation variant at compile time, so each instantiation carries one
 // straight-line generation body.
 #pragma once
@@ -47,10 +48,13 @@ __device__ __forceinline__ double steer_distances(const MfJob &jb, bool valid, d
 // the 8 independent phasors of a group interleave without selects.  Slots without a real
 // microphone or grid point get finite values (r = 1): the re-laid-out CSM is zero in the rows and
 // columns of padded microphones and the maps of padded grid points are not stored.
-template <bool BF16X3, bool FAST, int WK, int NS>
+// With PR = kPrF16X2 the weights are multiplied by xs, a power of two that brings them near 1
+// (fp16 keeps 5 exponent bits); the epilogue divides q by xs^2.
+template <int PR, bool FAST, int WK, int NS>
 __device__ __forceinline__ void steer_generate(const double (&rr)[NS][8], double k, bool refph, int col, int part,
-                                               int TPC, int ngroups, int KS, int64_t pe, char *xbuf)
+                                               int TPC, int ngroups, int KS, int64_t pe, char *xbuf, float xs = 1.f)
 {
+    constexpr bool BF16X3 = PR == kPrBF16X3;
     uint16_t *xh16 = reinterpret_cast<uint16_t *>(xbuf);
     float *x32 = reinterpret_cast<float *>(xbuf);
 #pragma unroll
@@ -84,16 +88,22 @@ __device__ __forceinline__ void steer_generate(const double (&rr)[NS][8], double
                     xi[e] = -sn;
                 } else {
                     const float rf = (float)r;
-                    const float w = WK == 1 ? rf : __builtin_amdgcn_rcpf(rf);
+                    float w = WK == 1 ? rf : __builtin_amdgcn_rcpf(rf);
+                    if constexpr (PR == kPrF16X2) w *= xs;
                     xr[e] = w * c;
                     xi[e] = -w * sn;
                 }
             }
             const int64_t off = (int64_t)col * KS + gi * 8;
-            if constexpr (BF16X3) {
+            if constexpr (PR != kPrF32) {
                 u4 rh, rl, ih, il;
-                split8(xr, rh, rl);
-                split8(xi, ih, il);
+                if constexpr (BF16X3) {
+                    split8(xr, rh, rl);
+                    split8(xi, ih, il);
+                } else {
+                    split8_h(xr, rh, rl);
+                    split8_h(xi, ih, il);
+                }
                 *reinterpret_cast<u4 *>(xh16 + off) = rh;
                 *reinterpret_cast<u4 *>(xh16 + pe + off) = rl;
                 *reinterpret_cast<u4 *>(xh16 + 2 * pe + off) = ih;
@@ -117,10 +127,11 @@ __device__ __forceinline__ void steer_generate(const double (&rr)[NS][8], double
 // covers both 16-row halves of a 32-row block, so every X fragment read from LDS feeds twice the
 // MFMAs (half the LDS traffic per MFMA, twice the U traffic).  The partial q of each wave and tile
 // column is added into s_q[wave][BN].
-template <bool BF16X3, int CW, int TT>
+template <int PR, int CW, int TT>
 __device__ __forceinline__ void gemm_frequency(int wave, int lane, int nK, int NCG, const ABF_GLOBAL char *prep_f,
                                                const char *xbuf, int64_t pe, int KS, int BN, float *s_q)
 {
+    constexpr bool BF16X3 = PR == kPrBF16X3;
     const uint16_t *xh16 = reinterpret_cast<const uint16_t *>(xbuf);
     const float *x32 = reinterpret_cast<const float *>(xbuf);
     const u4 sg = u4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
@@ -145,7 +156,61 @@ __device__ __forceinline__ void gemm_frequency(int wave, int lane, int nK, int N
         const ABF_GLOBAL char *ablk = prep_f + ((int64_t)upper_index(I, I, nK) * 2 + t0) * 4096;
         const int colbase = cg * CW * 16 + (lane & 15);
         const int kq = (lane >> 4) * 8;
-        if constexpr (BF16X3) {
+        if constexpr (PR == kPrF16X2) {
+            // U rounded once to fp16 (planes 0 and 2 of a fragment: re, im), X split hi + lo:
+            // Yr = Ur Xr - Ui Xi, Yi = Ui Xr + Ur Xi with X = Xhi + Xlo, 8 MFMAs per (row tile,
+            // column tile) and block
+            u4 a[TT][2]; // re, im
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+                    a[tt][q] = reinterpret_cast<const ABF_GLOBAL u4 *>(ablk + tt * 4096)[128 * q + lane];
+            for (int J = I; J < nK; ++J) {
+                u4 n[TT][2];
+#pragma unroll
+                for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) n[tt][q] = a[tt][q];
+                if (J + 1 < nK) {
+                    const ABF_GLOBAL char *nb = ablk + (int64_t)(J + 1 - I) * 8192;
+#pragma unroll
+                    for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            n[tt][q] = reinterpret_cast<const ABF_GLOBAL u4 *>(nb + tt * 4096)[128 * q + lane];
+                }
+                const int koff = J * 32 + kq;
+#define MH(A, B, C) C = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, A), B, C, 0, 0, 0)
+                f16x8 Xrh[CW], Xrl[CW], Xih[CW], Xil[CW];
+#pragma unroll
+                for (int c = 0; c < CW; ++c) {
+                    const int64_t xo = (int64_t)(colbase + c * 16) * KS + koff;
+                    Xrh[c] = __builtin_bit_cast(f16x8, *reinterpret_cast<const u4 *>(xh16 + xo));
+                    Xrl[c] = __builtin_bit_cast(f16x8, *reinterpret_cast<const u4 *>(xh16 + pe + xo));
+                    Xih[c] = __builtin_bit_cast(f16x8, *reinterpret_cast<const u4 *>(xh16 + 2 * pe + xo));
+                    Xil[c] = __builtin_bit_cast(f16x8, *reinterpret_cast<const u4 *>(xh16 + 3 * pe + xo));
+                }
+#pragma unroll
+                for (int tt = 0; tt < TT; ++tt) {
+                    const u4 a_r = a[tt][0], a_i = a[tt][1];
+                    const u4 a_n = a_i ^ sg;
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MH(a_r, Xrh[c], accr[tt][c]); MH(a_i, Xrh[c], acci[tt][c]); }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MH(a_r, Xrl[c], accr[tt][c]); MH(a_i, Xrl[c], acci[tt][c]); }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MH(a_n, Xih[c], accr[tt][c]); MH(a_r, Xih[c], acci[tt][c]); }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) { MH(a_n, Xil[c], accr[tt][c]); MH(a_r, Xil[c], acci[tt][c]); }
+                }
+#undef MH
+#pragma unroll
+                for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) a[tt][q] = n[tt][q];
+            }
+        } else if constexpr (BF16X3) {
             u4 a[TT][4]; // re hi, re lo, im hi, im lo
 #pragma unroll
             for (int tt = 0; tt < TT; ++tt)
@@ -263,6 +328,9 @@ __device__ __forceinline__ void gemm_frequency(int wave, int lane, int nK, int N
                     if constexpr (BF16X3) {
                         xr = bf16_to_float(xh16[xo + e]) + bf16_to_float(xh16[pe + xo + e]);
                         xi = bf16_to_float(xh16[2 * pe + xo + e]) + bf16_to_float(xh16[3 * pe + xo + e]);
+                    } else if constexpr (PR == kPrF16X2) {
+                        xr = f16_to_float(xh16[xo + e]) + f16_to_float(xh16[pe + xo + e]);
+                        xi = f16_to_float(xh16[2 * pe + xo + e]) + f16_to_float(xh16[3 * pe + xo + e]);
                     } else {
                         xr = x32[xo + e];
                         xi = x32[pe + xo + e];
@@ -389,10 +457,12 @@ __device__ __forceinline__ void gemm_frequency_pf(int wave, int lane, int nK, in
 
 // Normalise and store frequency f of the tile: q = sum of the NQ partials in s_q[NQ][BN]
 // (fixed order, then cleared), steering sum = sum of help_parts partials of s_help[part][BN].
+// With the f16x2 form, q is multiplied by the powers of two that undo the scaling of U (fsc) and
+// of the grid point's steering weights (s_xf[col]); both are exact.
 template <int NQ>
 __device__ __forceinline__ void store_frequency(const MfJob &jb, int f, int tile, int BN, int idx0, int stride,
                                                 float *s_q, const double *s_r0, const double *s_help,
-                                                int help_parts)
+                                                int help_parts, double fsc = 1.0, const double *s_xf = nullptr)
 {
     const int64_t G = jb.G;
     for (int cidx = idx0; cidx < BN; cidx += stride) {
@@ -402,6 +472,7 @@ __device__ __forceinline__ void store_frequency(const MfJob &jb, int f, int tile
             qs += (double)s_q[w * BN + cidx];
             s_q[w * BN + cidx] = 0.f;
         }
+        if (s_xf) qs = qs * fsc * s_xf[cidx];
         double help = 0.0;
         for (int p = 0; p < help_parts; ++p) help += s_help[p * BN + cidx];
         const int64_t gg = (int64_t)tile * BN + cidx;
@@ -422,10 +493,11 @@ __device__ __forceinline__ void store_frequency(const MfJob &jb, int f, int tile
 // steering matrix X of the tile (BN x Mp, split hi+lo bf16 or fp32, K-slot order of kslot_mic)
 // into LDS, then run gemm_frequency; q is reduced per grid point in a fixed order
 // (bit-identical results alone and batched), then normalised per steer type.
-template <bool BF16X3, int CW, int VAR, int TT, bool PF>
+template <int PR, int CW, int VAR, int TT, bool PF>
 __global__ __launch_bounds__(kMfWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void bf_mfma_kernel(
     const MfJob *__restrict__ jobs, int njobs)
 {
+    constexpr bool BF16X3 = PR == kPrBF16X3;
     constexpr int NT = kMfWaves * 64;
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int tid = threadIdx.x;
@@ -441,15 +513,16 @@ __global__ __launch_bounds__(kMfWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     const int tile = (int)(local % jb.ntiles);
     const int f0 = (int)(local / jb.ntiles) * FB;
     const int f1 = min(f0 + FB, F);
-    const int KS = BF16X3 ? Mp + 8 : Mp + 4; // LDS row stride in elements (16-byte skew)
+    const int KS = PR != kPrF32 ? Mp + 8 : Mp + 4; // LDS row stride in elements (16-byte skew)
     const int nK = Mp / 32;
     const int nUp = nK * (nK + 1) / 2;
 
-    // LDS: X planes [BN][KS] (8 bytes per element in both forms), then r0, help, q
+    // LDS: X planes [BN][KS] (8 bytes per element in all forms), then r0, help, q, weight scales
     const int64_t pe = (int64_t)BN * KS; // plane stride in elements
     double *s_r0 = reinterpret_cast<double *>(smem + pe * 8);
     double *s_help = s_r0 + BN;
     float *s_q = reinterpret_cast<float *>(s_help + BN); // [kMfWaves][BN]
+    double *s_xf = reinterpret_cast<double *>(s_q + kMfWaves * BN); // [BN], f16x2 form only
 
     // ---- prologue: distances of this thread's (point, 8-slot groups) into registers ----
     const int TPC = NT / BN; // threads per grid point (power of two)
@@ -467,11 +540,24 @@ __global__ __launch_bounds__(kMfWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     double rr[NS][8];
     double help = steer_distances<NS>(jb, valid, gx, gy, gz, part, TPC, ngroups, rr);
     for (int off = TPC >> 1; off > 0; off >>= 1) help += __shfl_xor(help, off);
+    // f16x2: the steering weights (r or 1/r) of this grid point are scaled by xs = 2^-e, e the
+    // exponent of its weight at microphone 0, so they stay well inside fp16's range
+    float xs = 1.f;
+    int xe = 0;
+    if constexpr (PR == kPrF16X2 && (VAR >> 1) != 0) {
+        const ABF_GLOBAL double *mp = gp(jb.mpos);
+        const double r = valid ? dist3(gx, gy, gz, mp[0], mp[jb.M], mp[2 * jb.M]) : 1.0;
+        const double w0 = (VAR >> 1) == 1 ? r : 1.0 / r;
+        xe = (w0 > 0.0 && w0 < INFINITY) ? ilogb(w0) : 0;
+        xe = xe < -60 ? -60 : (xe > 60 ? 60 : xe);
+        xs = ldexpf(1.f, -xe);
+    }
     if (part == 0) {
         s_help[col] = help;
         double r0 = jb.ref_dist;
         if (!(r0 > 0.0)) r0 = dist3(gx, gy, gz, jb.ref[0], jb.ref[1], jb.ref[2]);
         s_r0[col] = r0;
+        if constexpr (PR == kPrF16X2) s_xf[col] = ldexp(1.0, 2 * xe);
     }
     for (int idx = tid; idx < kMfWaves * BN; idx += NT) s_q[idx] = 0.f;
 
@@ -482,16 +568,19 @@ __global__ __launch_bounds__(kMfWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     // Frequencies in band order: workgroups running side by side on one XCD then read the U
     // blocks of the same one or two frequencies at any moment, which keeps them in that XCD's L2.
     for (int f = f0; f < f1; ++f) {
-        steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1), NS>(rr, gp(jb.wavenumber)[f], refph, col, part, TPC,
-                                                               ngroups, KS, pe, smem);
+        steer_generate<PR, (VAR & 1) != 0, (VAR >> 1), NS>(rr, gp(jb.wavenumber)[f], refph, col, part, TPC,
+                                                           ngroups, KS, pe, smem, xs);
         __syncthreads();
         if constexpr (BF16X3 && PF)
             gemm_frequency_pf<CW, TT>(wave, lane, nK, NCG, prep_base + (int64_t)f * nUp * 8192, smem, pe, KS, BN, s_q);
         else
-            gemm_frequency<BF16X3, CW, TT>(wave, lane, nK, NCG, prep_base + (int64_t)f * nUp * 8192, smem, pe, KS,
-                                           BN, s_q);
+            gemm_frequency<PR, CW, TT>(wave, lane, nK, NCG, prep_base + (int64_t)f * nUp * 8192, smem, pe, KS, BN,
+                                       s_q);
         __syncthreads();
-        store_frequency<kMfWaves>(jb, f, tile, BN, tid, NT, s_q, s_r0, s_help, 1);
+        if constexpr (PR == kPrF16X2)
+            store_frequency<kMfWaves>(jb, f, tile, BN, tid, NT, s_q, s_r0, s_help, 1, gp(jb.uscale)[f], s_xf);
+        else
+            store_frequency<kMfWaves>(jb, f, tile, BN, tid, NT, s_q, s_r0, s_help, 1);
         // the next frequency overwrites X only after every wave has passed this barrier
         __syncthreads();
     }
@@ -518,10 +607,12 @@ constexpr int kWsCW = 2; // column tiles per U fragment in the warp-specialised 
 // (across unit and frequency boundaries) into a ring of four register sets and X fragments one
 // step ahead, so one wave per SIMD keeps its MFMA pipe fed.  The steering waves hold the tile's
 // float64 distances in registers for all FB frequencies.
-template <bool BF16X3, int VAR>
+template <int PR, int VAR>
 __global__ __launch_bounds__((kWsMf + kGenWaves) * 64) void bf_mfma_ws_kernel(const MfJob *__restrict__ jobs,
                                                                               int njobs)
 {
+    static_assert(PR != kPrF16X2, "the warp-specialised kernel has no f16x2 form");
+    constexpr bool BF16X3 = PR == kPrBF16X3;
     constexpr int NG = kGenWaves * 64;
     constexpr int CW = kWsCW;
     extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -576,13 +667,13 @@ __global__ __launch_bounds__((kWsMf + kGenWaves) * 64) void bf_mfma_ws_kernel(co
         }
         for (int idx = gt; idx < 2 * kWsMf * BN; idx += NG) s_q[idx] = 0.f;
         const bool refph = jb.phase_mode == ABF_PHASE_REFERENCE;
-        steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1), kMfNS>(rr, gp(jb.wavenumber)[phys(f0)], refph, col, part, TPC,
+        steer_generate<PR, (VAR & 1) != 0, (VAR >> 1), kMfNS>(rr, gp(jb.wavenumber)[phys(f0)], refph, col, part, TPC,
                                                            ngroups, KS, pe, xb(0));
         lds_barrier();
         for (int f = f0; f < f1; ++f) {
             const int b = (f - f0) & 1;
             if (f + 1 < f1)
-                steer_generate<BF16X3, (VAR & 1) != 0, (VAR >> 1), kMfNS>(rr, gp(jb.wavenumber)[phys(f + 1)], refph, col, part,
+                steer_generate<PR, (VAR & 1) != 0, (VAR >> 1), kMfNS>(rr, gp(jb.wavenumber)[phys(f + 1)], refph, col, part,
                                                                    TPC, ngroups, KS, pe, xb(b ^ 1));
             if (f > f0)
                 store_frequency<kWsMf>(jb, phys(f - 1), tile, BN, gt, NG, s_q + (b ^ 1) * kWsMf * BN, s_r0, s_help, TPC);

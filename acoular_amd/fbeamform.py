@@ -132,7 +132,7 @@ class LazyBfResult:
 class BeamformerBase(HasParams):
     """Conventional (delay-and-sum) beamformer in the frequency domain (``fbeamform.py:289-685``).
 
-    Parameters beyond the reference's: ``form`` ('mfma_bf16x3', 'mfma_f32' or 'plain_f32') and
+    Parameters beyond the reference's: ``form`` ('mfma_bf16x3', 'mfma_f16x2', 'mfma_f32' or 'plain_f32') and
     ``phase_mode`` ('reference': the float32-rounded phase of ``fastFuncs.py:260``, or 'fp64').
     """
 

@@ -27,7 +27,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MI355X_PEAK = {'bf16': 2500.0, 'fp32_mfma': 157.3, 'fp32': 157.3}  # TFLOP/s, dense
+MI355X_PEAK = {'bf16': 2500.0, 'fp16': 2500.0, 'fp32_mfma': 157.3, 'fp32': 157.3}  # TFLOP/s, dense
 
 
 def log(msg):
@@ -121,7 +121,8 @@ def executed_flops_per_bin(form, nm):
         nrb = mp // 16
         return 8.0 * 16 * (mp * nrb - 16 * nrb * (nrb - 1) / 2)
     nk = (nm + 31) // 32
-    per_pair = 12 * 16384 / 16 if form == 'mfma_bf16x3' else 32 * 2048 / 16  # per (row tile, K-block)
+    # per (row tile, K-block): 12 or 8 16x16x32 MFMAs (split forms), 32 16x16x4 fp32 MFMAs
+    per_pair = {'mfma_bf16x3': 12 * 16384 / 16, 'mfma_f16x2': 8 * 16384 / 16}.get(form, 32 * 2048 / 16)
     return per_pair * nk * (nk + 1)
 
 
@@ -868,7 +869,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--form', default='mfma_bf16x3', choices=['mfma_bf16x3', 'mfma_f32', 'plain_f32'])
+    ap.add_argument('--form', default='mfma_bf16x3', choices=['mfma_bf16x3', 'mfma_f16x2', 'mfma_f32', 'plain_f32'])
     ap.add_argument('--no-compare', action='store_true', help='skip the other forms')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--no-tune', action='store_true', help='default launch shapes (no autotuner table, no tuning)')
@@ -955,7 +956,7 @@ def main():
     t_run = event_time(runner.run, max(3, args.steps))
     nm = wl['nm']
     alg_flops = runner.map_bins * 8.0 * nm * (nm + 1)
-    bound_peak = MI355X_PEAK['bf16'] if args.form == 'mfma_bf16x3' else MI355X_PEAK['fp32']
+    bound_peak = MI355X_PEAK['bf16'] if args.form in ('mfma_bf16x3', 'mfma_f16x2') else MI355X_PEAK['fp32']
     achieved = alg_flops / t_run / 1e12
     executed = executed_flops_per_bin(args.form, nm)
     roofline = {'bound': 'mfma' if args.form != 'plain_f32' else 'valu', 'achieved': round(achieved, 2),
@@ -974,7 +975,7 @@ def main():
 
     forms = {args.form: value}
     if not args.no_compare:
-        for other in ('mfma_f32', 'plain_f32', 'mfma_bf16x3'):
+        for other in ('mfma_f32', 'plain_f32', 'mfma_bf16x3', 'mfma_f16x2'):
             if other == args.form:
                 continue
             r = Runner(wl, other, dev=dev)
@@ -999,7 +1000,8 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': {'mfma_bf16x3': 'bf16x3 (split hi+lo, fp32 accumulate)', 'mfma_f32': 'f32',
+        'dtype': {'mfma_bf16x3': 'bf16x3 (split hi+lo, fp32 accumulate)',
+                  'mfma_f16x2': 'f16x2 (CSM fp16 scaled, steering split hi+lo, fp32 accumulate)', 'mfma_f32': 'f32',
                   'plain_f32': 'f32'}[args.form],
         'data': 'synthetic (seeded analytic CSM)',
         'config': {'workload': 'plain vs MFMA beamforming: 256 mics (2 m disk), 8 sources (20 dB spread), '

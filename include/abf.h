@@ -47,7 +47,9 @@ enum abf_steer {
 enum abf_form {
     ABF_FORM_PLAIN_F32 = 0,   /* FP32 VALU, one grid point per lane, CSM staged per row block */
     ABF_FORM_MFMA_BF16X3 = 1, /* complex GEMM on bf16 MFMA, operands split hi+lo (3 products) */
-    ABF_FORM_MFMA_F32 = 2     /* complex GEMM on fp32 MFMA */
+    ABF_FORM_MFMA_F32 = 2,    /* complex GEMM on fp32 MFMA */
+    ABF_FORM_MFMA_F16X2 = 3   /* complex GEMM on fp16 MFMA: U rounded once (scaled by a power of two per
+                                 frequency), steering split hi+lo (2 products per real product) */
 };
 
 /* Phase handling: REFERENCE rounds k*r to float32 before cos/sin like fastFuncs.py:260;

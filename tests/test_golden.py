@@ -38,7 +38,7 @@ def test_golden_known_answer_values(golden):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('form', ['plain_f32', 'mfma_f32', 'mfma_bf16x3'])
+@pytest.mark.parametrize('form', ['plain_f32', 'mfma_f32', 'mfma_bf16x3', 'mfma_f16x2'])
 def test_gpu_maps_match_golden(gpu, golden, form):
     mics = aa.MicGeom(pos_total=golden['bf_mpos'])
     grid = aa.ImportGrid(pos=golden['bf_gpos'])
@@ -51,7 +51,7 @@ def test_gpu_maps_match_golden(gpu, golden, form):
             norm = nm / (nm - 1.0) if rd else 1.0
             out = beamform_device(dcsm, freqs, np.arange(3), sv, st, bool(rd), norm, form, clip=bool(rd))
             err = peak_rel_err(out.cpu().numpy(), golden['bf_maps'][i, rd])
-            assert err <= 1e-4, (st, rd, err)
+            assert err <= (5e-4 if form == 'mfma_f16x2' else 1e-4), (st, rd, err)
         _, onorm = beamform_device(dcsm, freqs, np.arange(3), sv, st, False, 1.0, form, out_norm=True)
         np.testing.assert_allclose(onorm.cpu().numpy(), golden['bf_norms'][i], rtol=1e-5)
 

@@ -1,8 +1,11 @@
 """GPU parity: the HIP kernels of libabf.so against the CPU oracle on the same seeded inputs.
 
 Tolerance: the maps agree with the oracle (phase_mode 'reference', float64) to a peak-normalised
-error max_f max_g |B - B_ref| / max_g |B_ref| <= 1e-4 for every form; CSMs to 1e-5 of their
-largest element; transfer functions to 1e-6 absolute (float32 cos/sin).
+error max_f max_g |B - B_ref| / max_g |B_ref| <= 1e-4 for the plain, fp32 and split-bf16 forms;
+CSMs to 1e-5 of their largest element; transfer functions to 1e-6 absolute (float32 cos/sin).
+The opt-in f16x2 form rounds the CSM once to fp16 (unit roundoff 2^-11): its map error is the map
+of that rounding matrix, up to about 2e-4 of the peak where the grid misses the sources' beams
+(measured 1.8e-4 on the large-k r scene below), so its gate is 5e-4.
 """
 
 import ctypes
@@ -20,8 +23,13 @@ from tests.conftest import peak_rel_err
 pytestmark = pytest.mark.gpu
 
 STEERS = ['classic', 'inverse', 'true level', 'true location']
-FORMS = ['plain_f32', 'mfma_f32', 'mfma_bf16x3']
+FORMS = ['plain_f32', 'mfma_f32', 'mfma_bf16x3', 'mfma_f16x2']
 TOL = 1e-4
+TOL_F16X2 = 5e-4
+
+
+def _tol(form):
+    return TOL_F16X2 if form == 'mfma_f16x2' else TOL
 
 
 def _scene(nm, grid, freqs, nsrc=3, seed=1, z=0.5, diameter=1.0):
@@ -61,7 +69,7 @@ def test_grid_beamformer_matches_oracle(gpu, form, steer, r_diag):
     ref = _oracle_maps(csm, freqs, mics, grid, steer, r_diag)
     err = peak_rel_err(got, ref)
     print(f'form={form} steer={steer} r_diag={r_diag} max rel map err={err:.2e}')
-    assert err <= TOL
+    assert err <= _tol(form)
 
 
 @pytest.mark.parametrize('form', FORMS)
@@ -77,7 +85,7 @@ def test_phase_modes_and_trig_paths(gpu, form, fast_trig, phase_mode):
     ref = _oracle_maps(csm, freqs, mics, grid, 'true level', False, phase_mode)
     err = peak_rel_err(got, ref)
     print(f'form={form} fast_trig={fast_trig} phase={phase_mode} err={err:.2e}')
-    assert err <= TOL
+    assert err <= _tol(form)
 
 
 @pytest.mark.parametrize('form', FORMS)
@@ -95,10 +103,10 @@ def test_ragged_shapes(gpu, form, nm, nx, nf):
         ref = _oracle_maps(csm, freqs, mics, grid, steer, r_diag)
         err = peak_rel_err(got, ref)
         print(f'form={form} M={nm} G={grid.size} F={nf} steer={steer} err={err:.2e}')
-        assert err <= TOL
+        assert err <= _tol(form)
 
 
-@pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f32', 'plain_f32'])
+@pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f16x2', 'mfma_f32', 'plain_f32'])
 def test_many_mics(gpu, form):
     """1024 microphones (the largest array of the workloads), 3D grid slice, true level."""
     freqs = np.array([3000.0, 9000.0])
@@ -111,7 +119,7 @@ def test_many_mics(gpu, form):
     ref = _oracle_maps(csm, freqs, mics, grid, 'true level', True)
     err = peak_rel_err(got, ref)
     print(f'form={form} M=1024 G={grid.size} err={err:.2e}')
-    assert err <= TOL
+    assert err <= _tol(form)
 
 
 def test_reference_point_and_distance(gpu):
@@ -340,7 +348,7 @@ def test_batch_equals_alone_bitwise(gpu):
         np.testing.assert_array_equal(o.cpu().numpy(), a)
 
 
-@pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f32'])
+@pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f16x2', 'mfma_f32'])
 @pytest.mark.parametrize('kernel,tile_cols,tile_freqs', [(1, 0, 0), (1, 16, 1), (1, 64, 3), (2, 0, 0), (2, 64, 2),
                                                          (2, 32, 5), (3, 0, 0), (3, 16, 2), (3, 64, 4), (4, 0, 0),
                                                          (4, 64, 3), (5, 0, 0), (5, 32, 2)])
@@ -366,7 +374,7 @@ def test_mfma_kernel_variants_and_tiles(gpu, form, kernel, tile_cols, tile_freqs
     ref = _oracle_maps(csm, freqs, mics, grid, steer, True)
     err = peak_rel_err(out.cpu().numpy(), ref)
     print(f'form={form} kernel={kernel} tile=({tile_cols},{tile_freqs}) nm={nm} steer={steer} err={err:.2e}')
-    assert err <= TOL
+    assert err <= _tol(form)
 
 
 def test_mfma_kernel_variant_rejected_when_unknown(gpu):
@@ -383,3 +391,26 @@ def test_mfma_kernel_variant_rejected_when_unknown(gpu):
     job.tile_kernel = 7
     with pytest.raises(ValueError):
         _lib.check(_lib.lib().abf_beamform(job, None, 0, _lib.stream_handle()))
+
+
+@pytest.mark.parametrize('scale', [1e-20, 1.0, 1e20])
+@pytest.mark.parametrize('steer', STEERS)
+def test_f16x2_scaling(gpu, scale, steer):
+    """The f16x2 form scales U by a power of two per frequency and the steering weights by one per
+    grid point; maps of CSMs far outside fp16's range (x 1e-20, x 1e20) and of a grid 60-140 m
+    away (weights r or 1/r far from 1) keep the oracle's accuracy."""
+    freqs = np.array([900.0, 5000.0])
+    grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.5, increment=0.04)
+    mics, grid, csm = _scene(48, grid, freqs, z=0.5)
+    csm = csm * np.float32(scale)
+    got = _gpu_maps(csm, freqs, mics, grid, steer, True, 'mfma_f16x2')
+    want = _oracle_maps(csm, freqs, mics, grid, steer, True)
+    assert peak_rel_err(got, want) <= TOL_F16X2
+    far = aa.RectGrid(x_min=-40.0, x_max=40.0, y_min=-40.0, y_max=40.0, z=100.0, increment=8.0)
+    mpos = synthetic.disk_array(48, 1.0, 1)
+    spos = np.array([[8.0], [-16.0], [100.0]])
+    fcsm = synthetic.analytic_csm(mpos, spos, np.array([1.0]), freqs) * np.float32(scale)
+    fm = aa.MicGeom(pos_total=mpos)
+    got = _gpu_maps(fcsm, freqs, fm, far, steer, False, 'mfma_f16x2')
+    want = _oracle_maps(fcsm, freqs, fm, far, steer, False)
+    assert peak_rel_err(got, want) <= TOL_F16X2
