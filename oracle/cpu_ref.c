@@ -35,6 +35,17 @@ int cref_num_threads(void)
 #endif
 }
 
+/* threads of the following parallel loops (OpenMP); no effect without OpenMP */
+void cref_set_threads(int n)
+{
+#ifdef _OPENMP
+    extern void omp_set_num_threads(int);
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
 void cref_dist_mat(const double *gpos, const double *mpos, int64_t ng, int64_t nm, double *rm)
 {
     const double *m0 = mpos, *m1 = mpos + nm, *m2 = mpos + 2 * nm;

@@ -32,6 +32,7 @@ def lib():
         _lib.cref_calc_csm.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int64]
         _lib.cref_dist_mat.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int64, dp]
         _lib.cref_num_threads.restype = ctypes.c_int
+        _lib.cref_set_threads.argtypes = [ctypes.c_int]
         ip = ctypes.POINTER(ctypes.c_int64)
         _lib.cref_psf.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ip, ctypes.c_int64,
                                   ctypes.c_int, dp]
@@ -46,6 +47,11 @@ def _p(a):
 
 def num_threads() -> int:
     return lib().cref_num_threads()
+
+
+def set_threads(n: int) -> None:
+    """Threads of the OpenMP loops that follow."""
+    lib().cref_set_threads(int(n))
 
 
 def dist_mat(gpos, mpos):
