@@ -32,6 +32,7 @@ from .fbeamform import (
     integrate,
     psf_device,
 )
+from .fprocess import RFFT, AutoPowerSpectra, CrossPowerSpectra
 from .grids import Grid, ImportGrid, RectGrid, RectGrid3D
 from .microphones import MicGeom
 from .signals import NoiseGenerator, SignalGenerator, SineGenerator, WNoiseGenerator
@@ -39,6 +40,9 @@ from .sources import PointSource, SourceMixer, TimeSamples, UncorrelatedNoiseSou
 from .spectra import BaseSpectra, PowerSpectra, PowerSpectraImport
 
 __all__ = [
+    'RFFT',
+    'AutoPowerSpectra',
+    'CrossPowerSpectra',
     'BaseSpectra',
     'BeamformerBase',
     'BeamformerCapon',

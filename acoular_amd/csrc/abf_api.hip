@@ -892,7 +892,7 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
         a.b0 = b0;
         a.nb = nb;
         a.P = P;
-        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)cdiv(M, 2 * P), nb), dim3(256),
+        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)cdiv(M, 2 * P), nb), dim3(512),
                            (size_t)P * bs * sizeof(float2), s, a);
         hipLaunchKernelGGL(welch_herk_kernel, dim3(nT * nT, F), dim3(64), 0, s, spec, nb, M, nT,
                            reinterpret_cast<float2 *>(csm_out));
@@ -900,6 +900,86 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
     hipLaunchKernelGGL(welch_finalize_kernel, dim3((unsigned)cdiv((int64_t)M * M, 256), F), dim3(256), 0, s,
                        reinterpret_cast<float2 *>(csm_out), M, (float)scale);
     return hip_status("Welch CSM launch");
+}
+
+size_t abf_rfft_workspace_size(int32_t block_size)
+{
+    if (block_size < 2) return 0;
+    return align256((size_t)block_size / 2 * sizeof(float2));
+}
+
+int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, int32_t block_size, int32_t hop,
+                    int32_t num_blocks, const float *window, const float *bin_scale, void *out, void *workspace,
+                    size_t workspace_bytes, void *stream)
+{
+    const int bs = block_size;
+    if (num_mics < 1 || bs < 2 || (bs & (bs - 1)) || hop < 1 || num_blocks < 0)
+        return fail(ABF_ERR_INVALID, "bad FFT block parameters (block_size must be a power of two >= 2)");
+    if (bs > 16384) return fail(ABF_ERR_UNSUPPORTED, "block_size above 16384 is not supported by the LDS FFT");
+    if (num_blocks > 0 && (int64_t)(num_blocks - 1) * hop + bs > num_samples)
+        return fail(ABF_ERR_INVALID, "blocks exceed the time data");
+    if (num_blocks == 0) return ABF_OK;
+    if (!out || !time || !window) return fail(ABF_ERR_INVALID, "null pointer");
+    if (!workspace || workspace_bytes < abf_rfft_workspace_size(bs)) return fail(ABF_ERR_WORKSPACE, "workspace too small");
+    const hipStream_t s = S(stream);
+    const int M = num_mics;
+    int logbs = 0;
+    while ((1 << logbs) < bs) ++logbs;
+    float2 *tw = reinterpret_cast<float2 *>(workspace);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)welch_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
+        attr = true;
+    }
+    const int P = std::max(1, std::min({16, 16384 / bs, (M + 1) / 2}));
+    for (int b0 = 0; b0 < num_blocks; b0 += 65535) {
+        WelchFftArgs a{};
+        a.time = time;
+        a.window = window;
+        a.twiddle = tw;
+        a.spec = reinterpret_cast<float2 *>(out);
+        a.bin_scale = bin_scale;
+        a.N = num_samples;
+        a.M = M;
+        a.bs = bs;
+        a.logbs = logbs;
+        a.hop = hop;
+        a.b0 = b0;
+        a.nb = std::min(65535, num_blocks - b0);
+        a.P = P;
+        a.block_major = 1;
+        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)cdiv(M, 2 * P), a.nb), dim3(512),
+                           (size_t)P * bs * sizeof(float2), s, a);
+    }
+    return hip_status("block FFT launch");
+}
+
+int abf_auto_power(const void *spec, int64_t n, double scale, float *out, void *stream)
+{
+    if (n < 0) return fail(ABF_ERR_INVALID, "negative size");
+    if (n == 0) return ABF_OK;
+    if (!spec || !out) return fail(ABF_ERR_INVALID, "null pointer");
+    hipLaunchKernelGGL(auto_power_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, S(stream),
+                       reinterpret_cast<const float2 *>(spec), n, (float)scale, out);
+    return hip_status("auto power launch");
+}
+
+int abf_cross_power(const void *spec, int64_t num_snapshots, int32_t num_freqs, int32_t num_mics, double scale,
+                    void *out, void *stream)
+{
+    if (num_snapshots < 0 || num_freqs < 0 || num_mics < 1) return fail(ABF_ERR_INVALID, "bad sizes");
+    const int64_t nbf = num_snapshots * num_freqs;
+    if (nbf == 0) return ABF_OK;
+    if (!spec || !out) return fail(ABF_ERR_INVALID, "null pointer");
+    const int64_t mm = (int64_t)num_mics * num_mics;
+    const int64_t gz = cdiv(nbf, 65535);
+    if (gz > 65535) return fail(ABF_ERR_UNSUPPORTED, "too many snapshots x bins for one call");
+    hipLaunchKernelGGL(cross_power_kernel, dim3((unsigned)cdiv(mm, 256), (unsigned)std::min<int64_t>(nbf, 65535),
+                                                (unsigned)gz),
+                       dim3(256), 0, S(stream), reinterpret_cast<const float2 *>(spec), nbf, num_mics, (float)scale,
+                       reinterpret_cast<float2 *>(out));
+    return hip_status("cross power launch");
 }
 
 } // extern "C"

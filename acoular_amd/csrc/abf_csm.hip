@@ -56,7 +56,7 @@ __global__ void calc_csm_c128_kernel(double2 *__restrict__ csm, const double2 *_
 __device__ __forceinline__ int bitrev(int n, int bits) { return (int)(__builtin_bitreverse32((uint32_t)n) >> (32 - bits)); }
 
 // grid: (ceil(M / 2P), nb); P channel pairs of one Welch block per workgroup
-__global__ __launch_bounds__(256) void welch_fft_kernel(WelchFftArgs a)
+__global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
 {
     extern __shared__ float2 z[]; // [P][bs]
     const int bs = a.bs, P = a.P;
@@ -73,22 +73,43 @@ __global__ __launch_bounds__(256) void welch_fft_kernel(WelchFftArgs a)
         z[p * bs + bitrev(n, a.logbs)] = make_float2(w * xa, w * xb);
     }
     __syncthreads();
-    const int nbf = bs / 2;
-    for (int s = 1; s <= a.logbs; ++s) {
+    // radix-2 decimation-in-time stages, up to three per pass through LDS: a thread takes the
+    // 2^g elements base + j half (j < 2^g) that stages s .. s+g-1 combine only among themselves
+    // and runs those stages in registers
+    for (int s = 1; s <= a.logbs;) {
+        const int g = min(3, a.logbs - s + 1);
         const int half = 1 << (s - 1);
-        const int tstride = bs >> s;
-        for (int idx = threadIdx.x; idx < P * nbf; idx += blockDim.x) {
-            const int p = idx / nbf, bi = idx % nbf;
-            const int grp = bi / half, pos = bi % half;
-            const int i0 = p * bs + grp * 2 * half + pos, i1 = i0 + half;
-            const float2 w = a.twiddle[pos * tstride];
-            const float2 x0 = z[i0], x1 = z[i1];
-            const float tr = x1.x * w.x - x1.y * w.y;
-            const float ti = x1.x * w.y + x1.y * w.x;
-            z[i0] = make_float2(x0.x + tr, x0.y + ti);
-            z[i1] = make_float2(x0.x - tr, x0.y - ti);
+        const int per_chan = bs >> g; // groups per channel
+        for (int idx = threadIdx.x; idx < P * per_chan; idx += blockDim.x) {
+            const int p = idx / per_chan, gi = idx % per_chan;
+            const int blk = gi / half, pos = gi % half;
+            const int base = p * bs + blk * (half << g) + pos;
+            float2 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < (1 << g)) v[j] = z[base + j * half];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                if (t >= g) break;
+                const int tstride = bs >> (s + t);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (j >= (1 << g) || (j & (1 << t))) continue;
+                    const int jj = j | (1 << t);
+                    const float2 w = a.twiddle[(pos + (j & ((1 << t) - 1)) * half) * tstride];
+                    const float2 x1 = v[jj];
+                    const float tr = x1.x * w.x - x1.y * w.y;
+                    const float ti = x1.x * w.y + x1.y * w.x;
+                    v[jj] = make_float2(v[j].x - tr, v[j].y - ti);
+                    v[j] = make_float2(v[j].x + tr, v[j].y + ti);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < (1 << g)) z[base + j * half] = v[j];
         }
         __syncthreads();
+        s += g;
     }
     const int F = bs / 2 + 1;
     for (int idx = threadIdx.x; idx < P * F; idx += blockDim.x) {
@@ -96,9 +117,17 @@ __global__ __launch_bounds__(256) void welch_fft_kernel(WelchFftArgs a)
         const float2 zk = z[p * bs + k];
         const float2 zn = z[p * bs + ((bs - k) & (bs - 1))];
         const int ma = m0 + 2 * p, mb = ma + 1;
-        float2 *dst = a.spec + ((int64_t)k * a.nb + bl) * a.M;
-        if (ma < a.M) dst[ma] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-        if (mb < a.M) dst[mb] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+        float2 *dst = a.block_major ? a.spec + ((int64_t)(a.b0 + bl) * F + k) * a.M
+                                    : a.spec + ((int64_t)k * a.nb + bl) * a.M;
+        float2 va = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+        float2 vb = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+        if (a.bin_scale) {
+            const float sc = a.bin_scale[k];
+            va = make_float2(va.x * sc, va.y * sc);
+            vb = make_float2(vb.x * sc, vb.y * sc);
+        }
+        if (ma < a.M) dst[ma] = va;
+        if (mb < a.M) dst[mb] = vb;
     }
 }
 
@@ -173,6 +202,30 @@ __global__ void welch_finalize_kernel(float2 *__restrict__ csm, int M, float sca
     if (r == c) v.y = 0.f;
     C[(int64_t)r * M + c] = v;
     if (r != c) C[(int64_t)c * M + r] = make_float2(v.x, -v.y);
+}
+
+// AutoPowerSpectra.result (fprocess.py:359-366): |X|^2 * scale per value
+__global__ __launch_bounds__(256) void auto_power_kernel(const float2 *__restrict__ spec, int64_t n, float scale,
+                                                         float *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 x = spec[i];
+    out[i] = (x.x * x.x + x.y * x.y) * scale;
+}
+
+// CrossPowerSpectra.result, calc_mode 'full' (fprocess.py:446-470): per snapshot and bin the
+// matrix X X^H, element (r, c) = scale * X_r conj(X_c) (upper part from calcCSM, lower part its
+// conjugate transpose); grid (ceil(M^2 / 256), snapshots x bins)
+__global__ __launch_bounds__(256) void cross_power_kernel(const float2 *__restrict__ spec, int64_t nbf, int M,
+                                                          float scale, float2 *__restrict__ out)
+{
+    const int64_t sb = blockIdx.y + (int64_t)blockIdx.z * 65535; // snapshot x bin
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= nbf || e >= (int64_t)M * M) return;
+    const int r = (int)(e / M), c = (int)(e % M);
+    const float2 xr = spec[sb * M + r], xc = spec[sb * M + c];
+    out[sb * M * M + e] = make_float2((xr.x * xc.x + xr.y * xc.y) * scale, (xr.y * xc.x - xr.x * xc.y) * scale);
 }
 
 } // namespace abf

@@ -8,9 +8,10 @@ struct WelchFftArgs {
     const float *time;     // [N][M]
     const float *window;   // [bs]
     const float2 *twiddle; // [bs/2]  exp(-2 pi i j / bs)
-    float2 *spec;          // [bs/2+1][nb][M]
+    float2 *spec;          // [bs/2+1][nb][M], or [num_blocks][bs/2+1][M] when block_major
+    const float *bin_scale; // [bs/2+1] factor per frequency bin, or null
     int64_t N;
-    int M, bs, logbs, hop, b0, nb, P;
+    int M, bs, logbs, hop, b0, nb, P, block_major;
 };
 
 __global__ void fill_zero_kernel(float4 *p, int64_t n4);
@@ -19,5 +20,7 @@ __global__ void calc_csm_c128_kernel(double2 *csm, const double2 *spec, int F, i
 __global__ void welch_fft_kernel(WelchFftArgs a);
 __global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nT, float2 *csm);
 __global__ void welch_finalize_kernel(float2 *csm, int M, float scale);
+__global__ void auto_power_kernel(const float2 *spec, int64_t n, float scale, float *out);
+__global__ void cross_power_kernel(const float2 *spec, int64_t nbf, int M, float scale, float2 *out);
 
 } // namespace abf

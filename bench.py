@@ -656,6 +656,85 @@ def damas_bench(args):
     }
 
 
+def rfft_bench(args):
+    """RFFT (block-wise one-sided FFT, fprocess.py:35-180) of the fused workload's time data: 512
+    channels, 10 s at 51.2 kHz (float32, 1.05 GB resident), block size 2048, Hanning, 50 %
+    overlap (499 blocks), 'energy' scaling: all blocks and channels in one abf_rfft_blocks call,
+    output [499][1025][512] complex64.  HBM-bound: algorithmic bytes = every time sample read
+    once + every spectrum value written once."""
+    import torch
+
+    import acoular_amd as aa
+    from acoular_amd import _lib
+    from oracle import fprocess as P
+
+    dev = torch.device('cuda', 0)
+    nch, fs, n, bs = 512, 51200.0, 512000, 2048
+    g = torch.Generator(device=dev).manual_seed(7)
+    tdev = torch.randn((n, nch), generator=g, device=dev, dtype=torch.float32)
+
+    class _Src:
+        sample_freq = fs
+        num_samples = n
+        num_channels = nch
+
+        def result_device(self, num):
+            return tdev
+
+    fft = aa.RFFT(source=_Src(), block_size=bs, window='Hanning', overlap='50%', scaling='energy')
+    hop = bs // 2
+    nb = (n - bs) // hop + 1
+    nf = bs // 2 + 1
+    out = torch.empty((nb, nf, nch), dtype=torch.complex64, device=dev)
+    lib = _lib.lib()
+    wdev = _lib.to_device(np.hanning(bs).astype(np.float32))
+    sdev = _lib.to_device(fft.bin_scale().astype(np.float32))
+    ws = _lib.workspace(lib.abf_rfft_workspace_size(bs))
+
+    def step():
+        _lib.check(lib.abf_rfft_blocks(_lib.ptr(tdev), n, nch, bs, hop, nb, _lib.ptr(wdev), _lib.ptr(sdev),
+                                       _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream_handle()), 'RFFT')
+
+    step()
+    t_step = time_steps(step, max(1, args.steps), args.warmup)
+    t_k = event_time(step, 5)
+    alg_bytes = n * nch * 4.0 + nb * nf * nch * 8.0
+    roof = {'bound': 'hbm', 'achieved': round(alg_bytes / t_k / 1e9, 1), 'peak': 8000.0, 'unit': 'GB/s',
+            'kernel': 'welch_fft_kernel (block-major output)', 'algorithmic_bytes_per_launch': alg_bytes,
+            'traffic': None}
+    roof['frac'] = round(roof['achieved'] / roof['peak'], 4)
+    values = nb * nf * nch
+    log(f'RFFT step {t_step * 1e3:.2f} ms ({t_k * 1e3:.2f} ms kernel time) -> {values / t_step:.4g} spectrum '
+        f'values/s, {roof["achieved"]} GB/s')
+    err = None
+    if not args.no_compare:
+        x = tdev[: 6 * hop + bs].cpu().numpy()
+        want = P.rfft_blocks(x, bs, 'Hanning', '50%', 'energy')
+        got = out[: want.shape[0]].cpu().numpy().reshape(want.shape[0], -1)
+        err = float(np.abs(got - want).max() / np.abs(want).max())
+        log(f'RFFT max error relative to the largest magnitude on 7 blocks: {err:.2e}')
+    cpu = None
+    if not args.no_cpu:
+        x = tdev.cpu().numpy()
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds and done < nb:
+            b = done
+            P.rfft_blocks(x[b * hop : b * hop + bs], bs, 'Hanning', 'None', 'energy')
+            done += 1
+        dt = time.perf_counter() - t0
+        cpu = {'value': done * nf * nch / dt, 'unit': 'spectrum values/s', 'cores': 1, 'kind': 'port',
+               'sample': f'{done} blocks of the same workload (oracle: scipy rfft in float64), {dt:.1f} s'}
+        log(f"CPU: {cpu['value']:.4g} spectrum values/s")
+    return {
+        'metric': 'spectrum values/s (blocks x bins x channels)', 'value': values / t_step,
+        'unit': 'spectrum values/s', 'n_gpus': 1, 'ms_per_step': t_step * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded normal noise)',
+        'config': {'workload': 'RFFT: 512 channels, 10 s at 51.2 kHz, bs 2048, Hanning, 50 %, energy scaling',
+                   'blocks': nb, 'num_freqs': nf, 'channels': nch, 'parallelism': 'single GPU'},
+        'roofline': roof, 'max_rel_err': err, 'cpu_baseline': cpu,
+    }
+
+
 def clean_bench(args):
     """CLEAN on the 64-mic workload of the DAMAS mode (64 mics, RectGrid 41 x 41 at z = 0.3, 65
     bins of block size 128 at 51.2 kHz, analytic complex128 CSM of 3 seeded sources), true level,
@@ -910,7 +989,7 @@ def main():
     ap.add_argument('--tune', action='store_true', help='tune the launch shape now even if the table has it')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig',
-                                                          'damas', 'cleansc', 'clean', 'c1'],
+                                                          'damas', 'cleansc', 'clean', 'rfft', 'c1'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
                          "'serving': the mixed-size request loop (Poisson arrivals) against the unbatched "
                          "plain kernel; 'fused': Welch CSM + beamforming of 512 mics, 10 s at 51.2 kHz, "
@@ -952,6 +1031,9 @@ def main():
         return
     if args.mode == 'damas':
         print(json.dumps(damas_bench(args)), flush=True)
+        return
+    if args.mode == 'rfft':
+        print(json.dumps(rfft_bench(args)), flush=True)
         return
     if args.mode == 'clean':
         print(json.dumps(clean_bench(args)), flush=True)

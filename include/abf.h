@@ -278,6 +278,26 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
                   int32_t num_blocks, const float *window, double scale, void *csm_out, int32_t blocks_per_pass,
                   void *workspace, size_t workspace_bytes, void *stream);
 
+/* RFFT.result (fprocess.py:35-180): one-sided spectra of the blocks at hop*b, b < num_blocks, of
+ * time [num_samples][num_mics] float32 after multiplying by window [bs]:
+ * out[b][k][m] = bin_scale[k] * rfft(window * x_b)[k], complex64, k < bs/2+1 (the reference's
+ * frequency-major, channel-interlaced row).  bin_scale (device, [bs/2+1]) carries the scaling
+ * the caller chose ('none', 'energy', 'amplitude'); block_size a power of two <= 16384. */
+size_t abf_rfft_workspace_size(int32_t block_size);
+int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, int32_t block_size, int32_t hop,
+                    int32_t num_blocks, const float *window, const float *bin_scale, void *out, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
+/* AutoPowerSpectra.result (fprocess.py:286-366): out[i] = |spec[i]|^2 * scale, n complex64 values
+ * to float32. */
+int abf_auto_power(const void *spec, int64_t n, double scale, float *out, void *stream);
+
+/* CrossPowerSpectra.result with calc_mode 'full' (fprocess.py:369-470): for every snapshot s and
+ * bin k of spec [num_snapshots][num_freqs][num_mics] complex64,
+ * out[s][k][r][c] = scale * spec[s][k][r] * conj(spec[s][k][c]) (complex64). */
+int abf_cross_power(const void *spec, int64_t num_snapshots, int32_t num_freqs, int32_t num_mics, double scale,
+                    void *out, void *stream);
+
 const char *abf_last_error(void);
 const char *abf_version(void);
 

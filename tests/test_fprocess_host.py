@@ -1,0 +1,63 @@
+"""Host logic of RFFT / AutoPowerSpectra / CrossPowerSpectra (no GPU): parameters, derived sizes
+and digests as the reference's fprocess tests check them (fprocess.py:35-470)."""
+
+import numpy as np
+import pytest
+
+import acoular_amd as aa
+
+
+def _ts(n=8, fs=64.0):
+    return aa.TimeSamples(data=np.random.RandomState(1).standard_normal((n, 1)), sample_freq=fs)
+
+
+def test_rfft_parameters_and_sizes():
+    fft = aa.RFFT(source=_ts(200), block_size=64)
+    assert fft.num_freqs == 33 and fft.num_samples == 3 and fft.num_channels == 1
+    np.testing.assert_allclose(fft.freqs, np.arange(33))
+    with pytest.raises(ValueError):
+        fft.block_size = 63
+    with pytest.raises(ValueError):
+        fft.scaling = 'psd'
+    with pytest.raises(ValueError):
+        fft.precision = 'float64'
+    s = aa.RFFT(source=_ts(200), block_size=64, scaling='amplitude').bin_scale()
+    assert s[0] == pytest.approx(1 / 64) and s[1] == pytest.approx(2 / 64) and s[-1] == pytest.approx(1 / 64)
+
+
+def test_reflect_source_changes():
+    fft = aa.RFFT(source=_ts(), block_size=64)
+    d, f = fft.digest, fft.freqs
+    fft.source.sample_freq *= 2
+    assert fft.digest != d
+    assert not np.array_equal(fft.freqs, f)
+
+
+def test_digest_changes():
+    fft = aa.RFFT(source=_ts(), block_size=64)
+    for name, value in (('scaling', 'energy'), ('precision', 'complex64'), ('block_size', 16), ('window', 'Hanning'),
+                        ('overlap', '50%')):
+        d = fft.digest
+        setattr(fft, name, value)
+        assert fft.digest != d, name
+    d = fft.digest
+    fft.workers = 4
+    assert fft.digest == d
+
+
+def test_power_spectra_parameters():
+    fft = aa.RFFT(source=_ts(), block_size=16)
+    ap = aa.AutoPowerSpectra(source=fft)
+    assert ap.precision == 'float64' and ap.scaling == 'power' and ap.single_sided
+    assert ap._get_scaling_value() == pytest.approx(2 / 256)
+    ap.scaling = 'psd'
+    assert ap._get_scaling_value() == pytest.approx(2 / 256 * 16 * 64.0)
+    cp = aa.CrossPowerSpectra(source=aa.RFFT(source=aa.TimeSamples(data=np.zeros((8, 3)), sample_freq=1.0),
+                                             block_size=8))
+    assert cp.precision == 'complex128' and cp.num_channels == 9
+    cp.calc_mode = 'upper'
+    assert cp.num_channels == 6
+    with pytest.raises(ValueError):
+        cp.precision = 'float64'
+    with pytest.raises(ValueError):
+        cp.calc_mode = 'diag'
