@@ -10,10 +10,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libabf.so')
 SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f16.hip', 'abf_bf_mfma_f32.hip', 'abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip',
-           'abf_eig.hip', 'abf_deconv.hip', 'abf_sources.hip', 'abf_clean.hip',
+           'abf_eig.hip', 'abf_deconv.hip', 'abf_sources.hip', 'abf_clean.hip', 'abf_resample.hip',
            'abf_api.hip']
-# the batched Hermitian eigensolver of the eigenvalue path
-LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocsolver', '-lrocblas']
+# the batched Hermitian eigensolver of the eigenvalue path; the FFTs of the signal resampling
+LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocsolver', '-lrocblas', '-lrocfft']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 # packed fp32 instructions are disabled on purpose (build flags)
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-Xclang', '-target-feature', '-Xclang',

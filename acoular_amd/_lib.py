@@ -101,6 +101,8 @@ _SIGS = {
     'abf_cleansc': (ctypes.c_int, [ctypes.POINTER(AbfGeometry), _c_dp, ctypes.c_int32, _c_dp, _c_dp, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                    ctypes.c_int32, _c_dp, _c_dp, ctypes.c_size_t, _c_dp]),
+    'abf_resample_workspace_size': (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
+    'abf_resample': (ctypes.c_int, [_c_dp, ctypes.c_int64, ctypes.c_int64, _c_dp, _c_dp, ctypes.c_size_t, _c_dp]),
     'abf_rfft_workspace_size': (ctypes.c_size_t, [ctypes.c_int32]),
     'abf_rfft_blocks': (ctypes.c_int, [_c_dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32, _c_dp, _c_dp, _c_dp, _c_dp, ctypes.c_size_t, _c_dp]),

@@ -131,9 +131,11 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
     }
 }
 
-// grid: (nT * nT, F), one wave per upper 64x64 tile of one frequency
-__global__ __launch_bounds__(64) void welch_herk_kernel(const float2 *__restrict__ spec, int nb, int M, int nT,
-                                                        float2 *__restrict__ csm)
+// grid: (nT * nT, F), one wave per upper 64x64 tile of one frequency.  The waves-per-EU bound
+// keeps the accumulators in VGPRs (158, no AGPRs) so three waves share a SIMD and hide each
+// other's load latency; without it the compiler takes 136 VGPRs + 128 AGPRs, one wave per SIMD.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void welch_herk_kernel(
+    const float2 *__restrict__ spec, int nb, int M, int nT, float2 *__restrict__ csm)
 {
     const int I = blockIdx.x / nT, J = blockIdx.x % nT;
     if (J < I) return;

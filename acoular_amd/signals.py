@@ -2,9 +2,10 @@
 
 A generator describes one source signal: ``signal()`` returns ``num_samples`` float64 values at
 ``sample_freq`` and ``usignal(factor)`` the same signal resampled to ``factor`` times the rate
-(``signals.py:93-126``, Fourier resampling).  These are single 1-D sequences computed on the host
-exactly as the reference computes them; the per-microphone propagation and the per-channel noise
-of whole arrays run on the GPU (``acoular_amd.sources``).
+(``signals.py:93-126``, Fourier resampling).  ``signal()`` is computed on the host exactly as the
+reference computes it; ``signal_device()`` (white noise generated on the GPU) and
+``usignal_device()`` (the resampling on the GPU, ``abf_resample``) feed the point sources, whose
+per-microphone propagation runs on the GPU as well (``acoular_amd.sources``).
 """
 
 from __future__ import annotations
@@ -38,9 +39,33 @@ class SignalGenerator(HasParams):
 
     def usignal(self, factor: int) -> np.ndarray:
         """The signal resampled to ``factor * num_samples`` samples (``signals.py:93-126``)."""
-        from scipy.signal import resample
+        return self.usignal_device(factor).cpu().numpy()
 
-        return resample(self.signal(), factor * self.num_samples)
+    def signal_device(self):
+        """``signal()`` as a float64 device tensor."""
+        from . import _lib
+
+        return _lib.to_device(np.ascontiguousarray(self.signal(), dtype=np.float64))
+
+    def usignal_device(self, factor: int):
+        """``usignal(factor)`` as a float64 device tensor: the Fourier resampling of
+        ``scipy.signal.resample`` (``signals.py:93-126``) on the device (``abf_resample``)."""
+        from . import _lib
+
+        factor = int(factor)
+        if factor < 1:
+            raise ValueError('factor must be >= 1')
+        torch = _lib.torch()
+        x = self.signal_device().reshape(-1).contiguous()
+        n = int(x.numel())
+        y = torch.empty(factor * n, dtype=torch.float64, device=x.device)
+        if n == 0:
+            return y
+        lib = _lib.lib()
+        ws = _lib.workspace(lib.abf_resample_workspace_size(n, factor * n))
+        _lib.check(lib.abf_resample(_lib.ptr(x), n, factor * n, _lib.ptr(y), _lib.ptr(ws), ws.numel(),
+                                    _lib.stream_handle()), 'usignal')
+        return y
 
 
 class NoiseGenerator(SignalGenerator):
@@ -67,6 +92,22 @@ class WNoiseGenerator(NoiseGenerator):
     def signal(self) -> np.ndarray:
         rnd_gen = np.random.RandomState(self.seed)
         return self.rms * rnd_gen.standard_normal(self.num_samples)
+
+    def signal_device(self):
+        """The same numbers generated on the device (``abf_white_noise``: numpy's legacy MT19937
+        stream and polar method; equal to ``signal()`` up to the last bit of the float64 log)."""
+        from . import _lib
+
+        torch = _lib.torch()
+        if not 0 <= self.seed < 2**32:
+            raise ValueError('Seed must be between 0 and 2**32 - 1')
+        out = torch.empty((self.num_samples, 1), dtype=torch.float64, device=_lib.device())
+        if self.num_samples == 0:
+            return out.reshape(-1)
+        sd = torch.from_numpy(np.array([self.seed], dtype=np.uint32).view(np.int32)).to(out.device)
+        _lib.check(_lib.lib().abf_white_noise(_lib.ptr(sd), 1, self.num_samples, float(self.rms), _lib.ptr(out),
+                                              _lib.stream_handle()), 'WNoiseGenerator')
+        return out.reshape(-1)
 
 
 class SineGenerator(SignalGenerator):

@@ -125,29 +125,39 @@ class PointSource(HasParams):
             warnings.warn('Source and microphone locations are identical.', Warning, stacklevel=3)
         return rm
 
-    def _host_inputs(self):
-        """Upsampled signal, distances and start indices (host work, computed as the reference does)."""
+    def _geometry(self):
+        """Distances and start indices, computed on the host as the reference does (sources.py:804)."""
         rm = self._distances()
-        signal = np.ascontiguousarray(self.signal.usignal(self.up), dtype=np.float64)
-        # the reference's start index, same expression (sources.py:804)
         ind = (-rm / self.env.c - self.start_t + self.start) * self.sample_freq * self.up
-        return signal, rm, ind
+        return rm, ind
+
+    def _host_inputs(self):
+        """Upsampled signal (host copy of the device resampling), distances and start indices."""
+        rm, ind = self._geometry()
+        return np.ascontiguousarray(self.signal.usignal(self.up), dtype=np.float64), rm, ind
 
     def result_device(self, num: int = 128, host_inputs=None):
         """All microphone signals ``[T, num_mics]`` (float64, device), T as the block generator yields."""
         from . import _lib
 
         torch = _lib.torch()
-        signal, rm, ind = host_inputs if host_inputs is not None else self._host_inputs()
+        dev = _lib.device()
+        if host_inputs is not None:
+            signal, rm, ind = host_inputs
+            sig = torch.from_numpy(np.ascontiguousarray(signal, dtype=np.float64)).to(dev)
+        else:
+            rm, ind = self._geometry()
+            sig = None
         estimate = self.num_samples + (self.start_t - self.start) * self.sample_freq
         T = _block_plan(self.num_samples, estimate, int(num))
         M = self.num_channels
-        L = signal.shape[0]
-        dev = _lib.device()
         out = torch.empty((T, M), dtype=torch.float64, device=dev)
         if T == 0:
             return out
-        sig = torch.from_numpy(signal).to(dev)
+        if sig is None:
+            # the upsampled signal is generated and resampled on the device
+            sig = self.signal.usignal_device(self.up)
+        L = int(sig.shape[0])
         ind0 = torch.from_numpy(np.ascontiguousarray(ind[0])).to(dev)
         rmd = torch.from_numpy(np.ascontiguousarray(rm[0])).to(dev)
         lib = _lib.lib()
@@ -283,16 +293,7 @@ class SourceMixer(HasParams):
         if weights.shape[0] != len(self.sources):
             msg = f'weights must match the number of sources ({len(self.sources)})'
             raise ValueError(msg)
-        # the point sources' host work (Fourier resampling of each signal) runs concurrently
-        pts = [s for s in self.sources if isinstance(s, PointSource)]
-        inputs = {}
-        if len(pts) > 1:
-            from concurrent.futures import ThreadPoolExecutor
-
-            with ThreadPoolExecutor(min(len(pts), 8)) as ex:
-                inputs = dict(zip(map(id, pts), ex.map(PointSource._host_inputs, pts)))
-        parts = [s.result_device(num, inputs[id(s)]) if id(s) in inputs else s.result_device(num)
-                 for s in self.sources]
+        parts = [s.result_device(num) for s in self.sources]
         T = min(p.shape[0] for p in parts)
         acc = _lib.torch().empty((T, self.num_channels), dtype=parts[0].dtype, device=parts[0].device)
         lib = _lib.lib()

@@ -943,14 +943,15 @@ def c1_bench(args):
         log(f'max relative map error on bins {sample} (oracle Welch + beamformer on the same samples): {err:.2e}')
     cpu = None
     if not args.no_cpu:
-        # the oracle pipeline on the host: sources (numpy), Welch (numpy), beamformer (C + OpenMP)
+        # the oracle pipeline on the host: sources (numpy, scipy resampling), Welch (numpy), beamformer
+        # (C + OpenMP)
         t0 = time.perf_counter()
         parts = []
         for s in srcs:
             rm = s.env.apparent_r(s.loc, s.mics.pos).reshape(1, -1)
             ind0 = (-rm / s.env.c - s.start_t + s.start) * s.sample_freq * s.up
-            parts.append(OS.point_source(s.signal.usignal(s.up), rm, ind0, s.up, s.prepadding, s.num_samples, 128,
-                                         float(s.num_samples)))
+            parts.append(OS.point_source(OS.usignal(s.signal.signal(), s.up), rm, ind0, s.up, s.prepadding,
+                                         s.num_samples, 128, float(s.num_samples)))
         parts.append(OS.white_noise(np.arange(64) + noise.signal.seed, noise.num_samples, noise.signal.rms))
         data = OS.mix(parts, [1.0] * 4)
         csm = OSp.welch_csm(data, 128, 'Hanning', '50%')

@@ -298,6 +298,14 @@ int abf_auto_power(const void *spec, int64_t n, double scale, float *out, void *
 int abf_cross_power(const void *spec, int64_t num_snapshots, int32_t num_freqs, int32_t num_mics, double scale,
                     void *out, void *stream);
 
+/* SignalGenerator.usignal (signals.py:93-126, scipy.signal.resample of a real signal): y [num]
+ * float64 = the n samples of x resampled to num >= n samples through their one-sided spectrum
+ * (zero-padded, the Nyquist bin of an even n halved when num > n) and scaled by num / n.  The
+ * workspace query plans the transforms (rocFFT, double precision) and needs the device. */
+size_t abf_resample_workspace_size(int64_t num_samples, int64_t num);
+int abf_resample(const double *x, int64_t num_samples, int64_t num, double *y, void *workspace,
+                 size_t workspace_bytes, void *stream);
+
 const char *abf_last_error(void);
 const char *abf_version(void);
 

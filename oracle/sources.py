@@ -32,6 +32,17 @@ def _fill(out, signal, rm, ind, blocksize, up, prepadding):
         ind += up
 
 
+def usignal(signal, factor: int) -> np.ndarray:
+    """``SignalGenerator.usignal`` (``signals.py:93-126``): ``scipy.signal.resample`` of the signal
+    to ``factor`` times its length (the reference's own call; scipy's published algorithm: the
+    one-sided spectrum zero-padded, the Nyquist bin of an even length halved, scaled by
+    num / n)."""
+    from scipy.signal import resample
+
+    x = np.asarray(signal, dtype=np.float64)
+    return resample(x, int(factor) * x.shape[0])
+
+
 def point_source(usignal, rm, ind0, up, prepadding, num_samples, num, estimate):
     """Concatenated blocks of ``PointSource.result(num)`` for the start index ``ind0 [1, M]``."""
     signal = np.asarray(usignal, dtype=np.float64)

@@ -21,10 +21,37 @@ def _mics(nm=9, seed=3):
 
 
 def _oracle_ps(ps, num):
+    """The oracle's microphone signals from the device's own upsampled signal (the propagation
+    kernel is compared bit for bit; the resampling has its own test)."""
     rm = ps.env.apparent_r(ps.loc, ps.mics.pos).reshape(1, -1)
     ind0 = (-rm / ps.env.c - ps.start_t + ps.start) * ps.sample_freq * ps.up
     est = ps.num_samples + (ps.start_t - ps.start) * ps.sample_freq
     return O.point_source(ps.signal.usignal(ps.up), rm, ind0, ps.up, ps.prepadding, ps.num_samples, num, est)
+
+
+@pytest.mark.parametrize(('n', 'up'), [(51200, 16), (1001, 16), (6, 16), (4096, 1), (77, 3), (2, 2)])
+def test_device_resampling_matches_scipy(gpu, n, up):
+    """abf_resample against scipy.signal.resample (the reference's usignal): even and odd lengths
+    (the Nyquist split), factor 1, the 1 s / 16x case of the point sources; to 1e-12 of the
+    largest value (float64 FFTs of different libraries)."""
+    sig = aa.WNoiseGenerator(sample_freq=51200.0, num_samples=n, seed=n)
+    x = sig.signal()
+    got = sig.usignal_device(up).cpu().numpy()
+    want = O.usignal(x, up)
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    sine = aa.SineGenerator(sample_freq=1000.0, num_samples=n, freq=37.0, amplitude=2.0)
+    got = sine.usignal(up)
+    want = O.usignal(sine.signal(), up)
+    assert np.abs(got - want).max() <= 1e-12 * max(np.abs(want).max(), 1e-300)
+
+
+def test_device_white_noise_signal(gpu):
+    sig = aa.WNoiseGenerator(sample_freq=1.0, num_samples=5000, seed=4294967295, rms=0.3)
+    got = sig.signal_device().cpu().numpy()
+    np.testing.assert_allclose(got, sig.signal(), rtol=0, atol=1.5e-15)
+    with pytest.raises(ValueError):
+        aa.WNoiseGenerator(num_samples=3, seed=-1).signal_device()
 
 
 @pytest.mark.parametrize('prepadding', ['loop', 'zeros'])

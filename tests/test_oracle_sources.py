@@ -9,6 +9,7 @@
 """
 
 import numpy as np
+import pytest
 
 from oracle import sources as O
 
@@ -44,3 +45,13 @@ def test_white_noise_known_values():
     w = O.white_noise([0], 6, 1.0)[:, 0]
     np.testing.assert_allclose(w, [1.764052345967664, 0.4001572083672233, 0.9787379841057392, 2.240893199201458,
                                    1.8675579901499675, -0.977277879876411], rtol=1e-15)
+
+
+@pytest.mark.parametrize(('n', 'k', 'up'), [(64, 5, 16), (51, 7, 4), (200, 99, 2), (10, 1, 1)])
+def test_usignal_of_a_periodic_tone_is_the_tone(n, k, up):
+    """Fourier resampling is exact for a tone that is periodic in the record and below Nyquist:
+    sin(2 pi k t / n) sampled n times becomes the same sine at up n points."""
+    x = np.sin(2 * np.pi * k * np.arange(n) / n + 0.4)
+    y = O.usignal(x, up)
+    assert y.shape == (up * n,)
+    np.testing.assert_allclose(y, np.sin(2 * np.pi * k * np.arange(up * n) / (up * n) + 0.4), atol=1e-12)
