@@ -50,22 +50,47 @@ class SignalGenerator(HasParams):
     def usignal_device(self, factor: int):
         """``usignal(factor)`` as a float64 device tensor: the Fourier resampling of
         ``scipy.signal.resample`` (``signals.py:93-126``) on the device (``abf_resample``)."""
-        from . import _lib
+        return resample_device(self.signal_device(), factor)
 
-        factor = int(factor)
-        if factor < 1:
-            raise ValueError('factor must be >= 1')
-        torch = _lib.torch()
-        x = self.signal_device().reshape(-1).contiguous()
-        n = int(x.numel())
-        y = torch.empty(factor * n, dtype=torch.float64, device=x.device)
-        if n == 0:
-            return y
-        lib = _lib.lib()
-        ws = _lib.workspace(lib.abf_resample_workspace_size(n, factor * n))
-        _lib.check(lib.abf_resample(_lib.ptr(x), n, factor * n, _lib.ptr(y), _lib.ptr(ws), ws.numel(),
-                                    _lib.stream_handle()), 'usignal')
+
+def resample_device(x, factor: int):
+    """A float64 device signal resampled to ``factor`` times its length (``abf_resample``, the
+    algorithm of ``scipy.signal.resample`` that ``usignal`` calls)."""
+    from . import _lib
+
+    factor = int(factor)
+    if factor < 1:
+        raise ValueError('factor must be >= 1')
+    torch = _lib.torch()
+    x = x.reshape(-1).to(torch.float64).contiguous()
+    n = int(x.numel())
+    y = torch.empty(factor * n, dtype=torch.float64, device=x.device)
+    if n == 0:
         return y
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.abf_resample_workspace_size(n, factor * n))
+    _lib.check(lib.abf_resample(_lib.ptr(x), n, factor * n, _lib.ptr(y), _lib.ptr(ws), ws.numel(),
+                                _lib.stream_handle()), 'usignal')
+    return y
+
+
+def white_noise_device(seeds, num_samples: int, rms=1.0):
+    """Standard normal streams of numpy's legacy ``RandomState(seed)`` for every seed, times
+    ``rms`` (``abf_white_noise``, one workgroup per stream): float64 device tensor
+    ``[num_samples, len(seeds)]``."""
+    from . import _lib
+
+    torch = _lib.torch()
+    seeds = np.asarray(seeds, dtype=np.int64).ravel()
+    if seeds.size and (seeds.min() < 0 or seeds.max() > 2**32 - 1):
+        raise ValueError('Seed must be between 0 and 2**32 - 1')
+    out = torch.empty((int(num_samples), seeds.size), dtype=torch.float64, device=_lib.device())
+    if num_samples == 0 or seeds.size == 0:
+        return out
+    sd = torch.from_numpy(seeds.astype(np.uint32).view(np.int32)).to(out.device)
+    _lib.check(_lib.lib().abf_white_noise(_lib.ptr(sd), seeds.size, int(num_samples), float(rms), _lib.ptr(out),
+                                          _lib.stream_handle()), 'white noise')
+    return out
 
 
 class NoiseGenerator(SignalGenerator):
@@ -96,18 +121,7 @@ class WNoiseGenerator(NoiseGenerator):
     def signal_device(self):
         """The same numbers generated on the device (``abf_white_noise``: numpy's legacy MT19937
         stream and polar method; equal to ``signal()`` up to the last bit of the float64 log)."""
-        from . import _lib
-
-        torch = _lib.torch()
-        if not 0 <= self.seed < 2**32:
-            raise ValueError('Seed must be between 0 and 2**32 - 1')
-        out = torch.empty((self.num_samples, 1), dtype=torch.float64, device=_lib.device())
-        if self.num_samples == 0:
-            return out.reshape(-1)
-        sd = torch.from_numpy(np.array([self.seed], dtype=np.uint32).view(np.int32)).to(out.device)
-        _lib.check(_lib.lib().abf_white_noise(_lib.ptr(sd), 1, self.num_samples, float(self.rms), _lib.ptr(out),
-                                              _lib.stream_handle()), 'WNoiseGenerator')
-        return out.reshape(-1)
+        return white_noise_device([self.seed], self.num_samples, self.rms).reshape(-1)
 
 
 class SineGenerator(SignalGenerator):

@@ -136,9 +136,11 @@ class PointSource(HasParams):
         rm, ind = self._geometry()
         return np.ascontiguousarray(self.signal.usignal(self.up), dtype=np.float64), rm, ind
 
-    def result_device(self, num: int = 128, host_inputs=None):
-        """All microphone signals ``[T, num_mics]`` (float64, device), T as the block generator yields."""
+    def result_device(self, num: int = 128, host_inputs=None, raw_signal=None):
+        """All microphone signals ``[T, num_mics]`` (float64, device), T as the block generator yields.
+        ``raw_signal`` (a device copy of ``signal.signal()``) skips generating it here."""
         from . import _lib
+        from .signals import resample_device
 
         torch = _lib.torch()
         dev = _lib.device()
@@ -156,7 +158,8 @@ class PointSource(HasParams):
             return out
         if sig is None:
             # the upsampled signal is generated and resampled on the device
-            sig = self.signal.usignal_device(self.up)
+            raw = raw_signal if raw_signal is not None else self.signal.signal_device()
+            sig = resample_device(raw, self.up)
         L = int(sig.shape[0])
         ind0 = torch.from_numpy(np.ascontiguousarray(ind[0])).to(dev)
         rmd = torch.from_numpy(np.ascontiguousarray(rm[0])).to(dev)
@@ -218,20 +221,11 @@ class UncorrelatedNoiseSource(HasParams):
 
     def result_device(self, num: int = 128):
         """All channels ``[num_samples, num_mics]`` (float64, device)."""
-        from . import _lib
-        from .signals import WNoiseGenerator
+        from .signals import WNoiseGenerator, white_noise_device
 
         if type(self.signal) is not WNoiseGenerator:
             raise NotImplementedError('device noise generation supports WNoiseGenerator signals')
-        torch = _lib.torch()
-        seeds = self._seeds()
-        dev = _lib.device()
-        out = torch.empty((self.num_samples, self.num_channels), dtype=torch.float64, device=dev)
-        sd = torch.from_numpy(seeds.view(np.int32)).to(dev)
-        _lib.check(_lib.lib().abf_white_noise(_lib.ptr(sd), self.num_channels, self.num_samples,
-                                              float(self.signal.rms), _lib.ptr(out), _lib.stream_handle()),
-                   'UncorrelatedNoiseSource')
-        return out
+        return white_noise_device(self._seeds(), self.num_samples, self.signal.rms)
 
     def result(self, num: int = 128):
         yield from _blocks(self.result_device(num), num)
@@ -293,7 +287,7 @@ class SourceMixer(HasParams):
         if weights.shape[0] != len(self.sources):
             msg = f'weights must match the number of sources ({len(self.sources)})'
             raise ValueError(msg)
-        parts = [s.result_device(num) for s in self.sources]
+        parts = self._parts_device(num)
         T = min(p.shape[0] for p in parts)
         acc = _lib.torch().empty((T, self.num_channels), dtype=parts[0].dtype, device=parts[0].device)
         lib = _lib.lib()
@@ -302,6 +296,46 @@ class SourceMixer(HasParams):
             _lib.check(lib.abf_mix(_lib.ptr(src), float(weights[j]), src.numel(), int(j == 0), _lib.ptr(acc),
                                    _lib.stream_handle()), 'SourceMixer')
         return acc
+
+    def _parts_device(self, num):
+        """Every source's signals on the device.  The white-noise streams of all sources (the point
+        sources' signals and the uncorrelated noise channels) come from one ``abf_white_noise``
+        launch, one workgroup per stream, each scaled by its rms afterwards (``abf_mix``, the same
+        single product as generating it scaled)."""
+        from . import _lib
+        from .signals import WNoiseGenerator, white_noise_device
+
+        torch = _lib.torch()
+        streams = []  # (source index, seeds, rms)
+        for i, s in enumerate(self.sources):
+            if type(getattr(s, 'signal', None)) is not WNoiseGenerator:
+                continue
+            if isinstance(s, PointSource):
+                streams.append((i, np.array([s.signal.seed], dtype=np.int64), s.signal.rms))
+            elif isinstance(s, UncorrelatedNoiseSource):
+                streams.append((i, s._seeds().astype(np.int64), s.signal.rms))
+        pre = {}
+        if len(streams) > 1:
+            n = self.sources[streams[0][0]].num_samples
+            base = white_noise_device(np.concatenate([sd for _, sd, _ in streams]), n, 1.0)
+            lib = _lib.lib()
+            c = 0
+            for i, sd, rms in streams:
+                block = base[:, c : c + sd.size].contiguous()
+                scaled = torch.empty_like(block)
+                _lib.check(lib.abf_mix(_lib.ptr(block), float(rms), block.numel(), 1, _lib.ptr(scaled),
+                                       _lib.stream_handle()), 'white noise scaling')
+                pre[i] = scaled
+                c += sd.size
+        parts = []
+        for i, s in enumerate(self.sources):
+            if i in pre and isinstance(s, PointSource):
+                parts.append(s.result_device(num, raw_signal=pre[i].reshape(-1)))
+            elif i in pre:
+                parts.append(pre[i])
+            else:
+                parts.append(s.result_device(num))
+        return parts
 
     def result(self, num: int = 128):
         yield from _blocks(self.result_device(num), num)
