@@ -131,20 +131,25 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
     }
 }
 
-// grid: (nT * nT, F), one wave per upper 64x64 tile of one frequency.  The waves-per-EU bound
-// keeps the accumulators in VGPRs (158, no AGPRs) so three waves share a SIMD and hide each
-// other's load latency; without it the compiler takes 136 VGPRs + 128 AGPRs, one wave per SIMD.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void welch_herk_kernel(
-    const float2 *__restrict__ spec, int nb, int M, int nT, float2 *__restrict__ csm)
+// grid: (nTr * nTc, F), one wave per tile of 16 TR rows x 64 columns of one frequency on or above
+// the diagonal (TR = 4: 64 x 64 tiles, 128 accumulator registers; TR = 2: 32 x 64 tiles, 64
+// accumulator registers and twice the waves per SIMD, one third more loads per MFMA).  The
+// waves-per-EU bound keeps the accumulators in VGPRs so several waves share a SIMD and hide each
+// other's load latency; without it the compiler takes 136 VGPRs + 128 AGPRs for TR = 4, one wave
+// per SIMD.
+template <int TR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) void welch_herk_kernel(
+    const float2 *__restrict__ spec, int nb, int M, int nTc, float2 *__restrict__ csm)
 {
-    const int I = blockIdx.x / nT, J = blockIdx.x % nT;
-    if (J < I) return;
+    const int I = blockIdx.x / nTc, J = blockIdx.x % nTc;
+    const int r0 = I * 16 * TR, c0 = J * 64;
+    if (c0 + 63 < r0) return; // entirely below the diagonal
     const int f = blockIdx.y;
     const int l = threadIdx.x;
     const float2 *X = spec + (int64_t)f * nb * M;
-    f4 accr[4][4], acci[4][4];
+    f4 accr[TR][4], acci[TR][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < TR; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             accr[a][b] = f4{0.f, 0.f, 0.f, 0.f};
@@ -153,16 +158,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void we
     for (int k0 = 0; k0 < nb; k0 += 4) {
         const int b = k0 + (l >> 4);
         const bool bv = b < nb;
-        float2 ra[4], ca[4];
+        float2 ra[TR], ca[4];
+#pragma unroll
+        for (int t = 0; t < TR; ++t) {
+            const int r = r0 + t * 16 + (l & 15);
+            ra[t] = (bv && r < M) ? X[(int64_t)b * M + r] : make_float2(0.f, 0.f);
+        }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const int r = I * 64 + t * 16 + (l & 15);
-            const int c = J * 64 + t * 16 + (l & 15);
-            ra[t] = (bv && r < M) ? X[(int64_t)b * M + r] : make_float2(0.f, 0.f);
+            const int c = c0 + t * 16 + (l & 15);
             ca[t] = (bv && c < M) ? X[(int64_t)b * M + c] : make_float2(0.f, 0.f);
         }
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti)
+        for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
             for (int tj = 0; tj < 4; ++tj) {
                 // C_rc = sum_b X_r conj(X_c)
@@ -174,13 +182,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void we
     }
     float2 *C = csm + (int64_t)f * M * M;
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
+    for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
         for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int r = I * 64 + ti * 16 + (l >> 4) * 4 + i;
-                const int c = J * 64 + tj * 16 + (l & 15);
+                const int r = r0 + ti * 16 + (l >> 4) * 4 + i;
+                const int c = c0 + tj * 16 + (l & 15);
                 if (r < M && c < M && r <= c) {
                     float2 v = C[(int64_t)r * M + c];
                     v.x += accr[ti][tj][i];
@@ -189,6 +197,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void we
                 }
             }
 }
+
+template __global__ void welch_herk_kernel<4>(const float2 *, int, int, int, float2 *);
+template __global__ void welch_herk_kernel<2>(const float2 *, int, int, int, float2 *);
 
 __global__ void welch_finalize_kernel(float2 *__restrict__ csm, int M, float scale)
 {

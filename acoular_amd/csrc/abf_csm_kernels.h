@@ -18,7 +18,7 @@ __global__ void fill_zero_kernel(float4 *p, int64_t n4);
 __global__ void calc_csm_c64_kernel(float2 *csm, const float2 *spec, int F, int M);
 __global__ void calc_csm_c128_kernel(double2 *csm, const double2 *spec, int F, int M);
 __global__ void welch_fft_kernel(WelchFftArgs a);
-__global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nT, float2 *csm);
+template <int TR> __global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nTc, float2 *csm);
 __global__ void welch_finalize_kernel(float2 *csm, int M, float scale);
 __global__ void auto_power_kernel(const float2 *spec, int64_t n, float scale, float *out);
 __global__ void cross_power_kernel(const float2 *spec, int64_t nbf, int M, float scale, float2 *out);
