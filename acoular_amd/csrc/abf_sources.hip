@@ -17,11 +17,11 @@
 // RandomState(seed).standard_normal -- the MT19937 generator of Matsumoto & Nishimura (1998)
 // seeded with Knuth's multiplier 1812433253, doubles formed from two 32-bit outputs
 // ((a >> 5) * 2^26 + (b >> 6)) / 2^53, and the polar Box-Muller method returning f*x2 first and
-// f*x1 on the next call.  One 64-lane workgroup per channel keeps the 624-word state in LDS,
-// twists it in three parallel phases, tempers a whole batch of outputs, and evaluates 64
-// candidate pairs at a time: candidates consume the output stream four words at a time whether
-// accepted or not, so a ballot and a prefix count place each accepted pair's two normals exactly
-// where the sequential generator puts them.  Products and sums are written without fused
+// f*x1 on the next call.  One 256-thread workgroup per channel keeps the 624-word state in LDS,
+// twists it in three parallel phases, tempers a whole batch of outputs, and evaluates all (at
+// most 156) candidate pairs of a twist at once: candidates consume the output stream four words
+// at a time whether accepted or not, so the waves' ballots and a prefix count place each accepted
+// pair's two normals exactly where the sequential generator puts them.  Products and sums are written without fused
 // multiply-add (as numpy's baseline x86-64 build computes them; contraction is off for this
 // file).  log(r2) is taken as
 // log1p(r2 - 1) for r2 >= 1/2 (r2 - 1 is exact there), which keeps the device library's result
@@ -102,26 +102,24 @@ __device__ __forceinline__ uint32_t mt_next(uint32_t cur, uint32_t nxt, uint32_t
     return far ^ (y >> 1) ^ ((y & 1u) ? kMtA : 0u);
 }
 
-// one twist of the 624-word state by one wave: kk in [0, 227) reads old words only, [227, 454)
-// reads words the first phase renewed, [454, 624) words the second phase renewed (and the last
-// word the renewed word 0)
-__device__ void mt_twist(uint32_t *mt, int lane)
+constexpr int kWnT = 256; // threads per stream: one step per twist phase, one candidate round per twist
+
+// one twist of the 624-word state by the kWnT threads of a workgroup: kk in [0, 227) reads old
+// words only, [227, 454) reads words the first phase renewed, [454, 624) words the second phase
+// renewed (and the last word the renewed word 0)
+__device__ void mt_twist(uint32_t *mt, int tid)
 {
-    uint32_t v[4];
-    int n = 0;
-    for (int kk = lane; kk < kMtN - kMtM; kk += 64) v[n++] = mt_next(mt[kk], mt[kk + 1], mt[kk + kMtM]);
+    uint32_t v = 0;
+    if (tid < kMtN - kMtM) v = mt_next(mt[tid], mt[tid + 1], mt[tid + kMtM]);
     __syncthreads();
-    n = 0;
-    for (int kk = lane; kk < kMtN - kMtM; kk += 64) mt[kk] = v[n++];
+    if (tid < kMtN - kMtM) mt[tid] = v;
     __syncthreads();
     for (int base = kMtN - kMtM; base < kMtN; base += kMtN - kMtM) {
-        const int end = min(kMtN, base + (kMtN - kMtM));
-        n = 0;
-        for (int kk = base + lane; kk < end; kk += 64)
-            v[n++] = mt_next(mt[kk], mt[(kk + 1) % kMtN], mt[kk - (kMtN - kMtM)]);
+        const int kk = base + tid;
+        const bool on = tid < kMtN - kMtM && kk < kMtN;
+        if (on) v = mt_next(mt[kk], mt[(kk + 1) % kMtN], mt[kk - (kMtN - kMtM)]);
         __syncthreads();
-        n = 0;
-        for (int kk = base + lane; kk < end; kk += 64) mt[kk] = v[n++];
+        if (on) mt[kk] = v;
         __syncthreads();
     }
 }
@@ -131,13 +129,19 @@ __device__ __forceinline__ double mt_double(uint32_t a, uint32_t b)
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
 }
 
-__global__ __launch_bounds__(64) void wnoise_kernel(const uint32_t *__restrict__ seeds, int M, int64_t N, double rms,
-                                                    double *__restrict__ out)
+// One workgroup per channel (stream).  After each twist the 624 tempered words (after the < 4
+// words left over from the previous twist) form candidates of 4 words = two uniform doubles;
+// every thread tests one candidate with the polar method and the accepted pairs are placed in
+// candidate order through a workgroup prefix of the waves' ballots, so the output is numpy's
+// sequence whatever the thread count.
+__global__ __launch_bounds__(kWnT) void wnoise_kernel(const uint32_t *__restrict__ seeds, int M, int64_t N, double rms,
+                                                      double *__restrict__ out)
 {
     __shared__ uint32_t mt[kMtN];
     __shared__ uint32_t words[kMtN + 4];
-    const int ch = blockIdx.x, lane = threadIdx.x;
-    if (lane == 0) {
+    __shared__ int s_cnt[kWnT / 64];
+    const int ch = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) {
         uint32_t s = seeds[ch];
         for (int pos = 0; pos < kMtN; ++pos) {
             mt[pos] = s;
@@ -146,39 +150,44 @@ __global__ __launch_bounds__(64) void wnoise_kernel(const uint32_t *__restrict__
     }
     __syncthreads();
     int64_t produced = 0;
-    int left = 0; // words carried over from the previous batch (< 4)
+    int left = 0; // words carried over from the previous twist (< 4)
     while (produced < N) {
-        mt_twist(mt, lane);
-        for (int i = lane; i < kMtN; i += 64) words[left + i] = mt_temper(mt[i]);
+        mt_twist(mt, tid);
+        for (int i = tid; i < kMtN; i += kWnT) words[left + i] = mt_temper(mt[i]);
         __syncthreads();
-        const int total = left + kMtN, ncand = total / 4;
-        for (int c0 = 0; c0 < ncand && produced < N; c0 += 64) {
-            const int c = c0 + lane;
-            bool acc = false;
-            double f = 0.0, x1 = 0.0, x2 = 0.0;
-            if (c < ncand) {
-                x1 = 2.0 * mt_double(words[4 * c], words[4 * c + 1]) - 1.0;
-                x2 = 2.0 * mt_double(words[4 * c + 2], words[4 * c + 3]) - 1.0;
-                const double r2 = x1 * x1 + x2 * x2;
-                acc = !(r2 >= 1.0 || r2 == 0.0);
-                // log(r2) keeps its relative accuracy near r2 = 1 as log1p of the exact r2 - 1
-                const double lg = r2 >= 0.5 ? log1p(r2 - 1.0) : log(r2);
-                if (acc) f = sqrt(-2.0 * lg / r2);
-            }
-            const uint64_t mask = __ballot(acc);
-            const int64_t rank = __popcll(mask & ((1ull << lane) - 1ull));
-            if (acc) {
-                const int64_t pos = produced + 2 * rank;
-                if (pos < N) out[pos * M + ch] = rms * (f * x2);
-                if (pos + 1 < N) out[(pos + 1) * M + ch] = rms * (f * x1);
-            }
-            produced += 2 * (int64_t)__popcll(mask);
+        const int total = left + kMtN, ncand = total / 4; // ncand <= 156 < kWnT: one round
+        const int c = tid;
+        bool acc = false;
+        double f = 0.0, x1 = 0.0, x2 = 0.0;
+        if (c < ncand) {
+            x1 = 2.0 * mt_double(words[4 * c], words[4 * c + 1]) - 1.0;
+            x2 = 2.0 * mt_double(words[4 * c + 2], words[4 * c + 3]) - 1.0;
+            const double r2 = x1 * x1 + x2 * x2;
+            acc = !(r2 >= 1.0 || r2 == 0.0);
+            // log(r2) keeps its relative accuracy near r2 = 1 as log1p of the exact r2 - 1
+            const double lg = r2 >= 0.5 ? log1p(r2 - 1.0) : log(r2);
+            if (acc) f = sqrt(-2.0 * lg / r2);
         }
+        const uint64_t mask = __ballot(acc);
+        if (lane == 0) s_cnt[wv] = __popcll(mask);
         __syncthreads();
+        int64_t before = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < kWnT / 64; ++w) {
+            before += w < wv ? s_cnt[w] : 0;
+            all += s_cnt[w];
+        }
+        if (acc) {
+            const int64_t rank = before + __popcll(mask & ((1ull << lane) - 1ull));
+            const int64_t pos = produced + 2 * rank;
+            if (pos < N) out[pos * M + ch] = rms * (f * x2);
+            if (pos + 1 < N) out[(pos + 1) * M + ch] = rms * (f * x1);
+        }
+        produced += 2 * all;
         const int rest = total - 4 * ncand;
-        uint32_t keep = lane < rest ? words[4 * ncand + lane] : 0u;
+        const uint32_t keep = tid < rest ? words[4 * ncand + tid] : 0u;
         __syncthreads();
-        if (lane < rest) words[lane] = keep;
+        if (tid < rest) words[tid] = keep;
         left = rest;
         __syncthreads();
     }
@@ -236,7 +245,7 @@ int abf_white_noise(const uint32_t *seeds, int32_t num_channels, int64_t num_sam
     if (num_channels < 0 || num_samples < 0) return set_error(ABF_ERR_INVALID, "bad sizes");
     if (num_channels == 0 || num_samples == 0) return ABF_OK;
     if (!seeds || !out) return set_error(ABF_ERR_INVALID, "null pointer");
-    hipLaunchKernelGGL(wnoise_kernel, dim3((unsigned)num_channels), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
+    hipLaunchKernelGGL(wnoise_kernel, dim3((unsigned)num_channels), dim3(kWnT), 0, reinterpret_cast<hipStream_t>(stream),
                        seeds, num_channels, num_samples, rms, out);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(ABF_ERR_HIP, hipGetErrorString(e));
