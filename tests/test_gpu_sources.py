@@ -79,6 +79,16 @@ def test_point_source_doctest_shapes(gpu):
     assert [b.shape for b in ps.result(num=4)] == [(4, 9), (2, 9)]
 
 
+@pytest.mark.parametrize(('up', 'start_t'), [(3, 0.0123), (1, 0.5), (12, 0.0)])
+def test_point_source_other_upsampling_factors(gpu, up, start_t):
+    """Upsampling factors other than a power of two: the index still accumulates sample by sample
+    as the reference's, bit for bit (the kernel jumps over runs of exact additions)."""
+    sig = aa.SineGenerator(sample_freq=51200.0, num_samples=40_000, freq=777.0)
+    ps = aa.PointSource(signal=sig, loc=(0.3, -0.1, 1.5), mics=_mics(6, 2), start_t=start_t, up=up)
+    got = ps.result_device(4096).cpu().numpy()
+    np.testing.assert_array_equal(got, _oracle_ps(ps, 4096))
+
+
 def test_point_source_index_crosses_powers_of_two(gpu):
     """300,000 samples at up = 16: the index passes 2^22 and 2^23, where adding 16 sample by
     sample rounds; the kernel reproduces the sequential accumulation."""
