@@ -32,7 +32,7 @@ from .fbeamform import (
     integrate,
     psf_device,
 )
-from .fprocess import RFFT, AutoPowerSpectra, CrossPowerSpectra
+from .fprocess import IRFFT, RFFT, AutoPowerSpectra, CrossPowerSpectra
 from .grids import Grid, ImportGrid, RectGrid, RectGrid3D
 from .microphones import MicGeom
 from .signals import NoiseGenerator, SignalGenerator, SineGenerator, WNoiseGenerator
@@ -41,6 +41,7 @@ from .spectra import BaseSpectra, PowerSpectra, PowerSpectraImport
 
 __all__ = [
     'RFFT',
+    'IRFFT',
     'AutoPowerSpectra',
     'CrossPowerSpectra',
     'BaseSpectra',

@@ -106,6 +106,9 @@ _SIGS = {
     'abf_rfft_workspace_size': (ctypes.c_size_t, [ctypes.c_int32]),
     'abf_rfft_blocks': (ctypes.c_int, [_c_dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32, _c_dp, _c_dp, _c_dp, _c_dp, ctypes.c_size_t, _c_dp]),
+    'abf_irfft_workspace_size': (ctypes.c_size_t, [ctypes.c_int32]),
+    'abf_irfft_blocks': (ctypes.c_int, [_c_dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _c_dp, _c_dp,
+                                        ctypes.c_size_t, _c_dp]),
     'abf_auto_power': (ctypes.c_int, [_c_dp, ctypes.c_int64, ctypes.c_double, _c_dp, _c_dp]),
     'abf_cross_power': (ctypes.c_int, [_c_dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                        _c_dp, _c_dp]),

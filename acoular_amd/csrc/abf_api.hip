@@ -966,6 +966,38 @@ int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, in
     return hip_status("block FFT launch");
 }
 
+size_t abf_irfft_workspace_size(int32_t block_size) { return abf_rfft_workspace_size(block_size); }
+
+int abf_irfft_blocks(const void *spec, int32_t num_blocks, int32_t num_mics, int32_t block_size, float *out,
+                     void *workspace, size_t workspace_bytes, void *stream)
+{
+    const int bs = block_size;
+    if (num_mics < 1 || bs < 2 || (bs & (bs - 1)) || num_blocks < 0)
+        return fail(ABF_ERR_INVALID, "bad inverse FFT parameters (block_size must be a power of two >= 2)");
+    if (bs > 16384) return fail(ABF_ERR_UNSUPPORTED, "block_size above 16384 is not supported by the LDS FFT");
+    if (num_blocks == 0) return ABF_OK;
+    if (!spec || !out) return fail(ABF_ERR_INVALID, "null pointer");
+    if (!workspace || workspace_bytes < abf_irfft_workspace_size(bs)) return fail(ABF_ERR_WORKSPACE, "workspace too small");
+    const hipStream_t s = S(stream);
+    const int M = num_mics;
+    int logbs = 0;
+    while ((1 << logbs) < bs) ++logbs;
+    float2 *tw = reinterpret_cast<float2 *>(workspace);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)irfft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
+        attr = true;
+    }
+    const int P = std::max(1, std::min({16, 8192 / bs, (M + 1) / 2}));
+    for (int b0 = 0; b0 < num_blocks; b0 += 65535) {
+        const int nb = std::min(65535, num_blocks - b0);
+        hipLaunchKernelGGL(irfft_kernel, dim3((unsigned)cdiv(M, 2 * P), nb), dim3(512), (size_t)P * bs * sizeof(float2),
+                           s, reinterpret_cast<const float2 *>(spec), M, bs, logbs, b0, P, tw, out);
+    }
+    return hip_status("inverse block FFT launch");
+}
+
 int abf_auto_power(const void *spec, int64_t n, double scale, float *out, void *stream)
 {
     if (n < 0) return fail(ABF_ERR_INVALID, "negative size");

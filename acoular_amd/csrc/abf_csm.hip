@@ -55,29 +55,14 @@ __global__ void calc_csm_c128_kernel(double2 *__restrict__ csm, const double2 *_
 
 __device__ __forceinline__ int bitrev(int n, int bits) { return (int)(__builtin_bitreverse32((uint32_t)n) >> (32 - bits)); }
 
-// grid: (ceil(M / 2P), nb); P channel pairs of one Welch block per workgroup
-__global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
+// radix-2 decimation-in-time stages over P sequences of bs points in LDS (bit-reversed order on
+// entry, natural order on exit), up to three stages per pass through LDS: a thread takes the 2^g
+// elements base + j half (j < 2^g) that stages s .. s+g-1 combine only among themselves and runs
+// those stages in registers.  Ends with a barrier.
+__device__ __forceinline__ void fft_stages(float2 *z, int P, int bs, int logbs, const float2 *twiddle)
 {
-    extern __shared__ float2 z[]; // [P][bs]
-    const int bs = a.bs, P = a.P;
-    const int bl = blockIdx.y;
-    const int m0 = blockIdx.x * 2 * P;
-    const int64_t t0 = (int64_t)(a.b0 + bl) * a.hop;
-    for (int idx = threadIdx.x; idx < P * bs; idx += blockDim.x) {
-        const int p = idx % P, n = idx / P;
-        const int ma = m0 + 2 * p, mb = ma + 1;
-        const float w = a.window[n];
-        const float *row = a.time + (t0 + n) * a.M;
-        const float xa = ma < a.M ? row[ma] : 0.f;
-        const float xb = mb < a.M ? row[mb] : 0.f;
-        z[p * bs + bitrev(n, a.logbs)] = make_float2(w * xa, w * xb);
-    }
-    __syncthreads();
-    // radix-2 decimation-in-time stages, up to three per pass through LDS: a thread takes the
-    // 2^g elements base + j half (j < 2^g) that stages s .. s+g-1 combine only among themselves
-    // and runs those stages in registers
-    for (int s = 1; s <= a.logbs;) {
-        const int g = min(3, a.logbs - s + 1);
+    for (int s = 1; s <= logbs;) {
+        const int g = min(3, logbs - s + 1);
         const int half = 1 << (s - 1);
         const int per_chan = bs >> g; // groups per channel
         for (int idx = threadIdx.x; idx < P * per_chan; idx += blockDim.x) {
@@ -96,7 +81,7 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
                 for (int j = 0; j < 8; ++j) {
                     if (j >= (1 << g) || (j & (1 << t))) continue;
                     const int jj = j | (1 << t);
-                    const float2 w = a.twiddle[(pos + (j & ((1 << t) - 1)) * half) * tstride];
+                    const float2 w = twiddle[(pos + (j & ((1 << t) - 1)) * half) * tstride];
                     const float2 x1 = v[jj];
                     const float tr = x1.x * w.x - x1.y * w.y;
                     const float ti = x1.x * w.y + x1.y * w.x;
@@ -111,6 +96,27 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
         __syncthreads();
         s += g;
     }
+}
+
+// grid: (ceil(M / 2P), nb); P channel pairs of one Welch block per workgroup
+__global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
+{
+    extern __shared__ float2 z[]; // [P][bs]
+    const int bs = a.bs, P = a.P;
+    const int bl = blockIdx.y;
+    const int m0 = blockIdx.x * 2 * P;
+    const int64_t t0 = (int64_t)(a.b0 + bl) * a.hop;
+    for (int idx = threadIdx.x; idx < P * bs; idx += blockDim.x) {
+        const int p = idx % P, n = idx / P;
+        const int ma = m0 + 2 * p, mb = ma + 1;
+        const float w = a.window[n];
+        const float *row = a.time + (t0 + n) * a.M;
+        const float xa = ma < a.M ? row[ma] : 0.f;
+        const float xb = mb < a.M ? row[mb] : 0.f;
+        z[p * bs + bitrev(n, a.logbs)] = make_float2(w * xa, w * xb);
+    }
+    __syncthreads();
+    fft_stages(z, P, bs, a.logbs, a.twiddle);
     const int F = bs / 2 + 1;
     for (int idx = threadIdx.x; idx < P * F; idx += blockDim.x) {
         const int p = idx % P, k = idx / P;
@@ -215,6 +221,49 @@ __global__ void welch_finalize_kernel(float2 *__restrict__ csm, int M, float sca
     if (r == c) v.y = 0.f;
     C[(int64_t)r * M + c] = v;
     if (r != c) C[(int64_t)c * M + r] = make_float2(v.x, -v.y);
+}
+
+// IRFFT.result (fprocess.py:227-283): the real inverse transform of every snapshot spectrum
+// [nb][F][M], two channels per complex sequence: Z = Xa + i Xb over all bs bins (the upper half
+// from Hermitian symmetry, the imaginary parts of the DC and Nyquist bins dropped as a real
+// inverse transform drops them), z = conj(FFT(conj Z)) / bs = xa + i xb.  Output time-major
+// [nb * bs][M].  grid: (ceil(M / 2P), blocks of this launch)
+__global__ __launch_bounds__(512) void irfft_kernel(const float2 *__restrict__ spec, int M, int bs, int logbs, int b0,
+                                                    int P, const float2 *__restrict__ twiddle, float *__restrict__ out)
+{
+    extern __shared__ float2 z[]; // [P][bs]
+    const int F = bs / 2 + 1;
+    const int64_t bl = (int64_t)b0 + blockIdx.y;
+    const int m0 = blockIdx.x * 2 * P;
+    for (int idx = threadIdx.x; idx < P * bs; idx += blockDim.x) {
+        const int p = idx % P, k = idx / P;
+        const int ma = m0 + 2 * p, mb = ma + 1;
+        const int kk = k < F ? k : bs - k;
+        const float2 *row = spec + (bl * F + kk) * M;
+        float2 xa = ma < M ? row[ma] : make_float2(0.f, 0.f);
+        float2 xb = mb < M ? row[mb] : make_float2(0.f, 0.f);
+        if (kk == 0 || kk == F - 1) {
+            xa.y = 0.f;
+            xb.y = 0.f;
+        }
+        if (k >= F) { // bin bs - k mirrored: conjugate
+            xa.y = -xa.y;
+            xb.y = -xb.y;
+        }
+        // conj(Z) with Z = xa + i xb
+        z[p * bs + bitrev(k, logbs)] = make_float2(xa.x - xb.y, -(xa.y + xb.x));
+    }
+    __syncthreads();
+    fft_stages(z, P, bs, logbs, twiddle);
+    const float inv = 1.f / (float)bs; // a power of two: exact
+    for (int idx = threadIdx.x; idx < P * bs; idx += blockDim.x) {
+        const int p = idx % P, n = idx / P;
+        const int ma = m0 + 2 * p, mb = ma + 1;
+        const float2 v = z[p * bs + n];
+        float *dst = out + (bl * bs + n) * M;
+        if (ma < M) dst[ma] = v.x * inv;
+        if (mb < M) dst[mb] = -v.y * inv;
+    }
 }
 
 // AutoPowerSpectra.result (fprocess.py:359-366): |X|^2 * scale per value

@@ -20,6 +20,8 @@ __global__ void calc_csm_c128_kernel(double2 *csm, const double2 *spec, int F, i
 __global__ void welch_fft_kernel(WelchFftArgs a);
 template <int TR> __global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nTc, float2 *csm);
 __global__ void welch_finalize_kernel(float2 *csm, int M, float scale);
+__global__ void irfft_kernel(const float2 *spec, int M, int bs, int logbs, int b0, int P, const float2 *twiddle,
+                             float *out);
 __global__ void auto_power_kernel(const float2 *spec, int64_t n, float scale, float *out);
 __global__ void cross_power_kernel(const float2 *spec, int64_t nbf, int M, float scale, float2 *out);
 

@@ -3,7 +3,8 @@
 ``RFFT`` transforms every block of the Welch block generator (``spectra.py:171-183``) with the
 LDS FFT of the Welch CSM (``abf_rfft_blocks``: window, one-sided FFT, the scaling of
 ``RFFT._scale``), ``AutoPowerSpectra`` and ``CrossPowerSpectra`` form |X|^2 and X X^H per
-snapshot (``abf_auto_power``, ``abf_cross_power``).  Each stage keeps its whole result resident
+snapshot (``abf_auto_power``, ``abf_cross_power``), and ``IRFFT`` turns every snapshot spectrum
+back into a block of samples (``abf_irfft_blocks``).  Each stage keeps its whole result resident
 on the device (``result_device``); ``result(num)`` yields it as numpy rows of ``precision`` in the
 chunks the reference's generators yield.  The FFT runs in float32 on the device (the reference
 transforms in float64 and then casts to ``precision``).
@@ -282,3 +283,87 @@ class CrossPowerSpectra(AutoPowerSpectra):
     def result_device(self):
         """X X^H of every snapshot and bin, complex64 device tensor ``[blocks, num_freqs, M, M]``."""
         return self._cached('cross', self._calc)
+
+
+class IRFFT(HasParams):
+    """Real inverse FFT of one-sided spectrum snapshots back to time blocks (``fprocess.py:182-283``)."""
+
+    _params = ('source', 'precision', 'workers')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'source', None)
+        object.__setattr__(self, 'precision', 'float64')
+        object.__setattr__(self, 'workers', None)
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name == 'precision' and value not in ('float64', 'float32'):
+            raise ValueError("precision must be 'float64' or 'float32'")
+        super().__setattr__(name, value)
+
+    def _digest_items(self):
+        # workers only sets the host FFT's threads in the reference and is not part of its digest
+        return (self.source, self.precision)
+
+    @property
+    def sample_freq(self):
+        return self.source.sample_freq
+
+    @property
+    def num_channels(self):
+        return self.source.num_channels
+
+    @property
+    def num_samples(self):
+        """``IRFFT._get_num_samples`` (``fprocess.py:206-209``): snapshots times block size."""
+        if self.source.num_samples >= 0:
+            return int(self.source.num_samples * self.source.block_size)
+        return -1
+
+    def _validate(self):
+        """``IRFFT._validate`` (``fprocess.py:215-233``)."""
+        bs = self.source.block_size
+        if not bs or bs < 0:
+            msg = (f'Source of class {type(self).__name__} has an unknown blocksize: {bs}.'
+                   'This is likely due to incomplete spectral data from which the inverse FFT cannot be calculated.')
+            raise ValueError(msg)
+        if (self.source.num_freqs - 1) * 2 != bs:
+            msg = (f'Block size must be 2*(num_freqs-1) but is {bs}.'
+                   'This is likely due to incomplete spectral data from which the inverse FFT cannot be calculated.')
+            raise ValueError(msg)
+        if bs % 2 != 0:
+            msg = f'Block size must be even but is {bs}.'
+            raise ValueError(msg)
+
+    def _calc(self):
+        torch = _lib.torch()
+        self._validate()
+        if not hasattr(self.source, 'result_device'):
+            raise TypeError('the source must be an RFFT (device spectra)')
+        spec = self.source.result_device().to(torch.complex64).contiguous()
+        nb, nf, nc = (int(x) for x in spec.shape)
+        bs = self.source.block_size
+        if bs & (bs - 1) or bs > 16384:
+            raise NotImplementedError('the device FFT takes power-of-two block sizes up to 16384')
+        out = torch.empty((nb * bs, nc), dtype=torch.float32, device=spec.device)
+        if nb == 0:
+            return out
+        lib = _lib.lib()
+        ws = _lib.workspace(lib.abf_irfft_workspace_size(bs))
+        _lib.check(lib.abf_irfft_blocks(_lib.ptr(spec), nb, nc, bs, _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                                        _lib.stream_handle()), 'IRFFT')
+        return out
+
+    def result_device(self):
+        """The samples of every snapshot, float32 device tensor ``[blocks * block_size, channels]``."""
+        return self._cached('time', self._calc)
+
+    def result(self, num):
+        """Blocks of ``num`` samples ``(num, num_channels)`` of ``precision``, the last one shorter
+        (``fprocess.py:235-283``: one block per snapshot when ``num`` equals the block size,
+        re-blocked through a buffer otherwise -- the same sample sequence either way)."""
+        if num < 1:
+            raise ValueError('num must be >= 1')
+        host = self.result_device().cpu().numpy().astype(self.precision)
+        for b in range(0, host.shape[0], num):
+            yield host[b : b + num]

@@ -288,6 +288,14 @@ int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, in
                     int32_t num_blocks, const float *window, const float *bin_scale, void *out, void *workspace,
                     size_t workspace_bytes, void *stream);
 
+/* IRFFT.result (fprocess.py:182-283): the real inverse FFT of every snapshot spectrum
+ * spec [num_blocks][bs/2+1][num_mics] complex64 (the imaginary parts of the DC and Nyquist bins
+ * ignored, as a real inverse transform ignores them), out [num_blocks * bs][num_mics] float32,
+ * time-major; block_size a power of two <= 16384. */
+size_t abf_irfft_workspace_size(int32_t block_size);
+int abf_irfft_blocks(const void *spec, int32_t num_blocks, int32_t num_mics, int32_t block_size, float *out,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
 /* AutoPowerSpectra.result (fprocess.py:286-366): out[i] = |spec[i]|^2 * scale, n complex64 values
  * to float32. */
 int abf_auto_power(const void *spec, int64_t n, double scale, float *out, void *stream);

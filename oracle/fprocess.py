@@ -11,6 +11,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
   no block fits and ``num > 1`` (``fprocess.py:168-180``).
 * ``auto_power``: ``AutoPowerSpectra`` (``fprocess.py:320-366``), ``|X|^2`` times
   1 / bs^2, doubled when single-sided, times bs * fs for 'psd'.
+* ``irfft_blocks``: ``IRFFT.result`` (``fprocess.py:235-283``): ``scipy.fft.irfft`` of every
+  snapshot along the frequency axis, the blocks concatenated in time.
 * ``cross_power``: ``CrossPowerSpectra.result`` with ``calc_mode='full'``
   (``fprocess.py:430-470``): per row the upper triangle of ``calcCSM`` plus its conjugate
   transpose without the diagonal, times the auto-power scale.
@@ -86,3 +88,11 @@ def cross_power(rows, num_freqs, num_channels, scale, precision='complex128'):
         lower[:, idx, idx] = 0
         out[i] = (lower + upper).reshape(-1)
     return out * scale
+
+
+def irfft_blocks(rows, block_size, num_channels):
+    """``IRFFT.result`` samples of spectrum rows ``[n, F * M]``: ``[n * block_size, M]``."""
+    rows = np.asarray(rows)
+    nf = block_size // 2 + 1
+    out = [fft.irfft(r.reshape(nf, num_channels), n=block_size, axis=0) for r in rows]
+    return np.concatenate(out, axis=0) if out else np.zeros((0, num_channels))

@@ -61,3 +61,25 @@ def test_power_spectra_parameters():
         cp.precision = 'float64'
     with pytest.raises(ValueError):
         cp.calc_mode = 'diag'
+
+
+def test_irfft_parameters_and_validation():
+    fft = aa.RFFT(source=_ts(64), block_size=16)
+    ifft = aa.IRFFT(source=fft)
+    assert ifft.precision == 'float64' and ifft.num_samples == 64 and ifft.num_channels == 1
+    with pytest.raises(ValueError):
+        ifft.precision = 'complex64'
+    d = ifft.digest
+    ifft.workers = 2
+    assert ifft.digest == d
+    ifft.precision = 'float32'
+    assert ifft.digest != d
+
+    class _Bad:
+        block_size = 16
+        num_freqs = 8
+        num_samples = 1
+        num_channels = 1
+
+    with pytest.raises(ValueError):
+        aa.IRFFT(source=_Bad())._validate()

@@ -95,3 +95,35 @@ def test_cross_power_triangle_modes(gpu):
     many = aa.RFFT(source=_ts(64, 3), block_size=16)
     with pytest.raises(ValueError):
         next(aa.CrossPowerSpectra(source=many, calc_mode='upper').result(2))
+
+
+@pytest.mark.parametrize('num', [1, 4, 7, 8, 9, 16])
+def test_irfft_roundtrip_and_blocks(gpu, num):
+    """The reference's IRFFT tests: the inverse of the rectangular, unscaled RFFT gives back the
+    time data, in blocks of any num (test_equal_fft_ifft, test_irfft_buffer); float32 FFTs both
+    ways, so to 2e-6 of the largest sample."""
+    ts = _ts(64, 3, seed=3)
+    ifft = aa.IRFFT(source=aa.RFFT(source=ts, block_size=8))
+    assert ifft.num_samples == 64
+    blocks = list(ifft.result(num))
+    assert all(b.shape[0] == num for b in blocks[:-1]) and blocks[-1].shape[0] <= num
+    got = np.concatenate(blocks, axis=0)
+    assert got.dtype == np.float64 and got.shape == ts.data.shape
+    assert np.abs(got - ts.data).max() <= 2e-6 * np.abs(ts.data).max()
+
+
+def test_irfft_matches_scipy(gpu):
+    """Spectra that are not the transform of real blocks (nonzero imaginary parts at DC and
+    Nyquist, which a real inverse transform ignores) against scipy.fft.irfft."""
+    bs, nch = 256, 5
+    ts = _ts(bs * 6, nch, seed=9)
+    fft = aa.RFFT(source=ts, block_size=bs, window='Hanning', scaling='amplitude')
+    spec = fft.result_device()
+    spec[:, 0, :] += 0.3j
+    spec[:, -1, :] -= 0.2j
+    rows = spec.cpu().numpy().reshape(spec.shape[0], -1)
+    ifft = aa.IRFFT(source=fft, precision='float32')
+    got = ifft.result_device().cpu().numpy()
+    want = P.irfft_blocks(rows.astype(np.complex128), bs, nch)
+    assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max()
+    assert next(ifft.result(bs)).dtype == np.float32

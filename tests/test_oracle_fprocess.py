@@ -61,3 +61,11 @@ def test_cross_power_structure():
     # against calcCSM's upper triangle on one snapshot
     upper = S.calc_csm_update(np.zeros((nf, nc, nc), complex), xs[3])
     np.testing.assert_allclose(np.triu(cp[3]), upper * scale, rtol=1e-12, atol=1e-20)
+
+
+def test_irfft_inverts_rfft():
+    """The reference's test_equal_fft_ifft: with a rectangular window and no scaling the inverse
+    transform of the spectra gives back the samples of every complete block."""
+    x = np.random.default_rng(8).standard_normal((64, 3))
+    rows = P.rfft_blocks(x, 8, 'Rectangular')
+    np.testing.assert_allclose(P.irfft_blocks(rows, 8, 3), x, atol=1e-14)
