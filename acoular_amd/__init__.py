@@ -33,13 +33,38 @@ from .fbeamform import (
     psf_device,
 )
 from .fprocess import IRFFT, RFFT, AutoPowerSpectra, CrossPowerSpectra
-from .grids import Grid, ImportGrid, RectGrid, RectGrid3D
+from .grids import (
+    CircSector,
+    ConvexSector,
+    Grid,
+    ImportGrid,
+    LineGrid,
+    MergeGrid,
+    MultiSector,
+    PolySector,
+    RectGrid,
+    RectGrid3D,
+    RectSector,
+    RectSector3D,
+    Sector,
+    SingleSector,
+)
 from .microphones import MicGeom
 from .signals import NoiseGenerator, SignalGenerator, SineGenerator, WNoiseGenerator
 from .sources import PointSource, SourceMixer, TimeSamples, UncorrelatedNoiseSource
 from .spectra import BaseSpectra, PowerSpectra, PowerSpectraImport
 
 __all__ = [
+    'CircSector',
+    'LineGrid',
+    'MergeGrid',
+    'ConvexSector',
+    'MultiSector',
+    'PolySector',
+    'RectSector',
+    'RectSector3D',
+    'Sector',
+    'SingleSector',
     'RFFT',
     'IRFFT',
     'AutoPowerSpectra',

@@ -265,12 +265,9 @@ class BeamformerBase(HasParams):
         return h.reshape(self.steer.grid.shape)
 
     def integrate(self, sector, frange=None, num=0):
-        """Sector-integrated spectrum (``fbeamform.py:595-685``)."""
-        if hasattr(self.steer.grid, 'indices'):
-            ind = self.steer.grid.indices(*sector)
-        else:
-            msg = f'Grid of type {self.steer.grid.__class__.__name__} does not have an indices method!'
-            raise NotImplementedError(msg)
+        """Sector-integrated spectrum (``fbeamform.py:595-685``): ``sector`` is a ``Sector`` or the
+        arguments of the grid's ``indices``."""
+        ind = _sector_indices(self.steer.grid, sector)
         gshape = self.steer.grid.shape
         num_freqs = self.freq_data.fftfreq().shape[0]
         if num == 0 or frange is None:
@@ -1003,9 +1000,25 @@ def L_p(x):  # noqa: N802
     return 10 * np.log10(np.maximum(x / 4e-10, np.finfo(x.dtype).eps))
 
 
+def _sector_indices(grid, sector):
+    """``grid.subdomain(sector)`` for a ``Sector``, else ``grid.indices(*sector)``
+    (``fbeamform.py:640-650, 2478-2490``)."""
+    from .grids import Sector
+
+    if isinstance(sector, Sector):
+        return grid.subdomain(sector)
+    if hasattr(grid, 'indices'):
+        return grid.indices(*sector)
+    msg = (f'Grid of type {grid.__class__.__name__} does not have an indices method! '
+           f'Please use a sector derived instance of type :class:`~acoular.grids.Sector` '
+           'instead of type numpy.array.')
+    raise NotImplementedError(msg)
+
+
 def integrate(data, grid, sector):
-    """Integrate a map over a sector (``fbeamform.py:2435-2497``), rectangular/circular sectors."""
-    ind = grid.indices(*sector)
+    """Integrate a map over a sector (``fbeamform.py:2435-2497``): a ``Sector`` or the arguments
+    of the grid's ``indices``."""
+    ind = _sector_indices(grid, sector)
     gshape = grid.shape
     gsize = grid.size
     if np.size(data) == gsize:  # one value per grid point

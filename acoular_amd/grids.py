@@ -1,7 +1,8 @@
-"""Grid geometries of the beamforming map (``acoular/grids.py``).
+"""Grid geometries of the beamforming map and the sectors integrated over (``acoular/grids.py``).
 
-Only what the frequency-domain path needs: positions, shape and the rectangular
-``index``/``indices`` helpers used by ``BeamformerBase.integrate``.
+Positions, shape, the rectangular ``index``/``indices`` helpers and the ``Sector`` classes whose
+``contains`` masks select the grid points ``BeamformerBase.integrate`` and ``integrate`` sum
+over (``Grid.subdomain``).  Sector membership is host geometry, as in the reference.
 """
 
 from __future__ import annotations
@@ -32,6 +33,10 @@ class Grid(HasParams):
 
     def _calc_pos(self):  # pragma: no cover - abstract
         raise NotImplementedError
+
+    def subdomain(self, sector):
+        """Indices of the grid points inside ``sector`` in the grid's shape (``grids.py:187-212``)."""
+        return np.where(sector.contains(self.pos).reshape(self.shape))
 
 
 class RectGrid(Grid):
@@ -192,3 +197,288 @@ class ImportGrid(Grid):
     @property
     def shape(self):
         return (self.pos.shape[-1],)
+
+
+class LineGrid(Grid):
+    """``num_points`` equally spaced points over ``length`` from ``loc`` along ``direction``
+    (``grids.py:834-915``)."""
+
+    _params = ('loc', 'direction', 'length', 'num_points')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'loc', (0.0, 0.0, 0.0))
+        object.__setattr__(self, 'direction', (1.0, 0.0, 0.0))
+        object.__setattr__(self, 'length', 1.0)
+        object.__setattr__(self, 'num_points', 1)
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name in ('loc', 'direction'):
+            value = tuple(float(v) for v in value)
+        if name == 'length':
+            value = float(value)
+        if name == 'num_points':
+            value = int(value)
+        object.__setattr__(self, name, value)
+
+    @property
+    def shape(self):
+        return self.size
+
+    def _calc_pos(self):
+        loc = np.array(self.loc, dtype=float).reshape((3, 1))
+        if self.num_points <= 1:
+            return loc
+        dist = self.length / (self.num_points - 1)
+        direc = np.array(self.direction, dtype=float)
+        direc = direc / np.linalg.norm(direc)
+        steps = np.arange(self.num_points, dtype=float)
+        return loc + (direc * dist)[:, None] * steps[None, :]
+
+
+class MergeGrid(Grid):
+    """The distinct points of several grids, in ``numpy.unique`` column order (``grids.py:917-989``)."""
+
+    _params = ('grids',)
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'grids', [])
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name == 'grids':
+            value = list(value)
+        object.__setattr__(self, name, value)
+
+    @property
+    def shape(self):
+        return self.size
+
+    def _digest_items(self):
+        # each grid by its own digest, so a change inside a grid changes the merged positions
+        return tuple(self.grids)
+
+    @property
+    def subgrids(self) -> np.ndarray:
+        """Class name and digest of the grid each point came from, before duplicates are merged
+        (``grids.py:969-974``), shape ``(1, total points)``."""
+        names = [np.full(g.size, g.__class__.__name__ + g.digest) for g in self.grids]
+        return (np.concatenate(names) if names else np.zeros(0, dtype=str))[None, :]
+
+    def _calc_pos(self):
+        bpos = np.zeros((3, 0))
+        for g in self.grids:
+            bpos = np.append(bpos, g.pos, axis=1)
+        return np.unique(bpos, axis=1)
+
+
+# ----------------------------------------------------------------------------------------------
+# sectors (grids.py:991-1494)
+# ----------------------------------------------------------------------------------------------
+
+
+def polygon_signed_distance(vx, vy, px, py, small=1e-12) -> np.ndarray:
+    """Distance of the points (px, py) to the boundary of the polygon with vertices (vx, vy),
+    positive inside, negative outside, 0 within ``small`` of the boundary -- the convention of
+    the reference's ``Polygon.is_inside`` (``tools/utils.py:25-180``).  Inside/outside by the
+    even-odd crossing rule, the distance as the minimum over the edges."""
+    vx = np.asarray(vx, dtype=float)
+    vy = np.asarray(vy, dtype=float)
+    px = np.atleast_1d(np.asarray(px, dtype=float))
+    py = np.atleast_1d(np.asarray(py, dtype=float))
+    if vx.shape != vy.shape:
+        raise IndexError('x and y must be equally sized.')
+    if px.shape != py.shape:
+        raise IndexError('x and y have different shapes')
+    ax, ay = vx, vy
+    bx, by = np.roll(vx, -1), np.roll(vy, -1)
+    ex, ey = bx - ax, by - ay
+    d2 = np.full(px.shape, np.inf)
+    inside = np.zeros(px.shape, dtype=bool)
+    for i in range(vx.size):
+        if ex[i] == 0.0 and ey[i] == 0.0:  # repeated vertex (a closed vertex list)
+            t = np.zeros_like(px)
+        else:
+            t = np.clip(((px - ax[i]) * ex[i] + (py - ay[i]) * ey[i]) / (ex[i] ** 2 + ey[i] ** 2), 0.0, 1.0)
+        d2 = np.minimum(d2, (ax[i] + t * ex[i] - px) ** 2 + (ay[i] + t * ey[i] - py) ** 2)
+        # even-odd rule: edges that straddle the horizontal line through the point, crossed to its right
+        straddle = (ay[i] > py) != (by[i] > py)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            xcross = ax[i] + (py - ay[i]) * ex[i] / np.where(ey[i] == 0.0, 1.0, ey[i])
+        inside ^= straddle & (px < xcross)
+    d = np.sqrt(d2)
+    out = np.where(inside, d, -d)
+    out[np.abs(out) < small] = 0.0
+    return out
+
+
+def in_hull(p, hull, border=True, tol=0):
+    """Points of ``p`` inside the convex hull of ``hull`` (``grids.py:84-128``, a Delaunay
+    triangulation of the hull points)."""
+    from scipy.spatial import Delaunay
+
+    if not isinstance(hull, Delaunay):
+        hull = Delaunay(hull)
+    if border:
+        return hull.find_simplex(p, tol=tol) >= 0
+    return hull.find_simplex(p, tol=tol) > 0
+
+
+class Sector(HasParams):
+    """A sector that contains every grid point (``grids.py:991-1043``)."""
+
+    def contains(self, pos) -> np.ndarray:
+        return np.ones(np.asarray(pos).shape[1], dtype=bool)
+
+
+class SingleSector(Sector):
+    """Base of the single sectors (``grids.py:1045-1072``): border inclusion, tolerance and the
+    fallback to the nearest point when no point is inside."""
+
+    _params = ('include_border', 'abs_tol', 'default_nearest')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'include_border', True)
+        object.__setattr__(self, 'abs_tol', 1e-12)
+        object.__setattr__(self, 'default_nearest', True)
+        super().__init__(**kwargs)
+
+
+class RectSector(SingleSector):
+    """Rectangle in the x-y plane (``grids.py:1074-1158``)."""
+
+    _params = SingleSector._params + ('x_min', 'x_max', 'y_min', 'y_max')
+
+    def __init__(self, **kwargs):
+        for k in ('x_min', 'y_min'):
+            object.__setattr__(self, k, -1.0)
+        for k in ('x_max', 'y_max'):
+            object.__setattr__(self, k, 1.0)
+        super().__init__(**kwargs)
+
+    def _box(self):
+        return (min(self.x_min, self.x_max), max(self.x_min, self.x_max), min(self.y_min, self.y_max),
+                max(self.y_min, self.y_max))
+
+    def _inside(self, pos, lo, hi):
+        t = self.abs_tol if self.include_border else -self.abs_tol
+        inds = np.ones(pos.shape[1], dtype=bool)
+        for ax, (a, b) in enumerate(zip(lo, hi)):
+            inds &= (pos[ax, :] - a > -t) & (pos[ax, :] - b < t)
+        return inds
+
+    def contains(self, pos):
+        pos = np.asarray(pos)
+        xmin, xmax, ymin, ymax = self._box()
+        inds = self._inside(pos, (xmin, ymin), (xmax, ymax))
+        return self._nearest(pos, inds, xmin, xmax, ymin, ymax)
+
+    def _nearest(self, pos, inds, xmin, xmax, ymin, ymax):
+        if not inds.any() and self.default_nearest:
+            x, y = (xmin + xmax) / 2.0, (ymin + ymax) / 2.0
+            inds[np.argmin((pos[0, :] - x) ** 2 + (pos[1, :] - y) ** 2)] = True
+        return inds
+
+
+class RectSector3D(RectSector):
+    """Box (``grids.py:1160-1246``); the nearest-point fallback measures in x and y only."""
+
+    _params = RectSector._params + ('z_min', 'z_max')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'z_min', -1.0)
+        object.__setattr__(self, 'z_max', 1.0)
+        super().__init__(**kwargs)
+
+    def contains(self, pos):
+        pos = np.asarray(pos)
+        xmin, xmax, ymin, ymax = self._box()
+        zmin, zmax = min(self.z_min, self.z_max), max(self.z_min, self.z_max)
+        inds = self._inside(pos, (xmin, ymin, zmin), (xmax, ymax, zmax))
+        return self._nearest(pos, inds, xmin, xmax, ymin, ymax)
+
+
+class CircSector(SingleSector):
+    """Circle in the x-y plane (``grids.py:1248-1313``)."""
+
+    _params = SingleSector._params + ('x', 'y', 'r')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'x', 0.0)
+        object.__setattr__(self, 'y', 0.0)
+        object.__setattr__(self, 'r', 1.0)
+        super().__init__(**kwargs)
+
+    def contains(self, pos):
+        pos = np.asarray(pos)
+        dr2 = (pos[0, :] - self.x) ** 2 + (pos[1, :] - self.y) ** 2
+        t = self.abs_tol if self.include_border else -self.abs_tol
+        inds = dr2 - self.r**2 < t
+        if not inds.any() and self.default_nearest:
+            inds[np.argmin(dr2)] = True
+        return inds
+
+
+def _polygon_fallback(inds, edges):
+    # the reference takes argmin over the two coordinates of the vertices' mean and marks that
+    # grid index, 0 or 1 (grids.py:1372-1375, 1430-1433); kept as it is
+    dr2 = np.array(edges, dtype=float).reshape(-1, 2).mean(0)
+    inds[np.argmin(dr2)] = True
+    return inds
+
+
+class PolySector(SingleSector):
+    """Polygon in the x-y plane, vertices ``edges = [x0, y0, x1, y1, ...]`` (``grids.py:1316-1377``)."""
+
+    _params = SingleSector._params + ('edges',)
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'edges', [])
+        super().__init__(**kwargs)
+
+    def contains(self, pos):
+        pos = np.asarray(pos)
+        v = np.array(self.edges, dtype=float).reshape(-1, 2)
+        dists = polygon_signed_distance(v[:, 0], v[:, 1], pos[0, :], pos[1, :])
+        inds = dists >= -self.abs_tol if self.include_border else dists > 0
+        if not inds.any() and self.default_nearest:
+            inds = _polygon_fallback(inds, self.edges)
+        return inds
+
+
+class ConvexSector(SingleSector):
+    """Convex hull of the vertices ``edges`` in the x-y plane (``grids.py:1380-1435``)."""
+
+    _params = SingleSector._params + ('edges',)
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'edges', [])
+        super().__init__(**kwargs)
+
+    def contains(self, pos):
+        pos = np.asarray(pos)
+        inds = in_hull(pos[:2, :].T, np.array(self.edges, dtype=float).reshape(-1, 2), border=self.include_border,
+                       tol=self.abs_tol)
+        if not inds.any() and self.default_nearest:
+            inds = _polygon_fallback(inds, self.edges)
+        return inds
+
+
+class MultiSector(Sector):
+    """Union of sectors (``grids.py:1438-1494``)."""
+
+    _params = ('sectors',)
+
+    def _digest_items(self):
+        return tuple(self.sectors)
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'sectors', [])
+        super().__init__(**kwargs)
+
+    def contains(self, pos):
+        pos = np.asarray(pos)
+        inds = np.zeros(pos.shape[1], dtype=bool)
+        for sec in self.sectors:
+            inds |= sec.contains(pos)
+        return inds

@@ -236,6 +236,10 @@ class SourceMixer(HasParams):
 
     _params = ('sources', 'weights')
 
+    def _digest_items(self):
+        # every source by its own digest: a changed source changes what depends on the mixer
+        return (*self.sources, self.weights)
+
     def __init__(self, **kwargs):
         object.__setattr__(self, 'sources', [])
         object.__setattr__(self, 'weights', np.array([]))
