@@ -753,6 +753,101 @@ class BeamformerDamas(BeamformerBase):
             self._fr[sel] = 1
 
 
+class BeamformerDamasPlus(BeamformerDamas):
+    """DAMAS through a general solver instead of Gauss-Seidel sweeps (``fbeamform.py:1211-1331``):
+    ``method`` 'NNLS' or 'LP' (scipy.optimize) or 'LassoLars' or 'OMPCV' (scikit-learn), on the
+    dirty map scaled by ``unit_mult``.  The float64 dirty maps and the full PSFs of a group of bins
+    come from one ``abf_beamform_f64`` and one ``abf_psf`` launch; the solve of each bin is the
+    reference's library call on the host (those libraries have no device form).
+    """
+
+    _params = BeamformerDamas._params + ('method', 'alpha', 'unit_mult')
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'method', 'NNLS')
+        object.__setattr__(self, 'alpha', 0.0)
+        object.__setattr__(self, 'unit_mult', 1e9)
+        super().__init__(**kwargs)
+        if 'n_iter' not in kwargs:
+            object.__setattr__(self, 'n_iter', 500)
+
+    def __setattr__(self, name, value):
+        if name == 'method' and value not in ('NNLS', 'LP', 'LassoLars', 'OMPCV'):
+            raise ValueError("method must be one of 'NNLS', 'LP', 'LassoLars', 'OMPCV'")
+        if name == 'alpha':
+            value = float(value)
+            if not 0.0 <= value <= 1.0:
+                raise ValueError('alpha must lie in [0, 1]')
+        if name == 'unit_mult':
+            value = float(value)
+        super().__setattr__(name, value)
+
+    def _calc(self, ind):
+        ind = np.asarray(ind, dtype=np.int64)
+        normfactor = self.sig_loss_norm()
+        G = self.steer.grid.size
+        if type(self.steer) is not SteeringVector:
+            p = PointSpreadFunction(steer=self.steer, calcmode=self.calcmode, precision=self.psf_precision,
+                                    cached=self.cached, phase_mode=self.phase_mode)
+            _, steer_func = self._beamformer_params()
+            for i in ind:
+                csm = np.array(self.freq_data.csm[i], dtype='complex128')
+                y = beamformerFreq('custom', self.r_diag, normfactor, steer_func(self._f[i]), csm)[0]
+                if self.r_diag:
+                    y[np.sign(y) < 0] = 0.0
+                p.freq = self._f[i]
+                self._ac[i] = damas_plus_solve(p.psf, y, self.method, self.alpha, self.n_iter, self.unit_mult,
+                                               self.r_diag)
+                self._fr[i] = 1
+            return
+        per_bin = G * G * (8 if self.psf_precision == 'float64' else 4)
+        chunk = max(1, int(self.psf_budget // max(per_bin, 1)))
+        csm = self.freq_data.csm_device_f64()
+        for c0 in range(0, len(ind), chunk):
+            sel = ind[c0:c0 + chunk]
+            y = beamform_f64_device(csm, self._f[sel], sel, self.steer, self.steer.steer_type, self.r_diag,
+                                    normfactor, self.phase_mode, clip=self.r_diag).cpu().numpy()
+            psf = psf_device(self.steer, self._f[sel], np.arange(G), self.psf_precision, self.phase_mode)
+            psf = psf.cpu().numpy()
+            for j, i in enumerate(sel):
+                self._ac[i] = damas_plus_solve(psf[j], y[j], self.method, self.alpha, self.n_iter, self.unit_mult,
+                                               self.r_diag)
+                self._fr[i] = 1
+
+
+def damas_plus_solve(psf, y, method: str, alpha: float, n_iter: int, unit: float, r_diag: bool):
+    """One bin of ``BeamformerDamasPlus._calc`` (``fbeamform.py:1281-1331``): solve psf x = y
+    (x >= 0) for the dirty map ``y`` scaled by ``unit`` with the reference's solver and return
+    x / unit."""
+    y = np.asarray(y, dtype=np.float64) * unit
+    if method == 'NNLS':
+        from scipy.optimize import nnls
+        return nnls(psf, y)[0] / unit
+    if method == 'LP':
+        from scipy.optimize import linprog
+        if r_diag:
+            _warn('Linear programming solver may fail when CSM main diagonal is removed for delay-and-sum '
+                  'beamforming.', Warning, stacklevel=5)
+        # maximise the summed map: minimise -psf.sum(1) . x subject to psf x <= y, x >= 0
+        return linprog(c=-1 * psf.sum(1), A_ub=psf, b_ub=y).x / unit
+    if method not in ('LassoLars', 'OMPCV'):
+        raise NotImplementedError(f'Method {method} not implemented.')
+    import warnings
+
+    from sklearn.linear_model import LassoLars, OrthogonalMatchingPursuitCV
+    if method == 'LassoLars':
+        model = LassoLars(alpha=alpha * unit, max_iter=n_iter, positive=True)
+    else:
+        model = OrthogonalMatchingPursuitCV()
+    # columns normalised here and the coefficients scaled back, independent of scikit-learn's
+    # own (version-dependent) normalisation (fbeamform.py:1315-1330)
+    norms = np.linalg.norm(psf, axis=0)
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore', category=FutureWarning)
+        model.fit(psf / norms, y)
+    return model.coef_[:] / norms / unit
+
+
 def beamform_f64_device(csm, freqs, ind, steer: SteeringVector, steer_type: str, r_diag: bool, norm: float,
                         phase_mode: str = 'reference', clip: bool = False, out_norm: bool = False):
     """One ``abf_beamform_f64`` call: float64 maps ``[len(ind), G]`` of the complex128 device CSM

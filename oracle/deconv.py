@@ -12,6 +12,11 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
   iteration restated on the oracle's beamformer forms.
 * ``clean``: the iteration of ``BeamformerClean._calc`` (``fbeamform.py:1578-1598``) on one
   dirty map, with PSF columns from ``psf``.
+* ``damas_plus``: one bin of ``BeamformerDamasPlus._calc`` (``fbeamform.py:1281-1331``), the
+  dirty map scaled by ``unit_mult`` and handed to scipy's ``nnls``/``linprog`` or
+  scikit-learn's ``LassoLars``/``OrthogonalMatchingPursuitCV`` (normalised columns), the
+  third-party solvers the reference calls.  Pinned by exact recovery of a sparse non-negative
+  source distribution from its own blurred map (``tests/test_oracle_deconv.py``).
 * ``damas_gauss_seidel``: ``damasSolverGaussSeidel`` (``fastFuncs.py:820-855``), the
   row-by-row relaxed Gauss-Seidel sweep with the positivity clip, one scalar loop over rows
   per iteration (the dot product of a row with the current solution is the only vectorised
@@ -140,3 +145,23 @@ def clean(dirty, steer_type, r0, rm, k, n_iter=100, damp=0.6, psf_precision='flo
         i_iter += 1
         flag = dirty_sum > abs(dirty).sum(0) and i_iter < n_iter and max(dirty) > 0
     return cl
+
+
+def damas_plus(A, y, method='NNLS', alpha=0.0, n_iter=500, unit=1e9):
+    """``fbeamform.py:1281-1331`` for one bin: x >= 0 with A x ~ y, solved on y * unit."""
+    yu = np.asarray(y, dtype=np.float64) * unit
+    if method == 'NNLS':
+        from scipy.optimize import nnls
+
+        return nnls(A, yu)[0] / unit
+    if method == 'LP':
+        from scipy.optimize import linprog
+
+        return linprog(c=-A.sum(1), A_ub=A, b_ub=yu).x / unit
+    from sklearn.linear_model import LassoLars, OrthogonalMatchingPursuitCV
+
+    model = (LassoLars(alpha=alpha * unit, max_iter=n_iter, positive=True) if method == 'LassoLars'
+             else OrthogonalMatchingPursuitCV())
+    nrm = np.sqrt((A * A).sum(0))
+    model.fit(A / nrm, yu)
+    return model.coef_ / nrm / unit

@@ -1,6 +1,8 @@
 """Host logic of PointSpreadFunction, BeamformerDamas and BeamformerCleansc (no GPU): parameters,
 digests, workspace sizes and the index check that runs before any device work (fbeamform.py:948-1492)."""
 
+import warnings
+
 import numpy as np
 import pytest
 
@@ -115,3 +117,42 @@ def test_clean_parameters():
     n = 4 * 256 + (M * G * 16 + 255) // 256 * 256 + (G * 8 + 255) // 256 * 256
     assert lib.abf_clean_workspace_size(M, G, F) == n
     assert lib.abf_clean_workspace_size(0, G, F) == 0
+
+
+def test_damas_plus_parameters():
+    bf = aa.BeamformerDamasPlus()
+    assert (bf.method, bf.alpha, bf.n_iter, bf.unit_mult) == ('NNLS', 0.0, 500, 1e9)
+    assert aa.BeamformerDamasPlus(n_iter=7).n_iter == 7
+    with pytest.raises(ValueError):
+        aa.BeamformerDamasPlus(method='SIMPLEX')
+    with pytest.raises(ValueError):
+        aa.BeamformerDamasPlus(alpha=1.5)
+    d = bf.digest
+    bf.method = 'LP'
+    assert bf.digest != d
+    d = bf.digest
+    bf.unit_mult = 1e6
+    assert bf.digest != d
+
+
+@pytest.mark.parametrize('method', ['NNLS', 'LassoLars', 'OMPCV', 'LP'])
+def test_damas_plus_solve_recovers_sparse_sources(method):
+    """The host solve of every method recovers a sparse non-negative distribution from its own
+    blurred map (symmetric PSF for LP, whose optimum is then A^-1 y)."""
+    from acoular_amd.fbeamform import damas_plus_solve
+    from tests.test_oracle_deconv import _recovery_case
+
+    A, x, y = _recovery_case(method == 'LP')
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore')
+        got = damas_plus_solve(A, y, method, 0.0, 500, 1e9, r_diag=True)
+    assert np.abs(got - x).max() <= 1e-6 * x.max(), (method, got[[6, 18]])
+
+
+def test_damas_plus_lp_warns_with_r_diag():
+    from acoular_amd.fbeamform import damas_plus_solve
+    from tests.test_oracle_deconv import _recovery_case
+
+    A, _, y = _recovery_case(True)
+    with pytest.warns(Warning, match='Linear programming'):
+        damas_plus_solve(A, y, 'LP', 0.0, 500, 1e9, r_diag=True)

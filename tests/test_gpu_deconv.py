@@ -1,6 +1,7 @@
 """GPU parity of the point spread function (``abf_psf``, fastFuncs.py:563-801) and DAMAS
 (``abf_damas``, fastFuncs.py:820-855) and of the classes on top (``PointSpreadFunction``,
-fbeamform.py:948-1124; ``BeamformerDamas``, fbeamform.py:1127-1208) against the oracle
+fbeamform.py:948-1124; ``BeamformerDamas``, fbeamform.py:1127-1208; ``BeamformerDamasPlus``,
+fbeamform.py:1211-1331) against the oracle
 (oracle/deconv.py, oracle/cpu_ref.c).
 
 Tolerances: PSFs to 1e-5 of their largest value (float32 cos/sin and float32 partial sums
@@ -153,6 +154,29 @@ def test_beamformer_damas_matches_oracle(gpu, steer, r_diag):
     want = y.copy()
     cref.damas_gauss_seidel(A, y, want, 100, 1.0)
     assert peak_rel_err(got, want) <= 1e-4
+
+
+@pytest.mark.parametrize(('method', 'steer', 'r_diag'), [('NNLS', 'true level', True), ('NNLS', 'classic', False),
+                                                         ('LP', 'true level', False), ('LassoLars', 'true level', True),
+                                                         ('OMPCV', 'classic', True)])
+def test_beamformer_damas_plus_matches_oracle(gpu, method, steer, r_diag):
+    """BeamformerDamasPlus (fbeamform.py:1211-1331): device dirty maps and PSFs, the reference's
+    solver on the host, against the oracle's dirty maps and PSFs through the same solver."""
+    mics, grid = _setup(48, 9, seed=5, inc=0.05)
+    spos = synthetic.source_positions(2, [-0.15, -0.15, 0.6], [0.15, 0.15, 0.6], 7)
+    freqs = np.array([6000.0, 9000.0, 12000.0])
+    csm = synthetic.analytic_csm(mics.pos, spos, synthetic.level_powers(2, 6.0, 8), freqs, dtype=np.complex128)
+    ps = aa.PowerSpectraImport(csm=csm, frequencies=freqs)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer)
+    bf = aa.BeamformerDamasPlus(freq_data=ps, steer=sv, r_diag=r_diag, method=method)
+    got = bf.result[:]
+    y = B.beamformer_map(csm, freqs, grid.pos, mics.pos, steer, r_diag)
+    r0 = B.steer_r0(grid.pos, np.zeros(3))
+    rm = B.dist_mat(grid.pos, mics.pos)
+    for f, fr in enumerate(freqs):
+        A = D.psf(steer, r0, rm, 2 * np.pi * fr / 343.0, np.arange(grid.size))
+        want = D.damas_plus(A, y[f], method)
+        assert peak_rel_err(got[f][None], want[None]) <= 1e-4, (method, fr, peak_rel_err(got[f][None], want[None]))
 
 
 @pytest.mark.parametrize('steer', STEERS)

@@ -188,3 +188,38 @@ def test_clean_stop_rules():
     assert np.count_nonzero(out) == 1 and out[0] == 0.5 * dirty[0]
     out = D.clean(np.arange(1.0, 6.0), 'classic', r0, rm, 1.0, n_iter=0, damp=0.5)
     assert np.count_nonzero(out) == 1 and out[4] == 2.5
+
+
+def _recovery_case(sym):
+    """A real PSF (true level, 32 mics, 5 x 5 points, 8 kHz) and a two-source map made from it."""
+    mpos = np.random.default_rng(4).uniform(-0.5, 0.5, (3, 32))
+    mpos[2] = 0.0
+    xs, ys = np.meshgrid(np.linspace(-0.2, 0.2, 5), np.linspace(-0.2, 0.2, 5), indexing='ij')
+    gpos = np.stack([xs.ravel(), ys.ravel(), np.full(25, 0.5)])
+    rm = B.dist_mat(gpos, mpos)
+    A = D.psf('true level', B.steer_r0(gpos, np.zeros(3)), rm, 2 * np.pi * 8000 / 343.0, np.arange(25))
+    if sym:
+        A = 0.5 * (A + A.T)
+    x = np.zeros(25)
+    x[[6, 18]] = [2.0e-9, 0.5e-9]
+    return A, x, A @ x
+
+
+@pytest.mark.parametrize('method', ['NNLS', 'LassoLars', 'OMPCV'])
+def test_damas_plus_recovers_sparse_sources(method):
+    """A non-negative source distribution is recovered exactly from its own blurred map by every
+    least-squares method (the PSF is non-singular); the transposed PSF (not symmetric) misses it."""
+    A, x, y = _recovery_case(False)
+    assert np.abs(A - A.T).max() > 1e-3 * A.max()
+    got = D.damas_plus(A, y, method)
+    assert np.abs(got - x).max() <= 1e-6 * x.max(), (method, got[[6, 18]])
+    if method == 'NNLS':
+        assert np.abs(D.damas_plus(A.T, y, method) - x).max() > 1e-3 * x.max()
+
+
+def test_damas_plus_lp_optimum():
+    """With a symmetric non-singular PSF, maximising the summed map under A x <= y, x >= 0 has the
+    unique optimum x = A^-1 y (1'A x <= 1'y, equality only at A x = y)."""
+    A, x, y = _recovery_case(True)
+    got = D.damas_plus(A, y, 'LP')
+    assert np.abs(got - x).max() <= 1e-6 * x.max(), got[[6, 18]]
