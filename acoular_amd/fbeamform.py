@@ -848,6 +848,163 @@ def damas_plus_solve(psf, y, method: str, alpha: float, n_iter: int, unit: float
     return model.coef_[:] / norms / unit
 
 
+class BeamformerAdaptiveGrid(BeamformerBase, Grid):
+    """A beamformer whose result lives on positions it finds itself (``fbeamform.py:2242-2282``):
+    ``pos`` holds those positions (3, ``size``), ``shape`` is ``(size,)``."""
+
+    @property
+    def shape(self):
+        return (self.size,)
+
+    @property
+    def pos(self):
+        return getattr(self, '_gpos', None)
+
+    @property
+    def result(self):
+        d = self.digest
+        if self._result_digest != d:
+            self._f = self.freq_data.fftfreq()
+            self._numfreq = self._f.shape[0]
+            self._assert_equal_channels()
+            # fbeamform.py:425-428
+            self._gpos = np.zeros((3, self.size), dtype=self.precision)
+            self._ac = np.zeros((self._numfreq, self.size), dtype=self.precision)
+            self._fr = np.zeros(self._numfreq, dtype='int8')
+            object.__setattr__(self, '_result_digest', d)
+        return LazyBfResult(self)
+
+    def integrate(self, sector):
+        """Per-bin sum of the result over the found positions inside ``sector``
+        (``fbeamform.py:2255-2282``)."""
+        from .grids import Sector
+
+        if not isinstance(sector, Sector):
+            msg = (f'Please use a sector derived instance of type :class:`~acoular.grids.Sector` instead of type '
+                   f'{type(sector)}.')
+            raise NotImplementedError(msg)
+        # every bin first: the positions the sector selects from exist only once computed (the
+        # reference takes the subdomain before, and the bin count from the lazy result's
+        # r.shape[0], which it does not have)
+        r = self.result[:]
+        ind = self.subdomain(sector)
+        h = np.zeros(r.shape[0])
+        for i in range(r.shape[0]):
+            h[i] = r[i][ind].sum()
+        return h
+
+
+class BeamformerGridlessOrth(BeamformerAdaptiveGrid):
+    """Orthogonal beamforming without a grid (``fbeamform.py:2285-2410``): for every bin and every
+    selected eigenvector, scipy's ``shgo`` looks for the position inside ``bounds`` where the
+    eigenvector's map (eigenvalue 1) peaks; the position and eigenvalue / M are stored.
+
+    The objective is the drop-in ``beamformerFreq`` (eigenvalue form, one point) as in the
+    reference; the sampling points of each ``shgo`` iteration are evaluated together in one
+    ``beamformerFreq`` call over all of them (``shgo``'s ``workers`` map), which gives the same
+    values as one call per point.
+    """
+
+    _params = BeamformerBase._params + ('eva_list', 'n', 'bounds', 'shgo')
+    _fixed = {'r_diag_norm': 1.0}
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, 'eva_list', np.array([-1]))
+        object.__setattr__(self, 'n', 1)
+        object.__setattr__(self, 'bounds', [(-1.0, 1.0), (-1.0, 1.0), (0.01, 1.0)])
+        object.__setattr__(self, 'shgo', {})
+        super().__init__(**kwargs)
+
+    def __setattr__(self, name, value):
+        if name == 'eva_list':
+            value = np.atleast_1d(np.asarray(value, dtype=int))
+        if name == 'bounds':
+            value = [(float(a), float(b)) for a, b in value]
+            if len(value) != 3:
+                raise ValueError('bounds must hold three (min, max) pairs')
+        if name == 'shgo':
+            value = dict(value)
+        if name == 'n':
+            value = int(value)
+            super().__setattr__('n', value)
+            # the observer of the reference (fbeamform.py:2331-2334)
+            super().__setattr__('eva_list', np.arange(-1, -1 - value, -1))
+            return
+        super().__setattr__(name, value)
+
+    def _digest_items(self):
+        # fbeamform.py:2322-2324: n acts through eva_list and the size
+        return tuple(getattr(self, p) for p in self._params if p not in ('r_diag_norm', 'form'))
+
+    @property
+    def size(self):
+        return self.n * self.freq_data.fftfreq().shape[0]
+
+    def _roi(self):
+        return np.array([(x, y, z) for x in self.bounds[0] for y in self.bounds[1] for z in self.bounds[2]]).T
+
+    @property
+    def result(self):
+        # the region of interest the reference sets inside _calc (fbeamform.py:2374-2380), set
+        # before the digest is taken so that it does not invalidate the result it belongs to
+        roi = self._roi()
+        if self.steer.env.roi is None or not np.array_equal(np.asarray(self.steer.env.roi), roi):
+            self.steer.env.roi = roi
+        return super().result
+
+    def _calc(self, ind):
+        """``fbeamform.py:2340-2410``."""
+        from scipy.optimize import shgo
+
+        f = self._f
+        normfactor = self.sig_loss_norm()
+        num_channels = self.freq_data.num_channels
+        eva_list = np.unique(self.eva_list % self.steer.mics.num_mics)[::-1]
+        steer_type = self.steer.steer_type
+        if steer_type == 'custom':
+            raise NotImplementedError('custom steer_type is not implemented')
+        mpos = self.steer.mics.pos
+        env = self.steer.env
+        opts = {'n': 256, 'iters': 1, 'sampling_method': 'sobol', 'minimizer_kwargs': {'method': 'Nelder-Mead'}}
+        opts.update(self.shgo)
+        bmin = np.array(tuple(map(min, self.bounds)))
+        bmax = np.array(tuple(map(max, self.bounds)))
+        for i in ind:
+            eigvals = np.array(self.freq_data.eigvals[i], dtype='float64')
+            eigvecs = np.array(self.freq_data.eigvecs[i], dtype='complex128')
+            k = 2 * np.pi * f[i] / env.c
+            for j, n in enumerate(eva_list):
+                vec = (np.ones(1), eigvecs[:, n:n + 1])
+
+                def levels(pts, vec=vec, k=k):
+                    # minus the map of eigenvector n at the clipped points (3, P)
+                    pts = np.clip(pts, bmin[:, None], bmax[:, None])
+                    r0 = env.apparent_r(pts)
+                    rm = env.apparent_r(pts, mpos)
+                    return -beamformerFreq(steer_type, self.r_diag, normfactor, (np.atleast_1d(r0),
+                                                                                  np.atleast_2d(rm), k), vec)[0]
+
+                def func(xy, levels=levels):
+                    return levels(np.asarray(xy, dtype=np.float64)[:, None])[0]
+
+                def batch(fn, points, levels=levels):
+                    # shgo's map over sampling points: one device call for all of them, with
+                    # FieldWrapper's treatment of nan (scipy/optimize/_shgo_lib/_vertex.py)
+                    pts = list(points)
+                    if not pts:
+                        return []
+                    v = levels(np.stack([np.asarray(p, dtype=np.float64) for p in pts], axis=1))
+                    return [np.inf if np.isnan(x) else float(x) for x in v]
+
+                o = dict(opts)
+                o.setdefault('workers', batch)
+                res = shgo(func, self.bounds, **o)
+                i1 = i * self.n + j
+                self._gpos[:, i1] = res['x']
+                self._ac[i, i1] = eigvals[n] / num_channels
+            self._fr[i] = 1
+
+
 def beamform_f64_device(csm, freqs, ind, steer: SteeringVector, steer_type: str, r_diag: bool, norm: float,
                         phase_mode: str = 'reference', clip: bool = False, out_norm: bool = False):
     """One ``abf_beamform_f64`` call: float64 maps ``[len(ind), G]`` of the complex128 device CSM
