@@ -15,7 +15,8 @@ SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f16.hip', 'abf_bf_mfma_f32.hip',
 # the batched Hermitian eigensolver of the eigenvalue path; the FFTs of the signal resampling
 LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocsolver', '-lrocblas', '-lrocfft']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
-# packed fp32 instructions are disabled on purpose (build flags)
+# packed fp32 instructions (v_pk_*_f32) are disabled on purpose: some of their forms have given
+# silently wrong results on MI355X boxes
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-Xclang', '-target-feature', '-Xclang',
          '-packed-fp32-ops', '-I', os.path.join(HERE, '..', 'include')]
 

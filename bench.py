@@ -1002,8 +1002,9 @@ def main():
     ap.add_argument('--serve-plain', type=int, default=300, help='serving: requests of the plain baseline runs')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is '
-              'described in DESIGN.md and not built.', file=sys.stderr)
+        print(f'bench.py runs on one GPU only (asked for {args.gpus}); sharding this path over several GPUs '
+              '(bins or grid tiles per GPU, spectra exchanged by one all-to-all) is designed but not built.',
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

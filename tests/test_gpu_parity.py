@@ -276,7 +276,7 @@ def test_welch_multi_pass(gpu, monkeypatch):
 # ---------------------------------------------------------------------------------------------
 def test_beamformer_base_api(gpu):
     """The reference's entry sequence: PowerSpectra -> SteeringVector -> BeamformerBase.synthetic
-    (README / demo), lazily per frequency, checked against the oracle chain."""
+    (the reference's README example), lazily per frequency, checked against the oracle chain."""
     mpos = synthetic.disk_array(32, 1.0, 3)
     spos = synthetic.source_positions(2, [-0.1, -0.1, 0.3], [0.1, 0.1, 0.3], 4)
     data = synthetic.point_source_signals(mpos, spos, 25600, 51200.0, seed=5, noise_rel=0.1)
