@@ -897,14 +897,25 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
         a.b0 = b0;
         a.nb = nb;
         a.P = P;
-        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)cdiv(M, 2 * P), nb), dim3(512),
-                           (size_t)P * bs * sizeof(float2), s, a);
-        if (herk_tr == 2)
-            hipLaunchKernelGGL(welch_herk_kernel<2>, dim3(cdiv(M, 32) * nT, F), dim3(64), 0, s, spec, nb, M, nT,
-                               reinterpret_cast<float2 *>(csm_out));
-        else
-            hipLaunchKernelGGL(welch_herk_kernel<4>, dim3(nT * nT, F), dim3(64), 0, s, spec, nb, M, nT,
-                               reinterpret_cast<float2 *>(csm_out));
+        const int ng = (int)cdiv(M, 2 * P);
+        const int per_launch = (int)std::max<int64_t>(1, kFftMaxWorkgroups / ng);
+        for (int c0 = 0; c0 < nb; c0 += per_launch) {
+            a.bl0 = c0;
+            hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)((int64_t)ng * std::min(per_launch, nb - c0))),
+                               dim3(512), (size_t)P * bs * sizeof(float2), s, a);
+        }
+        // one-wave workgroups: 2^24 per launch stays far below the 2^32 work-items a dispatch counts
+        const int64_t tiles = (int64_t)(herk_tr == 2 ? cdiv(M, 32) : nT) * nT;
+        const int fper = (int)std::max<int64_t>(1, (int64_t(1) << 24) / tiles);
+        for (int fc = 0; fc < F; fc += fper) {
+            const unsigned nwg = (unsigned)(tiles * std::min(fper, F - fc));
+            if (herk_tr == 2)
+                hipLaunchKernelGGL(welch_herk_kernel<2>, dim3(nwg), dim3(64), 0, s, spec, nb, M, nT, fc,
+                                   reinterpret_cast<float2 *>(csm_out));
+            else
+                hipLaunchKernelGGL(welch_herk_kernel<4>, dim3(nwg), dim3(64), 0, s, spec, nb, M, nT, fc,
+                                   reinterpret_cast<float2 *>(csm_out));
+        }
     }
     hipLaunchKernelGGL(welch_finalize_kernel, dim3((unsigned)cdiv((int64_t)M * M, 256), F), dim3(256), 0, s,
                        reinterpret_cast<float2 *>(csm_out), M, (float)scale);
@@ -944,7 +955,9 @@ int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, in
     // channel pairs per workgroup: at most 64 KB of LDS (two workgroups per CU; measured 2.36 vs 2.97 ms
     // for 128 KB on 499 blocks x 512 channels of 2048 samples); 16384-sample blocks need 128 KB
     const int P = std::max(1, std::min({env_int("ABF_FFT_P", 16), 8192 / bs, (M + 1) / 2}));
-    for (int b0 = 0; b0 < num_blocks; b0 += 65535) {
+    const int ng = (int)cdiv(M, 2 * P);
+    const int per_launch = (int)std::min<int64_t>(65535, std::max<int64_t>(1, kFftMaxWorkgroups / ng));
+    for (int b0 = 0; b0 < num_blocks; b0 += per_launch) {
         WelchFftArgs a{};
         a.time = time;
         a.window = window;
@@ -957,10 +970,10 @@ int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, in
         a.logbs = logbs;
         a.hop = hop;
         a.b0 = b0;
-        a.nb = std::min(65535, num_blocks - b0);
+        a.nb = std::min(per_launch, num_blocks - b0);
         a.P = P;
         a.block_major = 1;
-        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)cdiv(M, 2 * P), a.nb), dim3(512),
+        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)((int64_t)ng * a.nb)), dim3(512),
                            (size_t)P * bs * sizeof(float2), s, a);
     }
     return hip_status("block FFT launch");
@@ -990,9 +1003,11 @@ int abf_irfft_blocks(const void *spec, int32_t num_blocks, int32_t num_mics, int
         attr = true;
     }
     const int P = std::max(1, std::min({16, 8192 / bs, (M + 1) / 2}));
-    for (int b0 = 0; b0 < num_blocks; b0 += 65535) {
-        const int nb = std::min(65535, num_blocks - b0);
-        hipLaunchKernelGGL(irfft_kernel, dim3((unsigned)cdiv(M, 2 * P), nb), dim3(512), (size_t)P * bs * sizeof(float2),
+    const int ng = (int)cdiv(M, 2 * P);
+    const int per_launch = (int)std::min<int64_t>(65535, std::max<int64_t>(1, kFftMaxWorkgroups / ng));
+    for (int b0 = 0; b0 < num_blocks; b0 += per_launch) {
+        const int nb = std::min(per_launch, num_blocks - b0);
+        hipLaunchKernelGGL(irfft_kernel, dim3((unsigned)((int64_t)ng * nb)), dim3(512), (size_t)P * bs * sizeof(float2),
                            s, reinterpret_cast<const float2 *>(spec), M, bs, logbs, b0, P, tw, out);
     }
     return hip_status("inverse block FFT launch");

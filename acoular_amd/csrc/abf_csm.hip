@@ -53,6 +53,18 @@ __global__ void calc_csm_c128_kernel(double2 *__restrict__ csm, const double2 *_
     csm[idx] = v;
 }
 
+// Workgroups are handed to the eight XCDs in turn (workgroup w runs on XCD w % 8), and each XCD
+// has its own L2.  xcd_order(w, n) renumbers the n workgroups of a launch so that XCD x takes one
+// contiguous range of logical indices, x q + min(x, r) + w / 8 with n = 8 q + r: the neighbouring
+// channel groups of a block (which share the 128-byte lines of every time sample and spectrum
+// row) and the next, half-overlapping block then meet in one L2 instead of each XCD fetching
+// whole lines for a quarter of their bytes.  A bijection of [0, n).
+__device__ __forceinline__ int64_t xcd_order(int64_t w, int64_t n)
+{
+    const int64_t q = n >> 3, r = n & 7, x = w & 7;
+    return x * q + (x < r ? x : r) + (w >> 3);
+}
+
 __device__ __forceinline__ int bitrev(int n, int bits) { return (int)(__builtin_bitreverse32((uint32_t)n) >> (32 - bits)); }
 
 // radix-2 decimation-in-time stages over P sequences of bs points in LDS (bit-reversed order on
@@ -103,8 +115,11 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
 {
     extern __shared__ float2 z[]; // [P][bs]
     const int bs = a.bs, P = a.P;
-    const int bl = blockIdx.y;
-    const int m0 = blockIdx.x * 2 * P;
+    // grid: (channel groups x blocks of this launch) workgroups, channel group fastest
+    const int ng = (a.M + 2 * P - 1) / (2 * P);
+    const int64_t L = xcd_order(blockIdx.x, (int64_t)gridDim.x);
+    const int bl = a.bl0 + (int)(L / ng);
+    const int m0 = (int)(L % ng) * 2 * P;
     const int64_t t0 = (int64_t)(a.b0 + bl) * a.hop;
     for (int idx = threadIdx.x; idx < P * bs; idx += blockDim.x) {
         const int p = idx % P, n = idx / P;
@@ -137,20 +152,24 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
     }
 }
 
-// grid: (nTr * nTc, F), one wave per tile of 16 TR rows x 64 columns of one frequency on or above
-// the diagonal (TR = 4: 64 x 64 tiles, 128 accumulator registers; TR = 2: 32 x 64 tiles, 64
+// grid: nTr * nTc tiles x the frequencies f0 ... of this launch, tile fastest, in the XCD order of
+// xcd_order (the tiles of one frequency then read its spectra through one L2); one wave per tile
+// of 16 TR rows x 64 columns of one frequency on or above the diagonal (TR = 4: 64 x 64 tiles, 128 accumulator registers; TR = 2: 32 x 64 tiles, 64
 // accumulator registers and twice the waves per SIMD, one third more loads per MFMA).  The
 // waves-per-EU bound keeps the accumulators in VGPRs so several waves share a SIMD and hide each
 // other's load latency; without it the compiler takes 136 VGPRs + 128 AGPRs for TR = 4, one wave
 // per SIMD.
 template <int TR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) void welch_herk_kernel(
-    const float2 *__restrict__ spec, int nb, int M, int nTc, float2 *__restrict__ csm)
+    const float2 *__restrict__ spec, int nb, int M, int nTc, int f0, float2 *__restrict__ csm)
 {
-    const int I = blockIdx.x / nTc, J = blockIdx.x % nTc;
+    const int tiles = (M + 16 * TR - 1) / (16 * TR) * nTc;
+    const int64_t L = xcd_order(blockIdx.x, (int64_t)gridDim.x);
+    const int tile = (int)(L % tiles);
+    const int I = tile / nTc, J = tile % nTc;
     const int r0 = I * 16 * TR, c0 = J * 64;
     if (c0 + 63 < r0) return; // entirely below the diagonal
-    const int f = blockIdx.y;
+    const int f = f0 + (int)(L / tiles);
     const int l = threadIdx.x;
     const float2 *X = spec + (int64_t)f * nb * M;
     f4 accr[TR][4], acci[TR][4];
@@ -204,8 +223,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
             }
 }
 
-template __global__ void welch_herk_kernel<4>(const float2 *, int, int, int, float2 *);
-template __global__ void welch_herk_kernel<2>(const float2 *, int, int, int, float2 *);
+template __global__ void welch_herk_kernel<4>(const float2 *, int, int, int, int, float2 *);
+template __global__ void welch_herk_kernel<2>(const float2 *, int, int, int, int, float2 *);
 
 __global__ void welch_finalize_kernel(float2 *__restrict__ csm, int M, float scale)
 {
@@ -227,14 +246,17 @@ __global__ void welch_finalize_kernel(float2 *__restrict__ csm, int M, float sca
 // [nb][F][M], two channels per complex sequence: Z = Xa + i Xb over all bs bins (the upper half
 // from Hermitian symmetry, the imaginary parts of the DC and Nyquist bins dropped as a real
 // inverse transform drops them), z = conj(FFT(conj Z)) / bs = xa + i xb.  Output time-major
-// [nb * bs][M].  grid: (ceil(M / 2P), blocks of this launch)
+// [nb * bs][M].  grid: ceil(M / 2P) channel groups x blocks of this launch, channel group fastest,
+// in the XCD order of xcd_order
 __global__ __launch_bounds__(512) void irfft_kernel(const float2 *__restrict__ spec, int M, int bs, int logbs, int b0,
                                                     int P, const float2 *__restrict__ twiddle, float *__restrict__ out)
 {
     extern __shared__ float2 z[]; // [P][bs]
     const int F = bs / 2 + 1;
-    const int64_t bl = (int64_t)b0 + blockIdx.y;
-    const int m0 = blockIdx.x * 2 * P;
+    const int ng = (M + 2 * P - 1) / (2 * P);
+    const int64_t L = xcd_order(blockIdx.x, (int64_t)gridDim.x);
+    const int64_t bl = (int64_t)b0 + L / ng; // b0: first block of this launch
+    const int m0 = (int)(L % ng) * 2 * P;
     for (int idx = threadIdx.x; idx < P * bs; idx += blockDim.x) {
         const int p = idx % P, k = idx / P;
         const int ma = m0 + 2 * p, mb = ma + 1;

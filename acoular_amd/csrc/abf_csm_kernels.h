@@ -12,13 +12,18 @@ struct WelchFftArgs {
     const float *bin_scale; // [bs/2+1] factor per frequency bin, or null
     int64_t N;
     int M, bs, logbs, hop, b0, nb, P, block_major;
+    int bl0; // first block of this launch within the pass (blocks b0 + bl0 ...)
 };
+
+// workgroups per FFT launch: 2^22 x 512 work-items stays below the 2^32 a dispatch can count
+constexpr int64_t kFftMaxWorkgroups = int64_t(1) << 22;
 
 __global__ void fill_zero_kernel(float4 *p, int64_t n4);
 __global__ void calc_csm_c64_kernel(float2 *csm, const float2 *spec, int F, int M);
 __global__ void calc_csm_c128_kernel(double2 *csm, const double2 *spec, int F, int M);
 __global__ void welch_fft_kernel(WelchFftArgs a);
-template <int TR> __global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nTc, float2 *csm);
+template <int TR>
+__global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nTc, int f0, float2 *csm);
 __global__ void welch_finalize_kernel(float2 *csm, int M, float scale);
 __global__ void irfft_kernel(const float2 *spec, int M, int bs, int logbs, int b0, int P, const float2 *twiddle,
                              float *out);
