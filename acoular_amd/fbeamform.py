@@ -816,7 +816,7 @@ class BeamformerDamasPlus(BeamformerDamas):
 
 
 def damas_plus_solve(psf, y, method: str, alpha: float, n_iter: int, unit: float, r_diag: bool):
-    """One bin of ``BeamformerDamasPlus._calc`` (``fbeamform.py:1281-1331``): solve psf x = y
+    """One bin of ``BeamformerDamasPlus._calc`` (``fbeamform.py:1286-1331``): solve psf x = y
     (x >= 0) for the dirty map ``y`` scaled by ``unit`` with the reference's solver and return
     x / unit."""
     y = np.asarray(y, dtype=np.float64) * unit
@@ -953,7 +953,7 @@ class BeamformerGridlessOrth(BeamformerAdaptiveGrid):
         return super().result
 
     def _calc(self, ind):
-        """``fbeamform.py:2340-2410``."""
+        """``fbeamform.py:2337-2410``."""
         from scipy.optimize import shgo
 
         f = self._f

@@ -12,7 +12,7 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
   iteration restated on the oracle's beamformer forms.
 * ``clean``: the iteration of ``BeamformerClean._calc`` (``fbeamform.py:1578-1598``) on one
   dirty map, with PSF columns from ``psf``.
-* ``damas_plus``: one bin of ``BeamformerDamasPlus._calc`` (``fbeamform.py:1281-1331``), the
+* ``damas_plus``: one bin of ``BeamformerDamasPlus._calc`` (``fbeamform.py:1286-1331``), the
   dirty map scaled by ``unit_mult`` and handed to scipy's ``nnls``/``linprog`` or
   scikit-learn's ``LassoLars``/``OrthogonalMatchingPursuitCV`` (normalised columns), the
   third-party solvers the reference calls.  Pinned by exact recovery of a sparse non-negative
@@ -148,7 +148,7 @@ def clean(dirty, steer_type, r0, rm, k, n_iter=100, damp=0.6, psf_precision='flo
 
 
 def damas_plus(A, y, method='NNLS', alpha=0.0, n_iter=500, unit=1e9):
-    """``fbeamform.py:1281-1331`` for one bin: x >= 0 with A x ~ y, solved on y * unit."""
+    """``fbeamform.py:1286-1331`` for one bin: x >= 0 with A x ~ y, solved on y * unit."""
     yu = np.asarray(y, dtype=np.float64) * unit
     if method == 'NNLS':
         from scipy.optimize import nnls

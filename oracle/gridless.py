@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
-* ``gridless_orth``: ``BeamformerGridlessOrth._calc`` (``fbeamform.py:2340-2410``) for one bin:
+* ``gridless_orth``: ``BeamformerGridlessOrth._calc`` (``fbeamform.py:2337-2410``) for one bin:
   the CSM's eigendecomposition (``numpy.linalg.eigh``, ``spectra.py:351-364``), the selected
   eigenvectors from the largest down, and for each one scipy's ``shgo`` (default options
   ``fbeamform.py:2366-2372``) minimising minus the eigenvalue-form map of that eigenvector

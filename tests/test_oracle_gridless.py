@@ -1,5 +1,5 @@
 """Pins of the gridless orthogonal beamformer's restatement (``oracle/gridless.py``,
-fbeamform.py:2340-2410) by a closed form.
+fbeamform.py:2337-2410) by a closed form.
 
 With true-location steering h = a / sqrt(M sum|a|^2) and no diagonal removal, the eigenvalue-form
 map of a unit eigenvector v is |v^H h(x)|^2 <= 1/M (Cauchy-Schwarz), with equality only where a(x)
