@@ -917,7 +917,7 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
                                    reinterpret_cast<float2 *>(csm_out));
         }
     }
-    hipLaunchKernelGGL(welch_finalize_kernel, dim3((unsigned)cdiv((int64_t)M * M, 256), F), dim3(256), 0, s,
+    hipLaunchKernelGGL(welch_finalize_kernel, dim3((unsigned)(cdiv(M, 32) * cdiv(M, 32)), F), dim3(256), 0, s,
                        reinterpret_cast<float2 *>(csm_out), M, (float)scale);
     return hip_status("Welch CSM launch");
 }

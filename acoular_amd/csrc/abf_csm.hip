@@ -226,20 +226,53 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
 template __global__ void welch_herk_kernel<4>(const float2 *, int, int, int, int, float2 *);
 template __global__ void welch_herk_kernel<2>(const float2 *, int, int, int, int, float2 *);
 
-__global__ void welch_finalize_kernel(float2 *__restrict__ csm, int M, float scale)
+// Hermitian completion (spectra.py:412-414) and scaling (spectra.py:416) by 32 x 32 tiles: tile
+// (I, J) on or above the diagonal is read once, scaled and staged in LDS, then written back and
+// written conjugate-transposed into tile (J, I), so that every global access is a row segment;
+// the diagonal comes out real and the lower half of a diagonal tile is taken from its mirror.
+// grid: (nT * nT tiles, F), 256 threads; tiles below the diagonal return at once.
+__global__ __launch_bounds__(256) void welch_finalize_kernel(float2 *__restrict__ csm, int M, float scale)
 {
-    const int f = blockIdx.y;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)M * M) return;
-    const int r = (int)(idx / M), c = (int)(idx % M);
-    if (r > c) return;
-    float2 *C = csm + (int64_t)f * M * M;
-    float2 v = C[(int64_t)r * M + c];
-    v.x *= scale;
-    v.y *= scale;
-    if (r == c) v.y = 0.f;
-    C[(int64_t)r * M + c] = v;
-    if (r != c) C[(int64_t)c * M + r] = make_float2(v.x, -v.y);
+    __shared__ float2 t[32][33];
+    const int nT = (M + 31) / 32;
+    const int I = blockIdx.x / nT, J = blockIdx.x % nT;
+    if (I > J) return;
+    float2 *C = csm + (int64_t)blockIdx.y * M * M;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int r0 = I * 32, c0 = J * 32;
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        float2 v = make_float2(0.f, 0.f);
+        if (r < M && c < M) {
+            v = C[(int64_t)r * M + c];
+            v.x *= scale;
+            v.y *= scale;
+        }
+        t[k][tx] = v;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < M && c < M) {
+            float2 v = t[k][tx];
+            if (r == c) {
+                v.y = 0.f;
+            } else if (r > c) { // only inside a diagonal tile: the conjugate of (c, r)
+                const float2 w = t[tx][k];
+                v = make_float2(w.x, -w.y);
+            }
+            C[(int64_t)r * M + c] = v;
+        }
+    }
+    if (I < J) {
+        for (int k = ty; k < 32; k += 8) {
+            const int r = c0 + k, c = r0 + tx; // element (r, c) of tile (J, I) mirrors (c, r)
+            if (r < M && c < M) {
+                const float2 w = t[tx][k];
+                C[(int64_t)r * M + c] = make_float2(w.x, -w.y);
+            }
+        }
+    }
 }
 
 // IRFFT.result (fprocess.py:227-283): the real inverse transform of every snapshot spectrum
