@@ -863,12 +863,15 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
     char *ws = reinterpret_cast<char *>(workspace);
     float2 *tw = reinterpret_cast<float2 *>(ws);
     float2 *spec = reinterpret_cast<float2 *>(ws + align256((size_t)bs / 2 * sizeof(float2)));
-    const int64_t n4 = (int64_t)F * M * M / 2; // complex64 pairs per float4
-    const int64_t total = (int64_t)F * M * M;
-    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)cdiv(cdiv(total, 2), 256)), dim3(256), 0, s,
-                       reinterpret_cast<float4 *>(csm_out), n4);
-    if (total & 1) // odd element count: clear the last complex64 separately
-        hipLaunchKernelGGL(zero_c64_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<float2 *>(csm_out) + (total - 1));
+    if (num_blocks == 0) { // no pass writes the CSM: it is zero
+        const int64_t n4 = (int64_t)F * M * M / 2; // complex64 pairs per float4
+        const int64_t total = (int64_t)F * M * M;
+        hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)cdiv(cdiv(total, 2), 256)), dim3(256), 0, s,
+                           reinterpret_cast<float4 *>(csm_out), n4);
+        if (total & 1) // odd element count: clear the last complex64 separately
+            hipLaunchKernelGGL(zero_c64_kernel, dim3(1), dim3(64), 0, s,
+                               reinterpret_cast<float2 *>(csm_out) + (total - 1));
+    }
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
     static bool attr = false;
     if (!attr) {
@@ -910,10 +913,10 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
         for (int fc = 0; fc < F; fc += fper) {
             const unsigned nwg = (unsigned)(tiles * std::min(fper, F - fc));
             if (herk_tr == 2)
-                hipLaunchKernelGGL(welch_herk_kernel<2>, dim3(nwg), dim3(64), 0, s, spec, nb, M, nT, fc,
+                hipLaunchKernelGGL(welch_herk_kernel<2>, dim3(nwg), dim3(64), 0, s, spec, nb, M, nT, fc, (int)(b0 > 0),
                                    reinterpret_cast<float2 *>(csm_out));
             else
-                hipLaunchKernelGGL(welch_herk_kernel<4>, dim3(nwg), dim3(64), 0, s, spec, nb, M, nT, fc,
+                hipLaunchKernelGGL(welch_herk_kernel<4>, dim3(nwg), dim3(64), 0, s, spec, nb, M, nT, fc, (int)(b0 > 0),
                                    reinterpret_cast<float2 *>(csm_out));
         }
     }

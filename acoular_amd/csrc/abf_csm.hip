@@ -152,6 +152,7 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
     }
 }
 
+// Writes the upper triangle only (r <= c): the completion reads nothing else.
 // grid: nTr * nTc tiles x the frequencies f0 ... of this launch, tile fastest, in the XCD order of
 // xcd_order (the tiles of one frequency then read its spectra through one L2); one wave per tile
 // of 16 TR rows x 64 columns of one frequency on or above the diagonal (TR = 4: 64 x 64 tiles, 128 accumulator registers; TR = 2: 32 x 64 tiles, 64
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
 // per SIMD.
 template <int TR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) void welch_herk_kernel(
-    const float2 *__restrict__ spec, int nb, int M, int nTc, int f0, float2 *__restrict__ csm)
+    const float2 *__restrict__ spec, int nb, int M, int nTc, int f0, int accumulate, float2 *__restrict__ csm)
 {
     const int tiles = (M + 16 * TR - 1) / (16 * TR) * nTc;
     const int64_t L = xcd_order(blockIdx.x, (int64_t)gridDim.x);
@@ -215,7 +216,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
                 const int r = r0 + ti * 16 + (l >> 4) * 4 + i;
                 const int c = c0 + tj * 16 + (l & 15);
                 if (r < M && c < M && r <= c) {
-                    float2 v = C[(int64_t)r * M + c];
+                    // the first pass stores, later passes add
+                    float2 v = accumulate ? C[(int64_t)r * M + c] : make_float2(0.f, 0.f);
                     v.x += accr[ti][tj][i];
                     v.y += acci[ti][tj][i];
                     C[(int64_t)r * M + c] = v;
@@ -223,8 +225,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
             }
 }
 
-template __global__ void welch_herk_kernel<4>(const float2 *, int, int, int, int, float2 *);
-template __global__ void welch_herk_kernel<2>(const float2 *, int, int, int, int, float2 *);
+template __global__ void welch_herk_kernel<4>(const float2 *, int, int, int, int, int, float2 *);
+template __global__ void welch_herk_kernel<2>(const float2 *, int, int, int, int, int, float2 *);
 
 // Hermitian completion (spectra.py:412-414) and scaling (spectra.py:416) by 32 x 32 tiles: tile
 // (I, J) on or above the diagonal is read once, scaled and staged in LDS, then written back and
