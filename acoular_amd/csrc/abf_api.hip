@@ -882,9 +882,11 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
     // for 128 KB on 499 blocks x 512 channels of 2048 samples); 16384-sample blocks need 128 KB
     const int P = std::max(1, std::min({env_int("ABF_FFT_P", 16), 8192 / bs, (M + 1) / 2}));
     const int nT = (M + 63) / 64;
-    // 32 x 64 tiles (five waves per SIMD): CSM of the 512-mic, 1025-bin workload 16.8 ms against
-    // 17.5 ms with 64 x 64 tiles (three waves per SIMD), and twice the waves for small arrays
-    const int herk_tr = env_int("ABF_HERK_TR", 2) == 4 ? 4 : 2;
+    // 64 x 64 tiles with the next step's loads in flight across the MFMAs (three waves per SIMD)
+    // from 256 mics: CSM of the 512-mic, 1025-bin workload 11.8 ms against 13.9 ms for 32 x 64
+    // tiles (12.4 ms without that prefetch, five waves per SIMD); 32 x 64 tiles below, where
+    // twice the waves matter more than the loads per MFMA
+    const int herk_tr = env_int("ABF_HERK_TR", M >= 256 ? 4 : 2) == 4 ? 4 : 2;
     for (int b0 = 0; b0 < num_blocks; b0 += blocks_per_pass) {
         const int nb = std::min(blocks_per_pass, num_blocks - b0);
         WelchFftArgs a{};

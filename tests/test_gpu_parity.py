@@ -256,6 +256,23 @@ def test_welch_csm_matches_oracle(gpu, bs, window, overlap, nm, n):
     np.testing.assert_array_equal(got, np.conj(got.transpose(0, 2, 1)))
 
 
+@pytest.mark.parametrize('tr', ['2', '4'])
+@pytest.mark.parametrize('nm', [33, 70, 300])
+def test_welch_csm_tile_forms(gpu, monkeypatch, tr, nm):
+    """Both forms of the CSM update (32 x 64 tiles, and 64 x 64 tiles with the next blocks' loads
+    in flight) on ragged mic counts and a pass count that is not a multiple of 4 blocks."""
+    monkeypatch.setenv('ABF_HERK_TR', tr)
+    rng = np.random.default_rng(nm)
+    data = rng.standard_normal((4500, nm))
+    data[:, 1:] += 0.3 * data[:, :1]
+    ps = aa.PowerSpectra(source=aa.TimeSamples(data=data, sample_freq=51200.0), block_size=256, window='Hanning',
+                         overlap='50%', precision='complex64')
+    got = ps.csm
+    want = S.welch_csm(data, 256, 'Hanning', '50%', 'complex128')
+    assert np.abs(got - want).max() / np.abs(want).max() <= 1e-5
+    np.testing.assert_array_equal(got, np.conj(got.transpose(0, 2, 1)))
+
+
 def test_welch_multi_pass(gpu, monkeypatch):
     """The CSM accumulated over several passes of blocks equals one pass."""
     from acoular_amd import spectra
