@@ -604,35 +604,54 @@ __global__ __launch_bounds__(256) void bf_eig_mfma_kernel(EigGridArgs a)
         const int e = a.e0 + rt * 16 + row; // this lane's eigenvector in the A fragment
         const bool eok = e < a.e0 + a.K;
         const int ec = eok ? e : a.e0;
-        f4 accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
-        // four 4-microphone steps per pass: all loads of the pass are issued before its MFMAs
+        // two accumulator pairs (even and odd 4-microphone steps) halve the dependent MFMA chains
+        f4 accr[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+        f4 acci[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+        // eigenvector values of microphones m0 + 4 u + grp, u < 4: loaded one pass ahead,
+        // unconditionally from a clamped row, and zeroed outside the array only after the
+        // current pass's MFMAs, so that the loads stay in flight across them
+        auto load_ev = [&](int m0, float2 (&ev)[4]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ev[u] = V[(int64_t)min(m0 + 4 * u + grp, a.M - 1) * a.M + ec];
+        };
+        auto mask_ev = [&](int m0, float2 (&ev)[4]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (!(m0 + 4 * u + grp < a.M && eok)) ev[u] = make_float2(0.f, 0.f);
+        };
+        float2 ev[4];
+        load_ev(0, ev);
+        mask_ev(0, ev);
         for (int m0 = 0; m0 < Mp; m0 += 16) {
-            float2 ev[4], hv[4];
+            float2 nev[4], hv[4];
+            load_ev(m0 + 16, nev);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int m = m0 + 4 * u + grp;
-                ev[u] = make_float2(0.f, 0.f);
-                hv[u] = make_float2(0.f, 0.f);
-                if (m < Mp) {
-                    if (m < a.M && eok) ev[u] = V[(int64_t)m * a.M + ec];
-                    hv[u] = hx[m * P + ct * 16 + row];
-                }
+                hv[u] = m < Mp ? hx[m * P + ct * 16 + row] : make_float2(0.f, 0.f);
             }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 // Y = conj(V)^T H: Re = Vr Hr + Vi Hi, Im = Vr Hi - Vi Hr
-                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].x, hv[u].x, accr, 0, 0, 0);
-                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].y, hv[u].y, accr, 0, 0, 0);
-                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].x, hv[u].y, acci, 0, 0, 0);
-                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(-ev[u].y, hv[u].x, acci, 0, 0, 0);
+                f4 &ar = accr[u & 1], &ai = acci[u & 1];
+                ar = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].x, hv[u].x, ar, 0, 0, 0);
+                ar = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].y, hv[u].y, ar, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_16x16x4f32(ev[u].x, hv[u].y, ai, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_16x16x4f32(-ev[u].y, hv[u].x, ai, 0, 0, 0);
             }
+            __builtin_amdgcn_sched_barrier(0);
+            mask_ev(m0 + 16, nev);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ev[u] = nev[u];
         }
+        const f4 yr = accr[0] + accr[1], yi = acci[0] + acci[1];
         // lane (col = l % 16, group) holds rows 4 group + i of the tile
         double sq = 0.0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int er = a.e0 + rt * 16 + 4 * grp + i;
-            if (er < a.e0 + a.K) sq += ((double)accr[i] * accr[i] + (double)acci[i] * acci[i]) * lam[er];
+            if (er < a.e0 + a.K) sq += ((double)yr[i] * yr[i] + (double)yi[i] * yi[i]) * lam[er];
         }
         sq += __shfl_xor(sq, 16, 64);
         sq += __shfl_xor(sq, 32, 64);
