@@ -126,6 +126,106 @@ def executed_flops_per_bin(form, nm):
     return per_pair * nk * (nk + 1)
 
 
+def api_bench(args):
+    """The drop-in array functions at their workloads' sizes, all HBM-bound: calcCSM
+    (fastFuncs.py:19-58) as the reference calls it once per block, on a 1025-bin, 512-mic
+    complex64 CSM (the fused workload's shape: every upper-triangle element read and written
+    once per call); calcTransfer (fastFuncs.py:859-897, steer_vector normalisation 'true level')
+    of 40,401 points x 256 mics into complex128 (distances read once, transfer written once);
+    dist_mat (environments.py:65-103) of a 64 x 64 x 32 grid to 1024 mics (float64 written
+    once).  Each against oracle values on a sample; the CPU leg is the oracle's calcCSM."""
+    import torch
+
+    from acoular_amd import _lib
+    from oracle import beamform as B
+    from oracle import spectra as S
+
+    dev = torch.device('cuda', 0)
+    lib = _lib.lib()
+    sh = _lib.stream_handle
+    g = torch.Generator(device=dev).manual_seed(11)
+    out = {}
+    # calcCSM
+    F, M = 1025, 512
+    csm = torch.randn((F, M, M, 2), generator=g, device=dev, dtype=torch.float32)
+    csm = torch.view_as_complex(csm).contiguous()
+    spec = torch.view_as_complex(torch.randn((F, M, 2), generator=g, device=dev, dtype=torch.float32)).contiguous()
+    csm0 = csm[[3, 700]].cpu().numpy()
+
+    def step_csm():
+        _lib.check(lib.abf_calc_csm(_lib.ptr(csm), _lib.ptr(spec), F, M, 0, sh()), 'calcCSM')
+
+    step_csm()
+    want = S.calc_csm_update(csm0.astype(np.complex128), spec[[3, 700]].cpu().numpy().astype(np.complex128))
+    got = csm[[3, 700]].cpu().numpy()
+    iu = np.triu_indices(M)
+    err_csm = float(np.abs(got[:, iu[0], iu[1]] - want[:, iu[0], iu[1]]).max() / np.abs(want).max())
+    t_csm = event_time(step_csm, 20)
+    b_csm = F * (M * (M + 1) // 2) * 16.0 + F * M * 8.0
+    log(f'calcCSM {F} x {M} x {M} complex64: {t_csm * 1e3:.3f} ms per call, {b_csm / t_csm / 1e9:.0f} GB/s, '
+        f'max error {err_csm:.1e}')
+    # calcTransfer + steer_vector 'true level'
+    G, Mt = 40401, 256
+    rm = torch.rand((G, Mt), generator=g, device=dev, dtype=torch.float64) + 0.5
+    r0 = torch.rand((G,), generator=g, device=dev, dtype=torch.float64) + 0.5
+    tr = torch.empty((G, Mt), dtype=torch.complex128, device=dev)
+    k = 2 * np.pi * 8000.0 / 343.0
+
+    def step_tr():
+        _lib.check(lib.abf_transfer(_lib.ptr(r0), _lib.ptr(rm), G, Mt, k, 0, 3, 1, _lib.ptr(tr), sh()), 'transfer')
+
+    step_tr()
+    sel = np.arange(0, G, 997)
+    a = B.calc_transfer(r0.cpu().numpy()[sel], rm.cpu().numpy()[sel], k)
+    want_tr = B.steer_vector(a, 'true level')
+    err_tr = float(np.abs(tr.cpu().numpy()[sel] - want_tr).max() / np.abs(want_tr).max())
+    t_tr = event_time(step_tr, 20)
+    b_tr = G * 8.0 + G * Mt * 8.0 + G * Mt * 16.0
+    log(f'steer_vector {G} x {Mt} complex128: {t_tr * 1e3:.3f} ms, {b_tr / t_tr / 1e9:.0f} GB/s, max error {err_tr:.1e}')
+    # dist_mat
+    Gd, Md = 131072, 1024
+    gpos = torch.rand((3, Gd), generator=g, device=dev, dtype=torch.float64)
+    mpos = torch.rand((3, Md), generator=g, device=dev, dtype=torch.float64)
+    rmd = torch.empty((Gd, Md), dtype=torch.float64, device=dev)
+
+    def step_dm():
+        _lib.check(lib.abf_dist_mat(_lib.ptr(gpos), _lib.ptr(mpos), Gd, Md, _lib.ptr(rmd), sh()), 'dist_mat')
+
+    step_dm()
+    seld = np.arange(0, Gd, 4099)
+    want_dm = B.dist_mat(gpos.cpu().numpy()[:, seld], mpos.cpu().numpy())
+    err_dm = float(np.abs(rmd.cpu().numpy()[seld] - want_dm).max() / want_dm.max())
+    t_dm = event_time(step_dm, 20)
+    b_dm = Gd * Md * 8.0 + (Gd + Md) * 24.0
+    log(f'dist_mat {Gd} x {Md}: {t_dm * 1e3:.3f} ms, {b_dm / t_dm / 1e9:.0f} GB/s, max error {err_dm:.1e}')
+    roof = {'bound': 'hbm', 'achieved': round(b_csm / t_csm / 1e9, 1), 'peak': 8000.0, 'unit': 'GB/s',
+            'kernel': 'calc_csm_c64_kernel', 'algorithmic_bytes_per_launch': b_csm, 'traffic': None}
+    roof['frac'] = round(roof['achieved'] / roof['peak'], 4)
+    cpu = None
+    if not args.no_cpu:
+        c = csm[:64].cpu().numpy().astype(np.complex128)
+        sp = spec[:64].cpu().numpy().astype(np.complex128)
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            S.calc_csm_update(c, sp)
+            done += 1
+        dt = time.perf_counter() - t0
+        cpu = {'value': done * 64 * (M * (M + 1) // 2) * 16.0 / dt / 1e9, 'unit': 'GB/s', 'cores': 1, 'kind': 'port',
+               'sample': f'{done} calcCSM updates of 64 of the 1025 bins (oracle, numpy), {dt:.1f} s'}
+        log(f"CPU calcCSM: {cpu['value']:.2f} GB/s")
+    return {
+        'metric': 'GB/s of the drop-in calcCSM update (1025 bins x 512 mics, complex64)',
+        'value': b_csm / t_csm / 1e9, 'unit': 'GB/s', 'n_gpus': 1, 'ms_per_step': t_csm * 1e3,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32 (complex64)',
+        'data': 'synthetic (seeded normal values)',
+        'config': {'workload': 'drop-in array functions: calcCSM 1025 x 512 x 512 c64, steer_vector 40401 x 256 '
+                               'c128, dist_mat 131072 x 1024 f64', 'parallelism': 'single GPU'},
+        'roofline': roof, 'max_rel_err': err_csm, 'cpu_baseline': cpu,
+        'steer_vector': {'ms': t_tr * 1e3, 'GB/s': b_tr / t_tr / 1e9, 'frac': b_tr / t_tr / 8e12, 'max_rel_err': err_tr},
+        'dist_mat': {'ms': t_dm * 1e3, 'GB/s': b_dm / t_dm / 1e9, 'frac': b_dm / t_dm / 8e12, 'max_rel_err': err_dm},
+    }
+
+
 def time_steps(fn, steps, warmup):
     import torch
 
@@ -989,7 +1089,7 @@ def main():
     ap.add_argument('--no-tune', action='store_true', help='default launch shapes (no autotuner table, no tuning)')
     ap.add_argument('--tune', action='store_true', help='tune the launch shape now even if the table has it')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
-    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig',
+    ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig', 'api',
                                                           'damas', 'cleansc', 'clean', 'rfft', 'c1'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
                          "'serving': the mixed-size request loop (Poisson arrivals) against the unbatched "
@@ -1045,6 +1145,9 @@ def main():
         return
     if args.mode == 'c1':
         print(json.dumps(c1_bench(args)), flush=True)
+        return
+    if args.mode == 'api':
+        print(json.dumps(api_bench(args)), flush=True)
         return
     log('building workload (analytic CSM 513 x 256 x 256 on the GPU)')
     wl = build_workload(dev)
