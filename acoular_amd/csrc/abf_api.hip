@@ -801,8 +801,11 @@ int abf_dist_mat(const double *gpos, const double *mpos, int64_t num_grid, int32
     if (num_mics < 1 || num_grid < 0) return fail(ABF_ERR_INVALID, "bad sizes");
     if (num_grid == 0) return ABF_OK;
     if (!gpos || !mpos || !rm) return fail(ABF_ERR_INVALID, "null pointer");
-    hipLaunchKernelGGL(dist_mat_kernel, dim3((unsigned)cdiv(num_grid * num_mics, 256)), dim3(256), 0, S(stream),
-                       gpos, mpos, num_grid, (int)num_mics, rm);
+    const int64_t nmb = cdiv((int64_t)num_mics, 256);
+    const int64_t per = std::max<int64_t>(1, (int64_t(1) << 22) / nmb); // points per launch
+    for (int64_t g0 = 0; g0 < num_grid; g0 += per)
+        hipLaunchKernelGGL(dist_mat_kernel, dim3((unsigned)(std::min(per, num_grid - g0) * nmb)), dim3(256), 0,
+                           S(stream), gpos, mpos, num_grid, (int)num_mics, g0, rm);
     return hip_status("dist_mat launch");
 }
 
@@ -814,8 +817,13 @@ int abf_transfer(const double *r0, const double *rm, int64_t num_grid, int32_t n
     if (int r = check_phase(phase_mode)) return r;
     if (num_grid == 0) return ABF_OK;
     if (!r0 || !rm || !out) return fail(ABF_ERR_INVALID, "null pointer");
-    hipLaunchKernelGGL(transfer_kernel, dim3((unsigned)cdiv(num_grid, 64)), dim3(64), 0, S(stream), r0, rm, num_grid,
-                       (int)num_mics, wavenumber, phase_mode & kPhaseMask, steer_mode, out_c128, out);
+    for (int64_t g0 = 0; g0 < num_grid; g0 += int64_t(1) << 24) { // 2^22 workgroups of 4 points per launch
+        const int64_t ng = std::min<int64_t>(int64_t(1) << 24, num_grid - g0);
+        const double *r0p = r0 + g0, *rmp = rm + g0 * num_mics;
+        void *outp = static_cast<char *>(out) + g0 * num_mics * (out_c128 ? 16 : 8);
+        hipLaunchKernelGGL(transfer_kernel, dim3((unsigned)cdiv(ng, 4)), dim3(256), 0, S(stream), r0p, rmp, ng,
+                           (int)num_mics, wavenumber, phase_mode & kPhaseMask, steer_mode, out_c128, outp);
+    }
     return hip_status("transfer launch");
 }
 
@@ -824,15 +832,17 @@ int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mic
     if (num_mics < 1 || num_freqs < 0) return fail(ABF_ERR_INVALID, "bad sizes");
     if (num_freqs == 0) return ABF_OK;
     if (!csm || !spec) return fail(ABF_ERR_INVALID, "null pointer");
-    const int64_t n = (int64_t)num_freqs * num_mics * num_mics;
-    if (is_c128)
-        hipLaunchKernelGGL(calc_csm_c128_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, S(stream),
-                           reinterpret_cast<double2 *>(csm), reinterpret_cast<const double2 *>(spec), num_freqs,
-                           num_mics);
-    else
-        hipLaunchKernelGGL(calc_csm_c64_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, S(stream),
-                           reinterpret_cast<float2 *>(csm), reinterpret_cast<const float2 *>(spec), num_freqs,
-                           num_mics);
+    if (num_mics > 65535) return fail(ABF_ERR_UNSUPPORTED, "more than 65535 microphones");
+    const int M = num_mics;
+    for (int f0 = 0; f0 < num_freqs; f0 += 65535) {
+        const dim3 grid((unsigned)cdiv(M, 256), (unsigned)M, (unsigned)std::min(65535, num_freqs - f0));
+        if (is_c128)
+            hipLaunchKernelGGL(calc_csm_c128_kernel, grid, dim3(256), 0, S(stream), reinterpret_cast<double2 *>(csm),
+                               reinterpret_cast<const double2 *>(spec), f0, M);
+        else
+            hipLaunchKernelGGL(calc_csm_c64_kernel, grid, dim3(256), 0, S(stream), reinterpret_cast<float2 *>(csm),
+                               reinterpret_cast<const float2 *>(spec), f0, M);
+    }
     return hip_status("calc_csm launch");
 }
 

@@ -23,34 +23,35 @@ __global__ void fill_zero_kernel(float4 *__restrict__ p, int64_t n4)
     if (i < n4) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-__global__ void calc_csm_c64_kernel(float2 *__restrict__ csm, const float2 *__restrict__ spec, int F, int M)
+// calcCSM update, csm[f][r][c] += conj(spec[f][c]) spec[f][r] for r <= c.  grid: (ceil(M / 256)
+// column blocks, M rows, frequencies f0 ... of this launch): the indices come from the grid, with
+// no 64-bit divisions per element; column blocks left of the diagonal exit at once.
+__global__ __launch_bounds__(256) void calc_csm_c64_kernel(float2 *__restrict__ csm, const float2 *__restrict__ spec,
+                                                           int f0, int M)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t mm = (int64_t)M * M;
-    if (idx >= (int64_t)F * mm) return;
-    const int f = (int)(idx / mm);
-    const int r = (int)((idx / M) % M), c = (int)(idx % M);
-    if (r > c) return;
-    const float2 sr = spec[(int64_t)f * M + r], sc = spec[(int64_t)f * M + c];
-    float2 v = csm[idx];
+    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
+    const int64_t f = (int64_t)f0 + blockIdx.z;
+    if (c >= M || r > c) return;
+    const float2 sr = spec[f * M + r], sc = spec[f * M + c];
+    float2 *p = csm + (f * M + r) * M + c;
+    float2 v = *p;
     v.x += sc.x * sr.x + sc.y * sr.y;
     v.y += sc.x * sr.y - sc.y * sr.x;
-    csm[idx] = v;
+    *p = v;
 }
 
-__global__ void calc_csm_c128_kernel(double2 *__restrict__ csm, const double2 *__restrict__ spec, int F, int M)
+__global__ __launch_bounds__(256) void calc_csm_c128_kernel(double2 *__restrict__ csm,
+                                                            const double2 *__restrict__ spec, int f0, int M)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t mm = (int64_t)M * M;
-    if (idx >= (int64_t)F * mm) return;
-    const int f = (int)(idx / mm);
-    const int r = (int)((idx / M) % M), c = (int)(idx % M);
-    if (r > c) return;
-    const double2 sr = spec[(int64_t)f * M + r], sc = spec[(int64_t)f * M + c];
-    double2 v = csm[idx];
+    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
+    const int64_t f = (int64_t)f0 + blockIdx.z;
+    if (c >= M || r > c) return;
+    const double2 sr = spec[f * M + r], sc = spec[f * M + c];
+    double2 *p = csm + (f * M + r) * M + c;
+    double2 v = *p;
     v.x += sc.x * sr.x + sc.y * sr.y;
     v.y += sc.x * sr.y - sc.y * sr.x;
-    csm[idx] = v;
+    *p = v;
 }
 
 // Workgroups are handed to the eight XCDs in turn (workgroup w runs on XCD w % 8), and each XCD

@@ -5,33 +5,42 @@
 
 namespace abf {
 
-__global__ void dist_mat_kernel(const double *__restrict__ gpos, const double *__restrict__ mpos, int64_t G, int M,
-                                double *__restrict__ rm)
+// grid: (points g0 ... of this launch) x ceil(M / 256) workgroups; workgroup (g, mb) writes
+// rm[g][256 mb ...] (one division per workgroup, none per element)
+__global__ __launch_bounds__(256) void dist_mat_kernel(const double *__restrict__ gpos, const double *__restrict__ mpos,
+                                                       int64_t G, int M, int64_t g0, double *__restrict__ rm)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= G * M) return;
-    const int64_t g = idx / M;
-    const int m = (int)(idx % M);
-    rm[idx] = dist3(gpos[g], gpos[G + g], gpos[2 * G + g], mpos[m], mpos[M + m], mpos[2 * M + m]);
+    const int nmb = (M + 255) / 256;
+    const int64_t g = g0 + blockIdx.x / nmb;
+    const int m = (int)(blockIdx.x % nmb) * 256 + threadIdx.x;
+    if (m >= M) return;
+    rm[g * M + m] = dist3(gpos[g], gpos[G + g], gpos[2 * G + g], mpos[m], mpos[M + m], mpos[2 * M + m]);
 }
 
-// One grid point per thread: a[g][m] = exp(-1j f32(k (rm - r0))) r0 / rm, then the optional
-// steer_vector normalisation; float64 except for the float32 phase (as the reference).
-__global__ void transfer_kernel(const double *__restrict__ r0, const double *__restrict__ rm, int64_t G, int M,
-                                double k, int phase_mode, int steer_mode, int out_c128, void *__restrict__ out)
+// One wave per grid point, the lanes over the microphones (rows of rm read and of out written
+// contiguously): a[g][m] = exp(-1j f32(k (rm - r0))) r0 / rm, then the optional steer_vector
+// normalisation, sum |a|^2 by a wave reduction; float64 except for the float32 phase (as the
+// reference).  grid: ceil(G / 4) workgroups of 4 waves.
+__global__ __launch_bounds__(256) void transfer_kernel(const double *__restrict__ r0, const double *__restrict__ rm,
+                                                       int64_t G, int M, double k, int phase_mode, int steer_mode,
+                                                       int out_c128, void *__restrict__ out)
 {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= G) return;
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (g >= G) return; // uniform per wave
     const double rr0 = r0[g];
+    const double *row = rm + g * M;
     double s2 = 0.0; // sum |a|^2 for true level / true location
     if (steer_mode == 3 || steer_mode == 4) {
-        for (int m = 0; m < M; ++m) {
-            const double q = rr0 / rm[g * M + m];
+        for (int m = lane; m < M; m += 64) {
+            const double q = rr0 / row[m];
             s2 += q * q;
         }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o, 64);
     }
-    for (int m = 0; m < M; ++m) {
-        const double r = rm[g * M + m];
+    for (int m = lane; m < M; m += 64) {
+        const double r = row[m];
         float c, s;
         phasor(k * (r - rr0), phase_mode, c, s);
         const double amp = rr0 / r;
