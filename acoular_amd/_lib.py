@@ -202,13 +202,15 @@ def ptr(t) -> ctypes.c_void_p | None:
 
 
 def to_device(a, dtype=None):
-    """numpy array or tensor -> contiguous device tensor (optionally cast)."""
+    """numpy array or tensor -> contiguous device tensor (optionally cast).  Host data is staged
+    in pinned memory and copied without blocking: the copy is ordered on the current stream like
+    a kernel, so the host does not wait for the work queued before it (a pageable copy does)."""
     t = torch()
     dev = device()
     if isinstance(a, t.Tensor):
-        x = a.to(dev)
+        x = a.to(dev) if a.is_cuda else a.contiguous().pin_memory().to(dev, non_blocking=True)
     else:
-        x = t.from_numpy(np.ascontiguousarray(a)).to(dev)
+        x = t.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dev, non_blocking=True)
     if dtype is not None and x.dtype != dtype:
         x = x.to(dtype)
     return x.contiguous()

@@ -1016,14 +1016,14 @@ def beamform_f64_device(csm, freqs, ind, steer: SteeringVector, steer_type: str,
         raise TypeError('csm must be a complex128 device tensor')
     dev = _lib.device()
     if len(ind) != csm.shape[0] or not np.array_equal(ind, np.arange(csm.shape[0])):
-        csm = csm.index_select(0, torch.from_numpy(ind).to(dev))
+        csm = csm.index_select(0, _lib.to_device(ind))
     csm = csm.contiguous()
     gpos, mpos = steer.device_geometry()
     G, M, F = gpos.shape[1], mpos.shape[1], len(ind)
     if csm.shape[1] != M:
         msg = f'{csm.shape[1]:d} channels do not fit {M:d} mics'
         raise ValueError(msg)
-    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    k = _lib.to_device(2 * np.pi * freqs / steer.env.c)
     out = torch.empty((F, G), dtype=torch.float64, device=dev)
     onorm = torch.empty((F, G), dtype=torch.float64, device=dev) if out_norm else None
     lib = _lib.lib()
@@ -1051,8 +1051,8 @@ def psf_device(steer: SteeringVector, freqs, sources, precision: str = 'float64'
     if fast_trig is None:
         fast_trig = config['fast_trig']
     gpos, mpos = steer.device_geometry()
-    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
-    src = torch.from_numpy(sources).to(dev)
+    k = _lib.to_device(2 * np.pi * freqs / steer.env.c)
+    src = _lib.to_device(sources)
     dt = torch.float64 if precision == 'float64' else torch.float32
     out = torch.empty((len(freqs), G, len(sources)), dtype=dt, device=dev)
     pm = _lib.PHASE_MODES[phase_mode] | (_lib.FAST_TRIG if fast_trig else 0)
@@ -1089,11 +1089,11 @@ def cleansc_device(csm, freqs, ind, steer: SteeringVector, steer_type: str, r_di
     dev = _lib.device()
     if csm.dtype != torch.complex128:
         raise TypeError('csm must be a complex128 device tensor')
-    work = csm.index_select(0, torch.from_numpy(ind).to(dev)).contiguous()
+    work = csm.index_select(0, _lib.to_device(ind)).contiguous()
     h = beamform_f64_device(work, freqs, np.arange(len(ind)), steer, steer_type, r_diag, norm, phase_mode)
     gpos, mpos = steer.device_geometry()
     F, M = len(ind), mpos.shape[1]
-    k = torch.from_numpy(2 * np.pi * np.asarray(freqs, dtype=np.float64) / steer.env.c).to(dev)
+    k = _lib.to_device(2 * np.pi * np.asarray(freqs, dtype=np.float64) / steer.env.c)
     out = torch.empty((F, gpos.shape[1]), dtype=torch.float32, device=dev)
     lib = _lib.lib()
     ws = _lib.workspace(lib.abf_cleansc_workspace_size(M, gpos.shape[1], F, int(n_iter)))
@@ -1121,7 +1121,7 @@ def clean_device(dirty, freqs, steer: SteeringVector, n_iter: int, damp: float, 
         raise ValueError(f'dirty maps of shape {tuple(dirty.shape)} do not fit {gpos.shape[1]:d} grid points and '
                          f'{len(freqs):d} frequencies')
     dev = dirty.device
-    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    k = _lib.to_device(2 * np.pi * freqs / steer.env.c)
     out = torch.empty((F, G), dtype=torch.float64, device=dev)
     lib = _lib.lib()
     ws = _lib.workspace(lib.abf_clean_workspace_size(mpos.shape[1], G, F))
@@ -1162,7 +1162,7 @@ def beamform_eig_device(eigvals, eigvecs, freqs, ind, steer: SteeringVector, ste
     if eigvals.dtype != torch.float64 or eigvecs.dtype != torch.complex64:
         raise TypeError('eigvals must be float64 and eigvecs complex64 device tensors')
     if len(ind) != eigvals.shape[0] or not np.array_equal(ind, np.arange(eigvals.shape[0])):
-        sel = torch.from_numpy(ind).to(dev)
+        sel = _lib.to_device(ind)
         eigvals, eigvecs = eigvals.index_select(0, sel), eigvecs.index_select(0, sel)
     eigvals, eigvecs = eigvals.contiguous(), eigvecs.contiguous()
     gpos, mpos = steer.device_geometry()
@@ -1172,7 +1172,7 @@ def beamform_eig_device(eigvals, eigvecs, freqs, ind, steer: SteeringVector, ste
         raise ValueError(msg)
     if fast_trig is None:
         fast_trig = config['fast_trig']
-    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    k = _lib.to_device(2 * np.pi * freqs / steer.env.c)
     out = torch.empty((F, G), dtype=torch.float32, device=dev)
     onorm = torch.empty((F, G), dtype=torch.float32, device=dev) if out_norm else None
     geom = _geometry(gpos, mpos, steer.ref)
@@ -1199,7 +1199,7 @@ def beamform_device(csm_device, freqs, ind, steer: SteeringVector, steer_type: s
     dev = _lib.device()
     csm = csm_device
     if len(ind) != csm.shape[0] or not np.array_equal(ind, np.arange(csm.shape[0])):
-        csm = csm.index_select(0, torch.from_numpy(ind).to(dev))
+        csm = csm.index_select(0, _lib.to_device(ind))
     csm = csm.contiguous()
     gpos, mpos = steer.device_geometry()
     G, M, F = gpos.shape[1], mpos.shape[1], len(ind)
@@ -1208,7 +1208,7 @@ def beamform_device(csm_device, freqs, ind, steer: SteeringVector, steer_type: s
         raise ValueError(msg)
     if M > 1150 and form != 'plain_f32':
         form = 'plain_f32'
-    k = torch.from_numpy(2 * np.pi * freqs / steer.env.c).to(dev)
+    k = _lib.to_device(2 * np.pi * freqs / steer.env.c)
     out = torch.empty((F, G), dtype=torch.float32, device=dev)
     onorm = torch.empty((F, G), dtype=torch.float32, device=dev) if out_norm else None
     job = make_job(csm, k, gpos, mpos, steer.ref, steer_type, r_diag, norm, form, phase_mode, out, onorm, clip,

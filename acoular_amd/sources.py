@@ -136,9 +136,10 @@ class PointSource(HasParams):
         rm, ind = self._geometry()
         return np.ascontiguousarray(self.signal.usignal(self.up), dtype=np.float64), rm, ind
 
-    def result_device(self, num: int = 128, host_inputs=None, raw_signal=None):
+    def result_device(self, num: int = 128, host_inputs=None, raw_signal=None, geometry=None):
         """All microphone signals ``[T, num_mics]`` (float64, device), T as the block generator yields.
-        ``raw_signal`` (a device copy of ``signal.signal()``) skips generating it here."""
+        ``raw_signal`` (a device copy of ``signal.signal()``) skips generating it here; ``geometry``
+        (``_geometry()``'s result, computed earlier) skips computing it here."""
         from . import _lib
         from .signals import resample_device
 
@@ -146,9 +147,9 @@ class PointSource(HasParams):
         dev = _lib.device()
         if host_inputs is not None:
             signal, rm, ind = host_inputs
-            sig = torch.from_numpy(np.ascontiguousarray(signal, dtype=np.float64)).to(dev)
+            sig = _lib.to_device(np.ascontiguousarray(signal, dtype=np.float64))
         else:
-            rm, ind = self._geometry()
+            rm, ind = geometry if geometry is not None else self._geometry()
             sig = None
         estimate = self.num_samples + (self.start_t - self.start) * self.sample_freq
         T = _block_plan(self.num_samples, estimate, int(num))
@@ -161,8 +162,8 @@ class PointSource(HasParams):
             raw = raw_signal if raw_signal is not None else self.signal.signal_device()
             sig = resample_device(raw, self.up)
         L = int(sig.shape[0])
-        ind0 = torch.from_numpy(np.ascontiguousarray(ind[0])).to(dev)
-        rmd = torch.from_numpy(np.ascontiguousarray(rm[0])).to(dev)
+        ind0 = _lib.to_device(np.ascontiguousarray(ind[0]))
+        rmd = _lib.to_device(np.ascontiguousarray(rm[0]))
         lib = _lib.lib()
         ws = _lib.workspace(lib.abf_point_source_workspace_size(M, T))
         _lib.check(lib.abf_point_source(_lib.ptr(sig), L, _lib.ptr(ind0), _lib.ptr(rmd), M, self.up,
@@ -310,6 +311,10 @@ class SourceMixer(HasParams):
         from .signals import WNoiseGenerator, white_noise_device
 
         torch = _lib.torch()
+        # the point sources' distances come back to the host (their start indices are computed
+        # there, as the reference does): done before any long kernel is queued, so that these
+        # reads do not wait for it
+        geo = {i: s._geometry() for i, s in enumerate(self.sources) if isinstance(s, PointSource)}
         streams = []  # (source index, seeds, rms)
         for i, s in enumerate(self.sources):
             if type(getattr(s, 'signal', None)) is not WNoiseGenerator:
@@ -334,9 +339,11 @@ class SourceMixer(HasParams):
         parts = []
         for i, s in enumerate(self.sources):
             if i in pre and isinstance(s, PointSource):
-                parts.append(s.result_device(num, raw_signal=pre[i].reshape(-1)))
+                parts.append(s.result_device(num, raw_signal=pre[i].reshape(-1), geometry=geo[i]))
             elif i in pre:
                 parts.append(pre[i])
+            elif i in geo:
+                parts.append(s.result_device(num, geometry=geo[i]))
             else:
                 parts.append(s.result_device(num))
         return parts
