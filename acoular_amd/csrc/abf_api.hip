@@ -846,11 +846,28 @@ int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mic
     return hip_status("calc_csm launch");
 }
 
+// Slices per pass of the CSM update: when the tiles of all bins give too few waves to fill the
+// GPU (small arrays, e.g. 64 mics x 65 bins: 130 waves walking 799 blocks each), the blocks of a
+// pass are split into up to 16 slices of at least 32 blocks, whose partial sums the workspace
+// holds (at most 256 MB) and welch_reduce_kernel adds in a fixed order.
+static int welch_split(int M, int bs, int blocks_per_pass)
+{
+    const int64_t F = bs / 2 + 1;
+    const int64_t waves = cdiv((int64_t)M, 64) * cdiv((int64_t)M, 64) * F;
+    int sp = 1;
+    while (sp < 16 && waves * sp * 2 <= 8192 && blocks_per_pass >= 64 * sp &&
+           (int64_t)sp * 2 * F * M * M * 8 <= (int64_t(256) << 20))
+        sp *= 2;
+    return sp;
+}
+
 size_t abf_welch_csm_workspace_size(int32_t num_mics, int32_t block_size, int32_t blocks_per_pass)
 {
     if (num_mics < 1 || block_size < 2 || blocks_per_pass < 1) return 0;
+    const int sp = welch_split(num_mics, block_size, blocks_per_pass);
+    const size_t parts = sp > 1 ? align256((size_t)sp * (block_size / 2 + 1) * num_mics * num_mics * sizeof(float2)) : 0;
     return align256((size_t)block_size / 2 * sizeof(float2)) +
-           align256((size_t)(block_size / 2 + 1) * blocks_per_pass * num_mics * sizeof(float2));
+           align256((size_t)(block_size / 2 + 1) * blocks_per_pass * num_mics * sizeof(float2)) + parts;
 }
 
 int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int32_t block_size, int32_t hop,
@@ -873,6 +890,10 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
     char *ws = reinterpret_cast<char *>(workspace);
     float2 *tw = reinterpret_cast<float2 *>(ws);
     float2 *spec = reinterpret_cast<float2 *>(ws + align256((size_t)bs / 2 * sizeof(float2)));
+    const int nsplit = welch_split(M, bs, blocks_per_pass);
+    float2 *part = reinterpret_cast<float2 *>(
+        ws + align256((size_t)bs / 2 * sizeof(float2)) +
+        align256((size_t)(bs / 2 + 1) * blocks_per_pass * M * sizeof(float2)));
     if (num_blocks == 0) { // no pass writes the CSM: it is zero
         const int64_t n4 = (int64_t)F * M * M / 2; // complex64 pairs per float4
         const int64_t total = (int64_t)F * M * M;
@@ -920,17 +941,20 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
                                dim3(512), (size_t)P * bs * sizeof(float2), s, a);
         }
         // one-wave workgroups: 2^24 per launch stays far below the 2^32 work-items a dispatch counts
-        const int64_t tiles = (int64_t)(herk_tr == 2 ? cdiv(M, 32) : nT) * nT;
+        const int64_t tiles = (int64_t)(herk_tr == 2 ? cdiv(M, 32) : nT) * nT * nsplit;
         const int fper = (int)std::max<int64_t>(1, (int64_t(1) << 24) / tiles);
         for (int fc = 0; fc < F; fc += fper) {
             const unsigned nwg = (unsigned)(tiles * std::min(fper, F - fc));
             if (herk_tr == 2)
                 hipLaunchKernelGGL(welch_herk_kernel<2>, dim3(nwg), dim3(64), 0, s, spec, nb, M, nT, fc, (int)(b0 > 0),
-                                   reinterpret_cast<float2 *>(csm_out));
+                                   reinterpret_cast<float2 *>(csm_out), nsplit, F, part);
             else
                 hipLaunchKernelGGL(welch_herk_kernel<4>, dim3(nwg), dim3(64), 0, s, spec, nb, M, nT, fc, (int)(b0 > 0),
-                                   reinterpret_cast<float2 *>(csm_out));
+                                   reinterpret_cast<float2 *>(csm_out), nsplit, F, part);
         }
+        if (nsplit > 1)
+            hipLaunchKernelGGL(welch_reduce_kernel, dim3((unsigned)cdiv(M, 256), (unsigned)M, (unsigned)F), dim3(256), 0,
+                               s, part, nsplit, F, M, (int)(b0 > 0), reinterpret_cast<float2 *>(csm_out));
     }
     hipLaunchKernelGGL(welch_finalize_kernel, dim3((unsigned)(cdiv(M, 32) * cdiv(M, 32)), F), dim3(256), 0, s,
                        reinterpret_cast<float2 *>(csm_out), M, (float)scale);

@@ -163,7 +163,8 @@ __global__ __launch_bounds__(512) void welch_fft_kernel(WelchFftArgs a)
 // per SIMD.
 template <int TR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) void welch_herk_kernel(
-    const float2 *__restrict__ spec, int nb, int M, int nTc, int f0, int accumulate, float2 *__restrict__ csm)
+    const float2 *__restrict__ spec, int nb, int M, int nTc, int f0, int accumulate, float2 *__restrict__ csm,
+    int nsplit, int F, float2 *__restrict__ part)
 {
     const int tiles = (M + 16 * TR - 1) / (16 * TR) * nTc;
     const int64_t L = xcd_order(blockIdx.x, (int64_t)gridDim.x);
@@ -171,7 +172,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
     const int I = tile / nTc, J = tile % nTc;
     const int r0 = I * 16 * TR, c0 = J * 64;
     if (c0 + 63 < r0) return; // entirely below the diagonal
-    const int f = f0 + (int)(L / tiles);
+    const int64_t rest = L / tiles;
+    const int sp = (int)(rest % nsplit); // slice of the pass's blocks
+    const int f = f0 + (int)(rest / nsplit);
+    // blocks [kb0, kb1) of this slice, slices 4-block aligned
+    const int chunk = ((nb + nsplit - 1) / nsplit + 3) & ~3;
+    const int kb0 = min(nb, sp * chunk), kb1 = min(nb, kb0 + chunk);
     const int l = threadIdx.x;
     const float2 *X = spec + (int64_t)f * nb * M;
     f4 accr[TR][4], acci[TR][4];
@@ -209,7 +215,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
             for (int t = 0; t < 4; ++t) ca[t] = row[min(c0 + t * 16 + (l & 15), M - 1)];
         };
         auto mask = [&](int k0, float2 (&ra)[TR], float2 (&ca)[4]) __attribute__((always_inline)) {
-            const bool bv = k0 + (l >> 4) < nb;
+            const bool bv = k0 + (l >> 4) < kb1;
 #pragma unroll
             for (int t = 0; t < TR; ++t)
                 if (!(bv && r0 + t * 16 + (l & 15) < M)) ra[t] = make_float2(0.f, 0.f);
@@ -218,9 +224,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
                 if (!(bv && c0 + t * 16 + (l & 15) < M)) ca[t] = make_float2(0.f, 0.f);
         };
         float2 ra[TR], ca[4];
-        load(0, ra, ca);
-        mask(0, ra, ca);
-        for (int k0 = 0; k0 < nb; k0 += 4) {
+        load(kb0, ra, ca);
+        mask(kb0, ra, ca);
+        for (int k0 = kb0; k0 < kb1; k0 += 4) {
             float2 na[TR], nc[4];
             load(k0 + 4, na, nc);
             __builtin_amdgcn_sched_barrier(0);
@@ -234,9 +240,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
         }
     } else {
         // 32 x 64 tiles (five waves per SIMD hide each other's loads)
-        for (int k0 = 0; k0 < nb; k0 += 4) {
+        for (int k0 = kb0; k0 < kb1; k0 += 4) {
             const int b = k0 + (l >> 4);
-            const bool bv = b < nb;
+            const bool bv = b < kb1;
             float2 ra[TR], ca[4];
 #pragma unroll
             for (int t = 0; t < TR; ++t) {
@@ -251,7 +257,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
             step(ra, ca);
         }
     }
-    float2 *C = csm + (int64_t)f * M * M;
+    // a split pass stores its slices' partial sums (welch_reduce_kernel adds them)
+    float2 *C = nsplit > 1 ? part + ((int64_t)sp * F + f) * M * M : csm + (int64_t)f * M * M;
+    const bool add = nsplit == 1 && accumulate;
 #pragma unroll
     for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
@@ -262,7 +270,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
                 const int c = c0 + tj * 16 + (l & 15);
                 if (r < M && c < M && r <= c) {
                     // the first pass stores, later passes add
-                    float2 v = accumulate ? C[(int64_t)r * M + c] : make_float2(0.f, 0.f);
+                    float2 v = add ? C[(int64_t)r * M + c] : make_float2(0.f, 0.f);
                     v.x += accr[ti][tj][i];
                     v.y += acci[ti][tj][i];
                     C[(int64_t)r * M + c] = v;
@@ -270,8 +278,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8 / TR))) vo
             }
 }
 
-template __global__ void welch_herk_kernel<4>(const float2 *, int, int, int, int, int, float2 *);
-template __global__ void welch_herk_kernel<2>(const float2 *, int, int, int, int, int, float2 *);
+template __global__ void welch_herk_kernel<4>(const float2 *, int, int, int, int, int, float2 *, int, int, float2 *);
+template __global__ void welch_herk_kernel<2>(const float2 *, int, int, int, int, int, float2 *, int, int, float2 *);
+
+// Sum of the nsplit slices' partial upper triangles of a split pass, slice by slice in order (the
+// same result on every run), stored into the CSM (first pass) or added to it.  grid: (ceil(M /
+// 256) column blocks, M rows, F).
+__global__ __launch_bounds__(256) void welch_reduce_kernel(const float2 *__restrict__ part, int nsplit, int F, int M,
+                                                           int accumulate, float2 *__restrict__ csm)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y, f = blockIdx.z;
+    if (c >= M || r > c) return;
+    const int64_t e = ((int64_t)f * M + r) * M + c, stride = (int64_t)F * M * M;
+    float2 v = accumulate ? csm[e] : make_float2(0.f, 0.f);
+    for (int s = 0; s < nsplit; ++s) {
+        const float2 p = part[s * stride + e];
+        v.x += p.x;
+        v.y += p.y;
+    }
+    csm[e] = v;
+}
 
 // Hermitian completion (spectra.py:412-414) and scaling (spectra.py:416) by 32 x 32 tiles: tile
 // (I, J) on or above the diagonal is read once, scaled and staged in LDS, then written back and

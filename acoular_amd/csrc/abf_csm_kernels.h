@@ -23,7 +23,9 @@ __global__ void calc_csm_c64_kernel(float2 *csm, const float2 *spec, int f0, int
 __global__ void calc_csm_c128_kernel(double2 *csm, const double2 *spec, int f0, int M);
 __global__ void welch_fft_kernel(WelchFftArgs a);
 template <int TR>
-__global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nTc, int f0, int accumulate, float2 *csm);
+__global__ void welch_herk_kernel(const float2 *spec, int nb, int M, int nTc, int f0, int accumulate, float2 *csm,
+                                  int nsplit, int F, float2 *part);
+__global__ void welch_reduce_kernel(const float2 *part, int nsplit, int F, int M, int accumulate, float2 *csm);
 __global__ void welch_finalize_kernel(float2 *csm, int M, float scale);
 __global__ void irfft_kernel(const float2 *spec, int M, int bs, int logbs, int b0, int P, const float2 *twiddle,
                              float *out);
