@@ -146,3 +146,31 @@ def test_fastfuncs_type_checks():
         aa.calcCSM(np.zeros((2, 3, 3), dtype=np.complex128), np.zeros((2, 3), dtype=np.complex64))
     with pytest.raises(KeyError):
         aa.beamformerFreq('nonsense', True, 1.0, (np.zeros(2), np.zeros((2, 3)), 1.0), np.zeros((3, 3), complex))
+
+
+def test_welch_workspace_holds_the_split_partials_only_for_small_arrays():
+    """The CSM update of a small array (few tiles per bin) is split over up to 16 slices of a
+    pass's blocks whose partial upper triangles the workspace holds; a large array is not split.
+    Host arithmetic, no device."""
+    lib = _lib.lib()
+
+    def base(m, bs, bpp):  # twiddles + the spectra of one pass, 256-byte aligned
+        def a(n):
+            return (n + 255) // 256 * 256
+
+        return a(bs // 2 * 8) + a((bs // 2 + 1) * bpp * m * 8)
+
+    # 64 mics, 65 bins, 799 blocks per pass: 65 waves of 64 x 64 tiles -> 16 slices
+    got = lib.abf_welch_csm_workspace_size(64, 128, 799)
+    assert got == base(64, 128, 799) + (16 * 65 * 64 * 64 * 8 + 255) // 256 * 256
+    # slices keep at least 32 blocks: 100 blocks per pass allow 2
+    assert lib.abf_welch_csm_workspace_size(64, 128, 100) == base(64, 128, 100) + (2 * 65 * 64 * 64 * 8 + 255) // 256 * 256
+    # 512 mics x 1025 bins fill the GPU with tiles alone: no partials
+    assert lib.abf_welch_csm_workspace_size(512, 2048, 125) == base(512, 2048, 125)
+
+
+def test_calc_csm_rejects_more_mics_than_a_grid_row_holds():
+    """calcCSM takes its rows from the grid's second dimension (at most 65535); larger arrays are
+    refused before any device work."""
+    rc = _lib.lib().abf_calc_csm(ctypes.c_void_p(8), ctypes.c_void_p(8), 1, 70000, 0, None)
+    assert rc == _lib.ABF_ERR_UNSUPPORTED
