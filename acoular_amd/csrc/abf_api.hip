@@ -835,13 +835,15 @@ int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mic
     if (num_mics > 65535) return fail(ABF_ERR_UNSUPPORTED, "more than 65535 microphones");
     const int M = num_mics;
     for (int f0 = 0; f0 < num_freqs; f0 += 65535) {
-        const dim3 grid((unsigned)cdiv(M, 256), (unsigned)M, (unsigned)std::min(65535, num_freqs - f0));
+        const unsigned nf = (unsigned)std::min(65535, num_freqs - f0);
         if (is_c128)
-            hipLaunchKernelGGL(calc_csm_c128_kernel, grid, dim3(256), 0, S(stream), reinterpret_cast<double2 *>(csm),
-                               reinterpret_cast<const double2 *>(spec), f0, M);
+            hipLaunchKernelGGL(calc_csm_c128_kernel, dim3((unsigned)cdiv(M, 256), (unsigned)M, nf), dim3(256), 0,
+                               S(stream), reinterpret_cast<double2 *>(csm), reinterpret_cast<const double2 *>(spec), f0,
+                               M);
         else
-            hipLaunchKernelGGL(calc_csm_c64_kernel, grid, dim3(256), 0, S(stream), reinterpret_cast<float2 *>(csm),
-                               reinterpret_cast<const float2 *>(spec), f0, M);
+            hipLaunchKernelGGL(calc_csm_c64_kernel, dim3((unsigned)cdiv(M, 512), (unsigned)M, nf), dim3(256), 0,
+                               S(stream), reinterpret_cast<float2 *>(csm), reinterpret_cast<const float2 *>(spec), f0,
+                               M);
     }
     return hip_status("calc_csm launch");
 }

@@ -23,21 +23,38 @@ __global__ void fill_zero_kernel(float4 *__restrict__ p, int64_t n4)
     if (i < n4) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// calcCSM update, csm[f][r][c] += conj(spec[f][c]) spec[f][r] for r <= c.  grid: (ceil(M / 256)
-// column blocks, M rows, frequencies f0 ... of this launch): the indices come from the grid, with
-// no 64-bit divisions per element; column blocks left of the diagonal exit at once.
+// calcCSM update, csm[f][r][c] += conj(spec[f][c]) spec[f][r] for r <= c.  grid: (ceil(M / 512)
+// column blocks, M rows, frequencies f0 ... of this launch): two adjacent columns per thread, as
+// one 16-byte access when M is even (every row and spectrum then starts on an even element);
+// indices come from the grid, with no 64-bit divisions per element; column pairs left of the
+// diagonal exit at once.
 __global__ __launch_bounds__(256) void calc_csm_c64_kernel(float2 *__restrict__ csm, const float2 *__restrict__ spec,
                                                            int f0, int M)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
+    const int c = (blockIdx.x * 256 + threadIdx.x) * 2, r = blockIdx.y;
     const int64_t f = (int64_t)f0 + blockIdx.z;
-    if (c >= M || r > c) return;
-    const float2 sr = spec[f * M + r], sc = spec[f * M + c];
-    float2 *p = csm + (f * M + r) * M + c;
-    float2 v = *p;
-    v.x += sc.x * sr.x + sc.y * sr.y;
-    v.y += sc.x * sr.y - sc.y * sr.x;
-    *p = v;
+    if (c >= M || c + 1 < r) return;
+    const float2 sr = spec[f * M + r];
+    float2 *row = csm + (f * M + r) * M;
+    const float2 *sp = spec + f * M;
+    if (!(M & 1) && c >= r) { // both columns on or above the diagonal, c + 1 < M
+        float4 v = *reinterpret_cast<const float4 *>(row + c);
+        const float4 sc = *reinterpret_cast<const float4 *>(sp + c);
+        v.x += sc.x * sr.x + sc.y * sr.y;
+        v.y += sc.x * sr.y - sc.y * sr.x;
+        v.z += sc.z * sr.x + sc.w * sr.y;
+        v.w += sc.z * sr.y - sc.w * sr.x;
+        *reinterpret_cast<float4 *>(row + c) = v;
+        return;
+    }
+    for (int cc = c; cc < c + 2 && cc < M; ++cc) {
+        if (cc < r) continue;
+        const float2 sc = sp[cc];
+        float2 v = row[cc];
+        v.x += sc.x * sr.x + sc.y * sr.y;
+        v.y += sc.x * sr.y - sc.y * sr.x;
+        row[cc] = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void calc_csm_c128_kernel(double2 *__restrict__ csm,

@@ -197,10 +197,13 @@ def test_calc_transfer_and_steer_vector(gpu):
 
 
 @pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
-def test_calc_csm_dropin(gpu, dtype):
+@pytest.mark.parametrize('nm', [13, 16, 70, 530])
+def test_calc_csm_dropin(gpu, dtype, nm):
+    """Odd and even mic counts (two columns per 16-byte access for an even count), a last column
+    block that is partly empty; the lower triangle stays as it was."""
     rng = np.random.default_rng(2)
-    spec = (rng.standard_normal((9, 13)) + 1j * rng.standard_normal((9, 13))).astype(dtype)
-    csm = (rng.standard_normal((9, 13, 13)) + 1j * rng.standard_normal((9, 13, 13))).astype(dtype)
+    spec = (rng.standard_normal((9, nm)) + 1j * rng.standard_normal((9, nm))).astype(dtype)
+    csm = (rng.standard_normal((9, nm, nm)) + 1j * rng.standard_normal((9, nm, nm))).astype(dtype)
     want = S.calc_csm_update(csm.astype(np.complex128).copy(), spec.astype(np.complex128))
     got = aa.calcCSM(csm, spec)
     assert got is csm
