@@ -801,7 +801,7 @@ int abf_dist_mat(const double *gpos, const double *mpos, int64_t num_grid, int32
     if (num_mics < 1 || num_grid < 0) return fail(ABF_ERR_INVALID, "bad sizes");
     if (num_grid == 0) return ABF_OK;
     if (!gpos || !mpos || !rm) return fail(ABF_ERR_INVALID, "null pointer");
-    const int64_t nmb = cdiv((int64_t)num_mics, 256);
+    const int64_t nmb = cdiv((int64_t)num_mics, 512);
     const int64_t per = std::max<int64_t>(1, (int64_t(1) << 22) / nmb); // points per launch
     for (int64_t g0 = 0; g0 < num_grid; g0 += per)
         hipLaunchKernelGGL(dist_mat_kernel, dim3((unsigned)(std::min(per, num_grid - g0) * nmb)), dim3(256), 0,

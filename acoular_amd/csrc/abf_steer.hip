@@ -5,16 +5,27 @@
 
 namespace abf {
 
-// grid: (points g0 ... of this launch) x ceil(M / 256) workgroups; workgroup (g, mb) writes
-// rm[g][256 mb ...] (one division per workgroup, none per element)
+// grid: (points g0 ... of this launch) x ceil(M / 512) workgroups; workgroup (g, mb) writes
+// rm[g][512 mb ...], two microphones per thread (one 16-byte store when M is even, every row then
+// starting on an even element); one division per workgroup, none per element
 __global__ __launch_bounds__(256) void dist_mat_kernel(const double *__restrict__ gpos, const double *__restrict__ mpos,
                                                        int64_t G, int M, int64_t g0, double *__restrict__ rm)
 {
-    const int nmb = (M + 255) / 256;
+    const int nmb = (M + 511) / 512;
     const int64_t g = g0 + blockIdx.x / nmb;
-    const int m = (int)(blockIdx.x % nmb) * 256 + threadIdx.x;
+    const int m = (int)(blockIdx.x % nmb) * 512 + 2 * threadIdx.x;
     if (m >= M) return;
-    rm[g * M + m] = dist3(gpos[g], gpos[G + g], gpos[2 * G + g], mpos[m], mpos[M + m], mpos[2 * M + m]);
+    const double gx = gpos[g], gy = gpos[G + g], gz = gpos[2 * G + g];
+    const double d0 = dist3(gx, gy, gz, mpos[m], mpos[M + m], mpos[2 * M + m]);
+    if (m + 1 < M) {
+        const double d1 = dist3(gx, gy, gz, mpos[m + 1], mpos[M + m + 1], mpos[2 * M + m + 1]);
+        if (!(M & 1)) {
+            *reinterpret_cast<double2 *>(rm + g * M + m) = make_double2(d0, d1);
+            return;
+        }
+        rm[g * M + m + 1] = d1;
+    }
+    rm[g * M + m] = d0;
 }
 
 // One wave per grid point, the lanes over the microphones (rows of rm read and of out written
