@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, 'libabf.so')
+# ACOULAR_AMD_LIB names another build of the same library (the sanitizer build of the host code
+# used by the CPU tests); by default the in-tree libabf.so
+LIB_PATH = os.environ.get('ACOULAR_AMD_LIB') or os.path.join(_HERE, 'libabf.so')
 
 ABF_OK = 0
 ABF_ERR_INVALID = -1

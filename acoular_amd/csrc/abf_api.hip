@@ -2,6 +2,7 @@
 // validation and launches; the kernels live in abf_bf.hip, abf_steer.hip and abf_csm.hip.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,7 @@ int hip_status(const char *what)
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 constexpr size_t kLdsMax = 163840; // gfx950 LDS per workgroup
@@ -230,23 +232,21 @@ __global__ void twiddle_kernel(float2 *tw, int bs)
 template <int B, int CW, int V, int TT, bool PF>
 void launch_mfma(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)bf_mfma_kernel<B, CW, V, TT, PF>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
-        attr = true;
-    }
+    });
     hipLaunchKernelGGL((bf_mfma_kernel<B, CW, V, TT, PF>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
 }
 
 template <int B, int V> void launch_mfma_ws(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)bf_mfma_ws_kernel<B, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)kLdsMax);
-        attr = true;
-    }
+    });
     hipLaunchKernelGGL((bf_mfma_ws_kernel<B, V>), dim3((unsigned)wgs), dim3(512), lds, s, dev, n);
 }
 
@@ -330,11 +330,10 @@ int run_plain_only(const abf_grid_job *j, void *ws, hipStream_t s)
     a.phase_mode = j->phase_mode & kPhaseMask;
     a.fast_trig = (j->phase_mode & kFastTrigFlag) != 0;
     a.clip = j->clip_negative;
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)bf_plain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-        attr = true;
-    }
+    });
     const size_t lds = (size_t)std::min(256, Mp) * 64 * sizeof(float2);
     hipLaunchKernelGGL(bf_plain_kernel, dim3((unsigned)cdiv(G, 64), F), dim3(256), lds, s, a);
     return hip_status("plain beamformer launch");
@@ -590,11 +589,10 @@ int abf_beamform_table(const void *csm, const double *r0, const double *rm, int6
     a.phase_mode = phase_mode & kPhaseMask;
     a.fast_trig = (phase_mode & kFastTrigFlag) != 0;
     a.clip = 0;
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)bf_plain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-        attr = true;
-    }
+    });
     const size_t lds = (size_t)std::min(256, Mp) * 64 * sizeof(float2);
     hipLaunchKernelGGL(bf_plain_kernel, dim3((unsigned)cdiv(num_grid, 64), 1), dim3(256), lds, s, a);
     return hip_status("table beamformer launch");
@@ -698,11 +696,10 @@ int abf_beamform_f64(const abf_geometry *geom, const void *csm, const double *wa
     a.steer_type = steer_type;
     a.phase_mode = phase_mode & kPhaseMask;
     a.clip = clip_negative;
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)bf_plain64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-        attr = true;
-    }
+    });
     const size_t lds = (size_t)std::min(128, Mp) * 64 * sizeof(double2);
     hipLaunchKernelGGL(bf_plain64_kernel, dim3((unsigned)cdiv(G, 64), num_freqs), dim3(256), lds, s, a);
     return hip_status("float64 beamformer launch");
@@ -740,12 +737,11 @@ int abf_beamform_eig_grid(const abf_geometry *geom, const double *wavenumber, in
     }
     const size_t lds = (size_t)(mfma ? Mp4 : M) * P * sizeof(float2);
     if (lds > 147456) return fail(ABF_ERR_UNSUPPORTED, "too many microphones for the eigenvalue form");
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)bf_eig_grid_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 147456);
         (void)hipFuncSetAttribute((const void *)bf_eig_mfma_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 147456);
-        attr = true;
-    }
+    });
     EigGridArgs a{};
     a.gpos = geom->gpos;
     a.mpos = geom->mpos;
@@ -803,9 +799,11 @@ int abf_dist_mat(const double *gpos, const double *mpos, int64_t num_grid, int32
     if (!gpos || !mpos || !rm) return fail(ABF_ERR_INVALID, "null pointer");
     const int64_t nmb = cdiv((int64_t)num_mics, 512);
     const int64_t per = std::max<int64_t>(1, (int64_t(1) << 22) / nmb); // points per launch
+    // two distances per 16-byte store when every row starts 16-byte aligned
+    const int vec2 = !(num_mics & 1) && aligned16(rm);
     for (int64_t g0 = 0; g0 < num_grid; g0 += per)
         hipLaunchKernelGGL(dist_mat_kernel, dim3((unsigned)(std::min(per, num_grid - g0) * nmb)), dim3(256), 0,
-                           S(stream), gpos, mpos, num_grid, (int)num_mics, g0, rm);
+                           S(stream), gpos, mpos, num_grid, (int)num_mics, g0, vec2, rm);
     return hip_status("dist_mat launch");
 }
 
@@ -834,6 +832,8 @@ int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mic
     if (!csm || !spec) return fail(ABF_ERR_INVALID, "null pointer");
     if (num_mics > 65535) return fail(ABF_ERR_UNSUPPORTED, "more than 65535 microphones");
     const int M = num_mics;
+    // complex64: two columns per 16-byte access when every row starts 16-byte aligned
+    const int vec2 = !(M & 1) && aligned16(csm) && aligned16(spec);
     for (int f0 = 0; f0 < num_freqs; f0 += 65535) {
         const unsigned nf = (unsigned)std::min(65535, num_freqs - f0);
         if (is_c128)
@@ -843,7 +843,7 @@ int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mic
         else
             hipLaunchKernelGGL(calc_csm_c64_kernel, dim3((unsigned)cdiv(M, 512), (unsigned)M, nf), dim3(256), 0,
                                S(stream), reinterpret_cast<float2 *>(csm), reinterpret_cast<const float2 *>(spec), f0,
-                               M);
+                               M, vec2);
     }
     return hip_status("calc_csm launch");
 }
@@ -906,11 +906,10 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
                                reinterpret_cast<float2 *>(csm_out) + (total - 1));
     }
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)welch_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-        attr = true;
-    }
+    });
     // channel pairs per workgroup: at most 64 KB of LDS (two workgroups per CU; measured 2.36 vs 2.97 ms
     // for 128 KB on 499 blocks x 512 channels of 2048 samples); 16384-sample blocks need 128 KB
     const int P = std::max(1, std::min({env_int("ABF_FFT_P", 16), 8192 / bs, (M + 1) / 2}));
@@ -988,11 +987,10 @@ int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, in
     while ((1 << logbs) < bs) ++logbs;
     float2 *tw = reinterpret_cast<float2 *>(workspace);
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)welch_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-        attr = true;
-    }
+    });
     // channel pairs per workgroup: at most 64 KB of LDS (two workgroups per CU; measured 2.36 vs 2.97 ms
     // for 128 KB on 499 blocks x 512 channels of 2048 samples); 16384-sample blocks need 128 KB
     const int P = std::max(1, std::min({env_int("ABF_FFT_P", 16), 8192 / bs, (M + 1) / 2}));
@@ -1038,11 +1036,10 @@ int abf_irfft_blocks(const void *spec, int32_t num_blocks, int32_t num_mics, int
     while ((1 << logbs) < bs) ++logbs;
     float2 *tw = reinterpret_cast<float2 *>(workspace);
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)irfft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-        attr = true;
-    }
+    });
     const int P = std::max(1, std::min({16, 8192 / bs, (M + 1) / 2}));
     const int ng = (int)cdiv(M, 2 * P);
     const int per_launch = (int)std::min<int64_t>(65535, std::max<int64_t>(1, kFftMaxWorkgroups / ng));

@@ -29,7 +29,7 @@ __global__ void fill_zero_kernel(float4 *__restrict__ p, int64_t n4)
 // indices come from the grid, with no 64-bit divisions per element; column pairs left of the
 // diagonal exit at once.
 __global__ __launch_bounds__(256) void calc_csm_c64_kernel(float2 *__restrict__ csm, const float2 *__restrict__ spec,
-                                                           int f0, int M)
+                                                           int f0, int M, int vec2)
 {
     const int c = (blockIdx.x * 256 + threadIdx.x) * 2, r = blockIdx.y;
     const int64_t f = (int64_t)f0 + blockIdx.z;
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void calc_csm_c64_kernel(float2 *__restrict__ 
     const float2 sr = spec[f * M + r];
     float2 *row = csm + (f * M + r) * M;
     const float2 *sp = spec + f * M;
-    if (!(M & 1) && c >= r) { // both columns on or above the diagonal, c + 1 < M
+    if (vec2 && c >= r) { // both columns on or above the diagonal, c + 1 < M
         float4 v = *reinterpret_cast<const float4 *>(row + c);
         const float4 sc = *reinterpret_cast<const float4 *>(sp + c);
         v.x += sc.x * sr.x + sc.y * sr.y;

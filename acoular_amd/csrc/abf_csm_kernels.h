@@ -19,7 +19,7 @@ struct WelchFftArgs {
 constexpr int64_t kFftMaxWorkgroups = int64_t(1) << 22;
 
 __global__ void fill_zero_kernel(float4 *p, int64_t n4);
-__global__ void calc_csm_c64_kernel(float2 *csm, const float2 *spec, int f0, int M);
+__global__ void calc_csm_c64_kernel(float2 *csm, const float2 *spec, int f0, int M, int vec2);
 __global__ void calc_csm_c128_kernel(double2 *csm, const double2 *spec, int f0, int M);
 __global__ void welch_fft_kernel(WelchFftArgs a);
 template <int TR>

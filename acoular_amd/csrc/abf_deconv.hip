@@ -19,6 +19,7 @@
 // is updated, and applies the relaxation and the positivity clip.  The update order is the
 // reference's; only the float64 summation order differs.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <stdint.h>
 
 #include <cmath>
@@ -400,14 +401,13 @@ int abf_damas(const void *psf, int32_t psf_f64, const double *dirty, double *x, 
     if (!psf || !dirty || !x) return set_error(ABF_ERR_INVALID, "null pointer");
     DamasArgs d{psf, dirty, x, num_grid, n_iter, psf_f64 != 0, relax};
     const size_t lds = (size_t)num_grid * sizeof(double);
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)damas_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(kDamasMaxG * sizeof(double)));
         (void)hipFuncSetAttribute((const void *)damas_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(kDamasMaxG * sizeof(double)));
-        attr = true;
-    }
+    });
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (psf_f64)
         hipLaunchKernelGGL(damas_kernel<true>, dim3((unsigned)num_freqs), dim3(1024), lds, s, d);

@@ -9,7 +9,8 @@ namespace abf {
 // rm[g][512 mb ...], two microphones per thread (one 16-byte store when M is even, every row then
 // starting on an even element); one division per workgroup, none per element
 __global__ __launch_bounds__(256) void dist_mat_kernel(const double *__restrict__ gpos, const double *__restrict__ mpos,
-                                                       int64_t G, int M, int64_t g0, double *__restrict__ rm)
+                                                       int64_t G, int M, int64_t g0, int vec2,
+                                                       double *__restrict__ rm)
 {
     const int nmb = (M + 511) / 512;
     const int64_t g = g0 + blockIdx.x / nmb;
@@ -19,7 +20,7 @@ __global__ __launch_bounds__(256) void dist_mat_kernel(const double *__restrict_
     const double d0 = dist3(gx, gy, gz, mpos[m], mpos[M + m], mpos[2 * M + m]);
     if (m + 1 < M) {
         const double d1 = dist3(gx, gy, gz, mpos[m + 1], mpos[M + m + 1], mpos[2 * M + m + 1]);
-        if (!(M & 1)) {
+        if (vec2) {
             *reinterpret_cast<double2 *>(rm + g * M + m) = make_double2(d0, d1);
             return;
         }

@@ -65,13 +65,18 @@ def analytic_csm(mpos, spos, powers, freqs, c: float = 343.0, dtype=np.complex64
         return out
     import torch
 
+    def up(x):
+        # staged in pinned memory and copied without blocking: the host does not wait for the
+        # work already queued on the stream (a pageable copy would)
+        return torch.from_numpy(np.ascontiguousarray(x)).pin_memory().to(device, non_blocking=True)
+
     tdt = {np.complex64: torch.complex64, np.complex128: torch.complex128}[np.dtype(dtype).type]
-    rsm_t = torch.from_numpy(rsm).to(device)
-    rs0_t = torch.from_numpy(rs0).to(device)
-    amp = (rs0_t[:, None] / rsm_t) * torch.from_numpy(np.sqrt(p)).to(device)[:, None]  # [S, M]
+    rsm_t = up(rsm)
+    rs0_t = up(rs0)
+    amp = (rs0_t[:, None] / rsm_t) * up(np.sqrt(p))[:, None]  # [S, M]
     dr = rsm_t - rs0_t[:, None]
     out = torch.empty((len(freqs), mpos.shape[1], mpos.shape[1]), dtype=tdt, device=device)
-    kk = torch.from_numpy(2 * np.pi * freqs / c).to(device)
+    kk = up(2 * np.pi * freqs / c)
     chunk = 64
     for i0 in range(0, len(freqs), chunk):
         k = kk[i0 : i0 + chunk]

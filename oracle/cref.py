@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, 'libcpu_ref.so')
+# ORACLE_CPU_REF_LIB names another build of cpu_ref.c (the sanitizer build used by the CPU tests)
+_LIB = os.environ.get('ORACLE_CPU_REF_LIB') or os.path.join(_HERE, 'libcpu_ref.so')
 _lib = None
 
 

@@ -105,6 +105,23 @@ def welch_csm(data: np.ndarray, block_size: int, window: str = 'Hanning', overla
     return (csm * (2.0 / block_size / weight / nb)).astype(precision)
 
 
+def welch_csm_bins(data: np.ndarray, block_size: int, bins, window: str = 'Hanning',
+                   overlap: str = '50%') -> np.ndarray:
+    """The Welch CSM of a few bins only (``spectra.py:366-416``), by a direct float64 DFT of every
+    block at those bins: ``[len(bins), M, M]`` complex128, Hermitian, the same blocks, window and
+    ``num_blocks`` scaling as :func:`welch_csm` (which it equals on every bin; pinned in the tests),
+    for arrays too large to transform whole."""
+    data = np.asarray(data)
+    n, m = data.shape
+    wind = WINDOWS[window](block_size)
+    e = np.exp(-2j * np.pi * np.outer(np.asarray(bins), np.arange(block_size)) / block_size)  # [nb, bs]
+    csm = np.zeros((len(bins), m, m), dtype=np.complex128)
+    for st in block_starts(n, block_size, overlap):
+        x = e @ (data[st:st + block_size].astype(np.float64) * wind[:, None])  # [nb, M]
+        csm += x[:, :, None] * np.conj(x[:, None, :])
+    return csm * (2.0 / block_size / float(np.dot(wind, wind)) / num_blocks(n, block_size, overlap))
+
+
 def synthetic_ev(eigvals: np.ndarray, freqs: np.ndarray, freq: float, num: int = 0) -> np.ndarray:
     """``PowerSpectra.synthetic_ev`` (``spectra.py:528-588``)."""
     if num == 0:

@@ -192,8 +192,18 @@ def test_calc_transfer_and_steer_vector(gpu):
         h = sv.steer_vector(12000.0)
         want = B.steer_vector(B.calc_transfer(sv.r0, sv.rm, 2 * np.pi * 12000.0 / 343.0), st)
         np.testing.assert_allclose(h, want, rtol=0, atol=2e-6 * np.abs(want).max())
+        # a subset of grid points: every row is normalised on its own (fbeamform.py:145-153), so
+        # the rows of the subset equal those of the full matrix for all four steer types
         h1 = sv.steer_vector(12000.0, ind=np.array([3, 5]))
-        np.testing.assert_allclose(h1, want[[3, 5]] if st in ('classic', 'inverse') else h1, atol=2e-6)
+        assert h1.shape == (2, 12)
+        np.testing.assert_allclose(h1, want[[3, 5]], rtol=0, atol=2e-6 * np.abs(want).max())
+        # a scalar index takes the rm[ind, :][np.newaxis] branch (fbeamform.py:229-230)
+        h2 = sv.steer_vector(12000.0, ind=4)
+        assert h2.shape == (1, 12)
+        np.testing.assert_allclose(h2, want[[4]], rtol=0, atol=2e-6 * np.abs(want).max())
+        t2 = sv.transfer(12000.0, ind=4)
+        np.testing.assert_allclose(t2, B.calc_transfer(sv.r0, sv.rm, 2 * np.pi * 12000.0 / 343.0)[[4]], rtol=0,
+                                   atol=2e-6 * np.abs(t2).max())
 
 
 @pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
@@ -211,9 +221,40 @@ def test_calc_csm_dropin(gpu, dtype, nm):
     np.testing.assert_allclose(got, want, rtol=0, atol=tol * np.abs(want).max())
 
 
-def test_dist_mat(gpu):
+def test_even_counts_on_views_that_start_off_16_bytes(gpu):
+    """calcCSM (complex64) and dist_mat with an even count on caller views that start 8 bytes
+    past a 16-byte boundary: the library takes the one-column path and the results stay exact."""
+    torch = _lib.torch()
+    lib = _lib.lib()
+    rng = np.random.default_rng(12)
+    nf, nm = 5, 34
+    spec = (rng.standard_normal((nf, nm)) + 1j * rng.standard_normal((nf, nm))).astype(np.complex64)
+    csm = (rng.standard_normal((nf, nm, nm)) + 1j * rng.standard_normal((nf, nm, nm))).astype(np.complex64)
+    want = S.calc_csm_update(csm.astype(np.complex128).copy(), spec.astype(np.complex128))
+    buf = torch.zeros(nf * nm * nm + 1, dtype=torch.complex64, device=gpu)
+    sbuf = torch.zeros(nf * nm + 1, dtype=torch.complex64, device=gpu)
+    dcsm, dspec = buf[1:], sbuf[1:]
+    assert dcsm.data_ptr() % 16 == 8 and dspec.data_ptr() % 16 == 8
+    dcsm.copy_(torch.from_numpy(csm.ravel()))
+    dspec.copy_(torch.from_numpy(spec.ravel()))
+    _lib.check(lib.abf_calc_csm(_lib.ptr(dcsm), _lib.ptr(dspec), nf, nm, 0, _lib.stream_handle()))
+    got = dcsm.cpu().numpy().reshape(nf, nm, nm)
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-6 * np.abs(want).max())
+    g, m = rng.standard_normal((3, 77)), rng.standard_normal((3, 34))
+    rbuf = torch.zeros(77 * 34 + 1, dtype=torch.float64, device=gpu)
+    drm = rbuf[1:]
+    assert drm.data_ptr() % 16 == 8
+    dg, dm = _lib.to_device(g), _lib.to_device(m)
+    _lib.check(lib.abf_dist_mat(_lib.ptr(dg), _lib.ptr(dm), 77, 34, _lib.ptr(drm), _lib.stream_handle()))
+    np.testing.assert_allclose(drm.cpu().numpy().reshape(77, 34), B.dist_mat(g, m), rtol=1e-15)
+
+
+@pytest.mark.parametrize('nm', [17, 16, 530, 1024])
+def test_dist_mat(gpu, nm):
+    """Odd and even microphone counts (an even count takes the two-distances-per-store path),
+    and counts above 512 (more than one microphone block per grid point)."""
     rng = np.random.default_rng(1)
-    g, m = rng.standard_normal((3, 301)), rng.standard_normal((3, 17))
+    g, m = rng.standard_normal((3, 301)), rng.standard_normal((3, nm))
     np.testing.assert_allclose(aa.dist_mat(g, m), B.dist_mat(g, m), rtol=1e-15)
     np.testing.assert_allclose(aa.Environment().apparent_r(g), B.apparent_r(g), rtol=1e-15)
 

@@ -27,6 +27,17 @@ def test_synthetic_ev_known_answer():
     assert csm.shape == (65, 4, 4)  # spectra.py:391-394
 
 
+@pytest.mark.parametrize('window,overlap', [('Hanning', '50%'), ('Hamming', 'None'), ('Blackman', '75%')])
+def test_welch_of_a_few_bins_equals_full_welch(window, overlap):
+    """The direct-DFT Welch of selected bins equals the FFT-based Welch on those bins (another
+    transform, the same blocks, window and scaling)."""
+    data = np.random.default_rng(4).standard_normal((3000, 5))
+    full = S.welch_csm(data, 256, window, overlap)
+    bins = [0, 7, 64, 128]
+    part = S.welch_csm_bins(data, 256, bins, window, overlap)
+    np.testing.assert_allclose(part, full[bins], rtol=0, atol=1e-12 * np.abs(full).max())
+
+
 def test_num_blocks_is_float():
     assert S.num_blocks(1000, 128, 'None') == 7.8125
     assert len(S.block_starts(1000, 128, 'None')) == 7
@@ -224,6 +235,37 @@ def test_eig_form_equals_full_csm(steer, r_diag):
     eig = B.freq_beamformer_eig(lam, V, r0, rm, 40.0, 1.1, r_diag, steer)
     np.testing.assert_allclose(eig[0], full[0], rtol=1e-9, atol=1e-12 * np.abs(full[0]).max())
     np.testing.assert_allclose(eig[1], full[1], rtol=1e-14)
+
+
+@pytest.mark.parametrize('r_diag', [False, True])
+def test_eig_custom_equals_explicit_quadratic_form(r_diag):
+    """The eigenvalue form with explicit steering vectors (fastFuncs.py:479-559) over all
+    eigenpairs equals h^H C h summed by brute force over every (i, j) of the full Hermitian CSM;
+    with r_diag its correction sum_k l_k sum_m |v_mk|^2 |h_m|^2 equals removing the diagonal of
+    C explicitly.  Over a subset of eigenpairs it equals h^H C_K h for the matrix rebuilt from
+    that subset.  A conjugated eigenvector product (v_k^T h instead of v_k^H h) or a missing
+    eigenvalue weight fails these."""
+    rng = np.random.default_rng(21)
+    M, G = 7, 5
+    X = rng.standard_normal((M, 3)) + 1j * rng.standard_normal((M, 3))
+    C = X @ X.conj().T + np.diag(rng.uniform(0.1, 0.5, M))
+    h = rng.standard_normal((G, M)) + 1j * rng.standard_normal((G, M))
+    lam, V = np.linalg.eigh(C)
+
+    def brute(Cm):
+        Cd = Cm - np.diag(np.diag(Cm)) if r_diag else Cm
+        return np.array([sum(np.conj(h[g, i]) * Cd[i, j] * h[g, j] for i in range(M) for j in range(M)).real
+                         for g in range(G)])
+
+    got, nst = B.freq_beamformer_eig_custom(lam, V, h, 1.3, r_diag)
+    np.testing.assert_allclose(got, 1.3 * brute(C), rtol=1e-10, atol=1e-12 * np.abs(brute(C)).max())
+    np.testing.assert_allclose(nst, np.sum(np.abs(h) ** 2, axis=1), rtol=1e-14)
+    full, _ = B.freq_beamformer_custom(C, h, 1.3, r_diag)
+    np.testing.assert_allclose(got, full, rtol=1e-10, atol=1e-12 * np.abs(full).max())
+    sub = [2, 4, 6]
+    Ck = (V[:, sub] * lam[sub]) @ V[:, sub].conj().T
+    got_k, _ = B.freq_beamformer_eig_custom(lam[sub], V[:, sub], h, 1.0, r_diag)
+    np.testing.assert_allclose(got_k, brute(Ck), rtol=1e-10, atol=1e-12 * np.abs(brute(Ck)).max())
 
 
 def test_reference_phase_rounding_differs_from_fp64_only_slightly():
