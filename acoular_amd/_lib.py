@@ -73,6 +73,7 @@ _SIGS = {
     'abf_beamform': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), _c_dp, ctypes.c_size_t, _c_dp]),
     'abf_beamform_prepare': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), _c_dp, ctypes.c_size_t, _c_dp]),
     'abf_beamform_run': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), _c_dp, ctypes.c_size_t, _c_dp]),
+    'abf_beamform_noalloc': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), _c_dp]),
     'abf_beamform_batch_workspace_size': (
         ctypes.c_size_t, [ctypes.POINTER(AbfGridJob), ctypes.c_int32, ctypes.POINTER(ctypes.c_size_t)]),
     'abf_beamform_batch': (ctypes.c_int, [ctypes.POINTER(AbfGridJob), ctypes.c_int32, _c_dp, ctypes.c_size_t, _c_dp]),

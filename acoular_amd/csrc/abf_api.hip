@@ -549,6 +549,76 @@ int abf_beamform_run(const abf_grid_job *job, void *workspace, size_t workspace_
     return beamform_single(job, workspace, workspace_bytes, stream, kRun);
 }
 
+} // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// abf_beamform_noalloc: abf_beamform with its workspace in a static device array
+// ---------------------------------------------------------------------------------------------
+namespace {
+// Workspace of abf_beamform_noalloc (job descriptors + re-laid-out CSM of a chunk of bins).  Each
+// GPU holds its own copy for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device (scratch_ptr), never by this name.
+static __device__ __align__(256) char scratch[ABF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// device address of the current device's scratch array (looked up once per device)
+int scratch_ptr(void **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(ABF_ERR_HIP, "hipGetDevice failed");
+    if (dev < 0 || dev >= kMaxDevices) return fail(ABF_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ABF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return ABF_OK;
+}
+} // namespace
+
+extern "C" {
+
+int abf_beamform_noalloc(const abf_grid_job *job, void *stream)
+{
+    if (int r = validate_job(job)) return r;
+    if (int r = validate_job_pointers(job)) return r;
+    if (job->geom.num_grid == 0 || job->num_freqs == 0) return ABF_OK;
+    // bins per chunk: descriptors + the re-laid-out CSM of the chunk must fit the scratch array
+    abf_grid_job one = *job;
+    one.num_freqs = 1;
+    const size_t desc = align256(sizeof(MfJob));
+    const size_t per_bin = job_workspace(&one);
+    if (per_bin == 0) return g_err.empty() ? fail(ABF_ERR_INVALID, "invalid job") : ABF_ERR_INVALID;
+    if (desc + per_bin > (size_t)ABF_NOALLOC_SCRATCH_BYTES)
+        return fail(ABF_ERR_WORKSPACE, "one bin of this job needs " + std::to_string(desc + per_bin) +
+                                           " bytes of scratch; abf_beamform_noalloc has " +
+                                           std::to_string((size_t)ABF_NOALLOC_SCRATCH_BYTES));
+    // f16x2 keeps its per-frequency scales behind the re-laid-out CSM: one more 256-byte unit
+    const size_t room = (size_t)ABF_NOALLOC_SCRATCH_BYTES - desc - 256;
+    const int chunk = (int)std::min<size_t>((size_t)job->num_freqs, std::max<size_t>(1, room / per_bin));
+    void *ws = nullptr;
+    if (int r = scratch_ptr(&ws)) return r;
+    const int64_t M = job->geom.num_mics, G = job->geom.num_grid;
+    for (int b0 = 0; b0 < job->num_freqs; b0 += chunk) {
+        abf_grid_job sub = *job;
+        sub.num_freqs = std::min(chunk, job->num_freqs - b0);
+        sub.csm = static_cast<const char *>(job->csm) + (size_t)b0 * M * M * 8;
+        sub.wavenumber = job->wavenumber + b0;
+        sub.out = job->out + (size_t)b0 * G;
+        sub.out_norm = job->out_norm ? job->out_norm + (size_t)b0 * G : nullptr;
+        if (int r = beamform_batch_impl(&sub, 1, ws, (size_t)ABF_NOALLOC_SCRATCH_BYTES, stream, kPrep | kRun))
+            return r;
+    }
+    return ABF_OK;
+}
+
 size_t abf_beamform_table_workspace_size(int32_t num_mics)
 {
     return align256(plain_ut_bytes(num_mics, 1)) + 256;

@@ -127,6 +127,17 @@ int abf_beamform(const abf_grid_job *job, void *workspace, size_t workspace_byte
 int abf_beamform_prepare(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream);
 int abf_beamform_run(const abf_grid_job *job, void *workspace, size_t workspace_bytes, void *stream);
 
+/* abf_beamform without a workspace argument and without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The workspace is a static device array of
+ * ABF_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; the job's bins are processed in
+ * chunks that fit it, so any number of bins works, and a job one of whose bins alone needs more
+ * (e.g. the plain form above ~2800 microphones) is rejected with ABF_ERR_WORKSPACE.  Every call on
+ * a GPU shares that GPU's array: calls that may run at the same time on one GPU (other streams,
+ * other threads) race and must be serialised by the caller. */
+#define ABF_NOALLOC_SCRATCH_BYTES (64u << 20)
+int abf_beamform_noalloc(const abf_grid_job *job, void *stream);
+
 /* A batch of independent jobs (different mic counts, grids and bands) beamformed together:
  * one re-layout launch and one beamform launch per distinct (form, padded mic count),
  * work items of every job interleaved.  Each job's workspace is carved from 'workspace' at

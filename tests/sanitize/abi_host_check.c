@@ -87,6 +87,18 @@ int main(void)
         CHECK(abf_beamform_prepare(&j, NULL, 0, NULL) != ABF_OK);
         CHECK(abf_beamform_run(&j, NULL, 0, NULL) != ABF_OK);
     }
+    /* the no-allocation entry point: null job, and a bin larger than its static array (rejected
+     * before any device work; the pointers are never dereferenced) */
+    CHECK(abf_beamform_noalloc(NULL, NULL) == ABF_ERR_INVALID);
+    {
+        abf_grid_job j = job(10, 4096, 1, ABF_FORM_PLAIN_F32, ABF_STEER_CLASSIC);
+        j.csm = (const void *)256;
+        j.wavenumber = (const double *)256;
+        j.out = (float *)256;
+        j.geom.gpos = j.geom.mpos = (const double *)256;
+        CHECK(abf_beamform_noalloc(&j, NULL) == ABF_ERR_WORKSPACE);
+        CHECK(strstr(abf_last_error(), "scratch") != NULL);
+    }
     /* other workspace queries */
     CHECK(abf_beamform_table_workspace_size(256) > 0);
     CHECK(abf_beamform_eig_workspace_size(256, 16) == 0); /* the eigenvalue form needs no workspace */
