@@ -9,6 +9,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libabf.so')
+# (source, extra flags, object name); the streamed kernel's file builds once per block count
+STREAM = [('abf_bf_mfma_stream.hip', [f'-DABF_ST_NK={nk}'], f'abf_bf_mfma_stream_{nk}.o') for nk in range(8, 1, -1)]
 SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f16.hip', 'abf_bf_mfma_f32.hip', 'abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip',
            'abf_eig.hip', 'abf_deconv.hip', 'abf_sources.hip', 'abf_clean.hip', 'abf_resample.hip',
            'abf_api.hip']
@@ -21,9 +23,10 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-Xclang', '-tar
          '-packed-fp32-ops', '-I', os.path.join(HERE, '..', 'include')]
 
 
-def _compile(src: str, build_dir: str) -> str:
-    obj = os.path.join(build_dir, src.replace('.hip', '.o'))
-    cmd = [HIPCC, *FLAGS, '-c', os.path.join(CSRC, src), '-o', obj]
+def _compile(unit, build_dir: str) -> str:
+    src, extra, name = unit if isinstance(unit, tuple) else (unit, [], unit.replace('.hip', '.o'))
+    obj = os.path.join(build_dir, name)
+    cmd = [HIPCC, *FLAGS, *extra, '-c', os.path.join(CSRC, src), '-o', obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f'hipcc failed for {src}:\n{r.stderr}')
@@ -33,8 +36,9 @@ def _compile(src: str, build_dir: str) -> str:
 def build(verbose: bool = False) -> str:
     build_dir = os.path.join(HERE, '_build')
     os.makedirs(build_dir, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=6) as ex:
-        objs = list(ex.map(lambda s: _compile(s, build_dir), SOURCES))
+    # the slowest units first
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, build_dir), STREAM + SOURCES))
     tmp = LIB + '.tmp'
     cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp, *objs, *LIBS]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -63,6 +63,7 @@ struct MfPlan {
     int TT;  // row tiles per work unit (alternating kernel)
     bool pf; // software-pipelined fragment loads (split-bf16 alternating kernel)
     bool ws; // warp-specialised kernel (double-buffered steering matrix)
+    bool st; // streamed kernel (steering blocks through an LDS ring)
     int64_t ntiles, wgs;
     size_t prep_bytes, lds;
     size_t uscale_off; // f16x2: offset of the per-frequency U scales in the job's workspace
@@ -121,6 +122,24 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     }
     p.CT = 0;
     p.ws = false;
+    p.st = false;
+    if (kreq == ABF_MFMA_KERNEL_STREAMED) {
+        // split-bf16, 2..8 blocks of 32 microphones, the reference's float32 phase with the hardware
+        // sin/cos; anything else runs the row-pair kernel
+        if (p.prec == kPrBF16X3 && p.Mp >= 64 && p.Mp <= 256 && (j->phase_mode & kFastTrigFlag) &&
+            (j->phase_mode & kPhaseMask) == ABF_PHASE_REFERENCE) {
+            p.st = true;
+            p.CT = kStBN / 16;
+            p.TT = 1;
+            p.CW = 1;
+            p.pf = false;
+            p.ntiles = cdiv(j->geom.num_grid, kStBN);
+            p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
+            p.lds = st_lds_bytes();
+            return ABF_OK;
+        }
+        kreq = ABF_MFMA_KERNEL_ROW_PAIRS;
+    }
     if (kreq == ABF_MFMA_KERNEL_SPECIALISED) {
         for (int ct = 4; ct >= 2; ct /= 2) { // tiles of 2 column groups of 32 grid points
             if (ct > ct_cap || (ct_req > 0 && ct > ct_req)) continue;
@@ -173,7 +192,7 @@ int validate_job(const abf_grid_job *j)
     if (int r = check_steer(j->steer_type)) return r;
     if (int r = check_phase(j->phase_mode)) return r;
     if (j->form < ABF_FORM_PLAIN_F32 || j->form > ABF_FORM_MFMA_F16X2) return fail(ABF_ERR_INVALID, "unknown form");
-    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS)
+    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_STREAMED)
         return fail(ABF_ERR_INVALID, "unknown MFMA kernel variant");
     if (j->tile_cols < 0 || j->tile_freqs < 0) return fail(ABF_ERR_INVALID, "negative tile size");
     return ABF_OK;
@@ -290,6 +309,29 @@ void launch_mfma_var(int var, int CW, int TT, bool pf, bool ws, const MfJob *dev
     case 3: launch_mfma_form<B, 3>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
     case 4: launch_mfma_form<B, 4>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
     default: launch_mfma_form<B, 5>(CW, TT, pf, ws, dev, n, wgs, lds, s); break;
+    }
+}
+
+template <int NK, int WK> void launch_stream(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+{
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
+        (void)hipFuncSetAttribute((const void *)bf_mfma_stream_kernel<NK, WK>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
+    });
+    hipLaunchKernelGGL((bf_mfma_stream_kernel<NK, WK>), dim3((unsigned)wgs), dim3(kStWaves * 64), lds, s, dev, n);
+}
+
+template <int WK> void launch_stream_nk(int nk, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+{
+    switch (nk) {
+    case 2: launch_stream<2, WK>(dev, n, wgs, lds, s); break;
+    case 3: launch_stream<3, WK>(dev, n, wgs, lds, s); break;
+    case 4: launch_stream<4, WK>(dev, n, wgs, lds, s); break;
+    case 5: launch_stream<5, WK>(dev, n, wgs, lds, s); break;
+    case 6: launch_stream<6, WK>(dev, n, wgs, lds, s); break;
+    case 7: launch_stream<7, WK>(dev, n, wgs, lds, s); break;
+    default: launch_stream<8, WK>(dev, n, wgs, lds, s); break;
     }
 }
 
@@ -413,7 +455,8 @@ int abf_beamform_plan(const abf_grid_job *job, int32_t *tile_kernel, int32_t *ti
     } else {
         MfPlan p;
         if (int r = plan_mfma(job, p)) return r;
-        if (p.ws) k = ABF_MFMA_KERNEL_SPECIALISED;
+        if (p.st) k = ABF_MFMA_KERNEL_STREAMED;
+        else if (p.ws) k = ABF_MFMA_KERNEL_SPECIALISED;
         else if (p.pf) k = p.TT == 2 ? ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS : ABF_MFMA_KERNEL_PIPELINED;
         else k = p.TT == 2 ? ABF_MFMA_KERNEL_ROW_PAIRS : ABF_MFMA_KERNEL_ALTERNATING;
         c = 16 * p.CT;
@@ -464,6 +507,8 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         bool pf, ws;
         int var;
         std::vector<int> idx;
+        bool st;
+        int nk; // streamed kernel: blocks of 32 microphones
     };
     std::vector<Group> groups;
     std::vector<MfPlan> plans(num_jobs);
@@ -480,15 +525,16 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         if (int r = plan_mfma(j, plans[i])) return r;
         if (mode & kPrep) launch_prep_mfma(j, plans[i], ws + offs[i], s);
         if (!(mode & kRun)) continue;
+        const int nk = plans[i].st ? plans[i].nK : 0;
         auto it = std::find_if(groups.begin(), groups.end(),
                                [&](const Group &g) {
                                    return g.prec == plans[i].prec && g.CW == plans[i].CW && g.TT == plans[i].TT &&
-                                          g.pf == plans[i].pf && g.ws == plans[i].ws &&
-                                          g.var == mfma_variant(&jobs[i]);
+                                          g.pf == plans[i].pf && g.ws == plans[i].ws && g.st == plans[i].st &&
+                                          g.nk == nk && g.var == mfma_variant(&jobs[i]);
                                });
         if (it == groups.end()) {
-            groups.push_back(
-                {plans[i].prec, plans[i].CW, plans[i].TT, plans[i].pf, plans[i].ws, mfma_variant(&jobs[i]), {}});
+            groups.push_back({plans[i].prec, plans[i].CW, plans[i].TT, plans[i].pf, plans[i].ws,
+                              mfma_variant(&jobs[i]), {}, plans[i].st, nk});
             it = groups.end() - 1;
         }
         it->idx.push_back(i);
@@ -510,7 +556,15 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
             for (int q = 0; q < pk.n; ++q) pk.jobs[q] = descs[b + q];
             hipLaunchKernelGGL(write_jobs_kernel, dim3(1), dim3(64), 0, s, pk, dev_jobs + desc_base + b);
         }
-        launch_mfma_group(gr.prec, gr.CW, gr.TT, gr.pf, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+        if (gr.st) {
+            const int wk = gr.var >> 1;
+            if (wk == 0) launch_stream_nk<0>(gr.nk, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+            else if (wk == 1) launch_stream_nk<1>(gr.nk, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+            else launch_stream_nk<2>(gr.nk, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+        } else {
+            launch_mfma_group(gr.prec, gr.CW, gr.TT, gr.pf, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg,
+                              lds, s);
+        }
         if (int r = hip_status("MFMA beamformer launch")) return r;
         desc_base += (int)descs.size();
     }

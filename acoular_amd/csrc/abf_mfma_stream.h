@@ -1,0 +1,362 @@
+// Streamed split-bf16 MFMA beamformer (device code), instantiated in abf_bf_mfma_stream.hip.
+//
+// The alternating kernels hold the whole steering matrix of a tile in LDS and run its generation
+// (VALU: phase, reduction, cos/sin, hi/lo split) and the MFMA work one after the other, so the
+// MFMA pipe idles during generation.  This kernel streams the steering matrix instead, one
+// 32-microphone K-block at a time through a two-block LDS ring, and overlaps the generation of
+// the next block with the MFMAs of the current one:
+//
+//  * the K-blocks J of a frequency are visited in descending order; at step J each wave adds
+//    U(I, J) X(J) into the accumulators of every row block I <= J of its output tile.  Row block
+//    J receives its last contribution at step J, so its epilogue q += Re sum conj(x) Y runs right
+//    there, on the steering values of block J that the wave holds anyway (the K-slot order of
+//    kslot_mic puts the values of result row r in the lane that holds row r);
+//  * every wave owns one 16-column tile and one 16-row half of all row blocks (8 waves: 4
+//    column tiles x 2 halves, 64 grid points per workgroup), so all waves carry the same MFMA
+//    work in every step;
+//  * each thread generates 4 phasors of the next block per step from float64 distances it keeps
+//    in registers for the whole band (4 per block);
+//  * the U blocks (both halves, 8 KiB) of the (I, J) pairs enter the workgroup once: a step is
+//    cut into sub-steps of at most 4 pairs, every wave loads its 1 KiB share of the blocks of
+//    sub-step t + 2 into registers and writes the share it loaded one sub-step earlier into a
+//    three-deep LDS ring, and one barrier per sub-step hands the ring over.  The four waves of a
+//    half read their fragments from LDS (global loads of the same fragment by four waves would
+//    go through the CU's L1 four times, the bound of the first form of this kernel);
+//  * the sign of -Ui Xi is taken from a negated copy of the steering fragment, made once per
+//    step, instead of flipping the U fragment of every pair.
+//
+// Supported: split-bf16 form, 64 <= Mp <= 256 (NK = Mp / 32 blocks), hardware sin/cos with the
+// reference's float32 phase; other cases use the alternating kernels.
+#pragma once
+
+#include <utility>
+
+#include "abf_mfma_impl.h" // store_frequency, find_job, dist3
+
+namespace abf {
+
+// fn(std::integral_constant<int, S>) for S = 0 .. N - 1, in order
+template <class Fn, int... S> __device__ __forceinline__ void st_for(Fn &&fn, std::integer_sequence<int, S...>)
+{
+    (fn(std::integral_constant<int, S>{}), ...);
+}
+
+// The sub-steps of a frequency: step J (descending) covers the pairs I = J, J - 1, .., 0 (the
+// pair (J, J) that closes row block J first), cut into ceil((J + 1) / kStGS) chunks of nearly
+// equal size [I0, I1) counted downwards; the top step takes one more chunk when the count would
+// be odd (an even count keeps the register and LDS double buffers of the U blocks in phase from
+// one frequency to the next).
+template <int NK> struct StSub {
+    static constexpr int kMax = 2 * NK + 2;
+    int n;
+    int J[kMax], I0[kMax], I1[kMax];
+    bool first[kMax], last[kMax];
+    constexpr StSub() : n(0), J(), I0(), I1(), first(), last()
+    {
+        int tot = 0;
+        for (int j = NK - 1; j >= 0; --j) tot += (j + kStGS) / kStGS;
+        const int extra = tot & 1;
+        for (int j = NK - 1; j >= 0; --j) {
+            const int np = j + 1;
+            const int nc = (np + kStGS - 1) / kStGS + (j == NK - 1 ? extra : 0);
+            int done = 0;
+            for (int q = 0; q < nc; ++q) {
+                const int sz = (np - done) / (nc - q) + ((np - done) % (nc - q) ? 1 : 0);
+                J[n] = j;
+                I0[n] = j - done;
+                I1[n] = j - done - sz;
+                first[n] = q == 0;
+                last[n] = q == nc - 1;
+                done += sz;
+                ++n;
+            }
+        }
+    }
+};
+
+template <int NK, int WK>
+__global__ __launch_bounds__(kStWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void bf_mfma_stream_kernel(
+    const MfJob *__restrict__ jobs, int njobs)
+{
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    uint16_t *xs = reinterpret_cast<uint16_t *>(smem);                   // steering, bf16 [2][4][BN][KSB]
+    float *xf = reinterpret_cast<float *>(smem + kStOffXf);              // steering, f32 [2][2][BN][KSF]
+    char *us = smem + kStOffU;                                           // U ring [2][kStGS][8 KiB]
+    float *s_q = reinterpret_cast<float *>(smem + kStOffQ);              // q partials [2][2][BN]
+    double *s_fac = reinterpret_cast<double *>(smem + kStOffFac);        // normalisation factor [BN]
+    float *s_nst = reinterpret_cast<float *>(smem + kStOffNst);          // steering normalisation [BN]
+    double *s_r0 = reinterpret_cast<double *>(smem + kStOffR0);          // [BN]
+    double *s_help = reinterpret_cast<double *>(smem + kStOffHelp);      // steering-sum partials [8][BN]
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t wg = blockIdx.x;
+    const MfJob &jb = jobs[find_job(jobs, njobs, wg)];
+    const int M = jb.M, F = jb.F, FB = jb.FB;
+    const int64_t G = jb.G;
+    const int64_t local = wg - jb.wg_begin;
+    const int tile = (int)(local % jb.ntiles);
+    const int f0 = (int)(local / jb.ntiles) * FB;
+    const int f1 = min(f0 + FB, F);
+
+    // ---- generation role: grid point col = lane, K-slots 4 sg .. 4 sg + 3 of every block ----
+    const int col = lane, sgp = wave;
+    const int64_t g = (int64_t)tile * kStBN + col;
+    const bool valid = g < G;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (valid) {
+        gx = gp(jb.gpos)[g];
+        gy = gp(jb.gpos)[G + g];
+        gz = gp(jb.gpos)[2 * G + g];
+    }
+    const ABF_GLOBAL double *mp = gp(jb.mpos);
+    double rr[NK][4];
+    double help = 0.0;
+#pragma unroll
+    for (int J = 0; J < NK; ++J)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 32 * J + 4 * (sgp >> 1) + i + 16 * (sgp & 1);
+            double r = 1.0;
+            if (valid && m < M) {
+                r = dist3(gx, gy, gz, mp[m], mp[M + m], mp[2 * M + m]);
+                help += help_term(jb.steer_type, r);
+            }
+            rr[J][i] = r;
+        }
+    s_help[sgp * kStBN + col] = help;
+    if (sgp == 0) {
+        double r0 = jb.ref_dist;
+        if (!(r0 > 0.0)) r0 = dist3(gx, gy, gz, jb.ref[0], jb.ref[1], jb.ref[2]);
+        s_r0[col] = r0;
+    }
+
+    // phasors of a block (distances rb) for wavenumber k into steering buffer b: K-slots
+    // 4 sg .. 4 sg + 3 of column col, as bf16 hi/lo planes (MFMA operand) and float32 (epilogue)
+    auto generate = [&](const double (&rb)[4], double k, int b) __attribute__((always_inline)) {
+        float xr[4], xi[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double r = rb[i];
+            if constexpr (WK != 0) asm volatile("" : "+v"(r)); // keeps the weight in the band loop
+            // the reference's float32 phase (fastFuncs.py:260), reduced to [-1/2, 1/2] revolution
+            // in float64 for the hardware sin/cos
+            const double phi = (double)(float)(k * r);
+            const double u = phi * kInvTwoPi;
+            const float t = (float)(u - rint(u));
+            const float sn = __builtin_amdgcn_sinf(t);
+            const float cs = __builtin_amdgcn_cosf(t);
+            if constexpr (WK == 0) {
+                xr[i] = cs;
+                xi[i] = -sn;
+            } else {
+                const float rf = (float)r;
+                const float w = WK == 1 ? rf : __builtin_amdgcn_rcpf(rf);
+                xr[i] = w * cs;
+                xi[i] = -w * sn;
+            }
+        }
+        // hi = RNE bf16 pair, lo = RNE bf16 of the residuals; 8 bytes per plane
+        uint32_t h[4], l[4];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const float *src = d == 0 ? xr : xi;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const bf16x2 hp = {(__bf16)src[2 * q], (__bf16)src[2 * q + 1]};
+                const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
+                const float h0 = __uint_as_float(hb << 16), h1 = __uint_as_float(hb & 0xffff0000u);
+                const bf16x2 lp = {(__bf16)(src[2 * q] - h0), (__bf16)(src[2 * q + 1] - h1)};
+                h[2 * d + q] = hb;
+                l[2 * d + q] = __builtin_bit_cast(uint32_t, lp);
+            }
+        }
+        uint16_t *dst = xs + b * kStBlock + col * kStKSB + 4 * sgp;
+        *reinterpret_cast<uint2 *>(dst) = make_uint2(h[0], h[1]);
+        *reinterpret_cast<uint2 *>(dst + kStPlane) = make_uint2(l[0], l[1]);
+        *reinterpret_cast<uint2 *>(dst + 2 * kStPlane) = make_uint2(h[2], h[3]);
+        *reinterpret_cast<uint2 *>(dst + 3 * kStPlane) = make_uint2(l[2], l[3]);
+        float *fd = xf + b * kStBlockF + col * kStKSF + 4 * sgp;
+        *reinterpret_cast<f4 *>(fd) = f4{xr[0], xr[1], xr[2], xr[3]};
+        *reinterpret_cast<f4 *>(fd + kStPlaneF) = f4{xi[0], xi[1], xi[2], xi[3]};
+    };
+
+    // ---- MFMA role: column tile c, row half p of every row block ----
+    const int c = wave & 3, p = wave >> 2;
+    const int xoff = (c * 16 + (lane & 15)) * kStKSB + (lane >> 4) * 8;             // B fragment
+    const int foff = (c * 16 + (lane & 15)) * kStKSF + (lane >> 4) * 8 + p * 4;     // epilogue values
+    const ABF_GLOBAL char *prep_base = gp(reinterpret_cast<const char *>(jb.prep));
+    constexpr int64_t kFreqBytes = (int64_t)(NK * (NK + 1) / 2) * 8192;
+    constexpr StSub<NK> sub;
+    constexpr int NS = sub.n;
+    static_assert(NS % 2 == 0, "an even number of sub-steps per frequency");
+    // this wave's 1 KiB share (half w >> 2, plane w & 3) of U block (I, J) of frequency f
+    auto load_share = [&](int f, int I, int J) __attribute__((always_inline)) {
+        const int ui = I * NK - (I * (I - 1)) / 2 + (J - I);
+        return *reinterpret_cast<const ABF_GLOBAL u4 *>(prep_base + (int64_t)f * kFreqBytes + (int64_t)ui * 8192 +
+                                                         wave * 1024 + lane * 16);
+    };
+    // shares of the blocks of sub-step ts (of frequency ff) into a register set; the pairs of a
+    // sub-step are I = I0 - q for q = 0 .. n - 1 (descending)
+    auto load_sub = [&](auto tsc, int ff, u4 (&st)[kStGS]) __attribute__((always_inline)) {
+        constexpr int ts = decltype(tsc)::value;
+#pragma unroll
+        for (int q = 0; q < sub.I0[ts] - sub.I1[ts]; ++q) st[q] = load_share(ff, sub.I0[ts] - q, sub.J[ts]);
+    };
+    auto write_sub = [&](auto tsc, int buf, const u4 (&st)[kStGS]) __attribute__((always_inline)) {
+        constexpr int ts = decltype(tsc)::value;
+#pragma unroll
+        for (int q = 0; q < sub.I0[ts] - sub.I1[ts]; ++q)
+            *reinterpret_cast<u4 *>(us + (buf * kStGS + q) * 8192 + wave * 1024 + lane * 16) = st[q];
+    };
+
+    f4 accr[NK], acci[NK];
+#pragma unroll
+    for (int I = 0; I < NK; ++I) {
+        accr[I] = f4{0.f, 0.f, 0.f, 0.f};
+        acci[I] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    const u4 sg = u4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+    u4 stage[2][kStGS];
+
+    // first block of the band; U of its sub-step 0 into LDS buffer 0, of sub-step 1 into registers
+    generate(rr[NK - 1], gp(jb.wavenumber)[f0], 0);
+    load_sub(std::integral_constant<int, 0>{}, f0, stage[0]);
+    load_sub(std::integral_constant<int, 1>{}, f0, stage[1]);
+    write_sub(std::integral_constant<int, 0>{}, 0, stage[0]);
+    __syncthreads();
+    // frequency-independent normalisation of every grid point (fastFuncs.py:293-310):
+    // result = q * fac, normalizeSteer = nst
+    if (wave == 0) {
+        double hs = 0.0;
+        for (int q = 0; q < 8; ++q) hs += s_help[q * kStBN + lane];
+        double fac, nst;
+        normalise(jb.steer_type, 1.0, s_r0[lane], hs, (double)M, jb.norm, fac, nst);
+        s_fac[lane] = fac;
+        s_nst[lane] = (float)nst;
+    }
+    __syncthreads();
+
+    // maps of frequency ff from its q partials (the halves added in a fixed order)
+    auto store = [&](int ff) __attribute__((always_inline)) {
+        const float *sq = s_q + ((ff - f0) & 1) * 2 * kStBN;
+        const int64_t gg = (int64_t)tile * kStBN + lane;
+        if (gg < G) {
+            const double qs = (double)sq[lane] + (double)sq[kStBN + lane];
+            float rf = (float)(qs * s_fac[lane]);
+            if (jb.clip && rf < 0.f) rf = 0.f;
+            gpw(jb.out)[(int64_t)ff * G + gg] = rf;
+            if (jb.out_norm) gpw(jb.out_norm)[(int64_t)ff * G + gg] = s_nst[lane];
+        }
+    };
+
+    int par = 0; // parity of the steering buffer in use at the first step of frequency f
+    for (int f = f0; f < f1; ++f) {
+        const int fn = min(f + 1, f1 - 1); // next frequency (clamped: the loads stay in bounds)
+        const double knext = gp(jb.wavenumber)[fn];
+        const double kcur = gp(jb.wavenumber)[f];
+        // maps of the previous frequency: its q partials were written before the last barrier
+        if (f > f0 && wave == 0) store(f - 1);
+        float qacc = 0.f;
+        u4 xrh, xrl, xih, xil, xnh, xnl;
+        auto substep = [&](auto tc) __attribute__((always_inline)) {
+            constexpr int t = decltype(tc)::value;
+            constexpr int J = sub.J[t];
+            constexpr int ub = t & 1; // LDS U buffer of this sub-step
+            const int bcur = (par + (NK - 1 - J)) & 1;
+            if constexpr (sub.first[t]) {
+                // steering fragment of block J, column tile c (K-slots of lane group lane >> 4)
+                const uint16_t *xb = xs + bcur * kStBlock + xoff;
+                xrh = *reinterpret_cast<const u4 *>(xb);
+                xrl = *reinterpret_cast<const u4 *>(xb + kStPlane);
+                xih = *reinterpret_cast<const u4 *>(xb + 2 * kStPlane);
+                xil = *reinterpret_cast<const u4 *>(xb + 3 * kStPlane);
+                xnh = xih ^ sg;
+                xnl = xil ^ sg;
+            }
+            // the VALU work (U staging, steering generation) and the MFMA work of a sub-step in
+            // opposite orders in the two waves that share a SIMD (waves w and w + 4, or w and w ^ 1,
+            // depending on how the workgroup is laid out over the SIMDs: the phase bit below differs
+            // within both pairs), so that one issues MFMAs while the other runs VALU
+            auto stage_gen = [&]() __attribute__((always_inline)) {
+            // U of sub-step t + 1 (loaded one sub-step ago) into the LDS ring; loads of t + 2
+            {
+                constexpr int t1 = t + 1 < NS ? t + 1 : t + 1 - NS;
+                constexpr int t2 = t + 2 < NS ? t + 2 : t + 2 - NS;
+                write_sub(std::integral_constant<int, t1>{}, ub ^ 1, stage[(t + 1) & 1]);
+                load_sub(std::integral_constant<int, t2>{}, t + 2 < NS ? f : fn, stage[t & 1]);
+            }
+            if constexpr (sub.first[t]) {
+                // next block: J - 1 of this frequency, or the last block of the next one
+                if constexpr (J > 0)
+                    generate(rr[J > 0 ? J - 1 : 0], kcur, bcur ^ 1);
+                else
+                    generate(rr[NK - 1], knext, bcur ^ 1);
+            }
+            };
+            auto mfma = [&]() __attribute__((always_inline)) {
+            const char *ua = us + ub * (kStGS * 8192) + p * 4096 + lane * 16;
+#pragma unroll
+            for (int q = 0; q < sub.I0[t] - sub.I1[t]; ++q) {
+                constexpr int I0 = sub.I0[t];
+                const int I = I0 - q;
+                const char *a = ua + q * 8192;
+                const u4 a_rh = *reinterpret_cast<const u4 *>(a);
+                const u4 a_rl = *reinterpret_cast<const u4 *>(a + 1024);
+                const u4 a_ih = *reinterpret_cast<const u4 *>(a + 2048);
+                const u4 a_il = *reinterpret_cast<const u4 *>(a + 3072);
+#define MF(A, B, C) C = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, A), \
+                                                                __builtin_bit_cast(bf16x8, B), C, 0, 0, 0)
+                // Yr = Ur Xr - Ui Xi, Yi = Ui Xr + Ur Xi, each product hi*hi + hi*lo + lo*hi
+                MF(a_rh, xrh, accr[I]);
+                MF(a_ih, xrh, acci[I]);
+                MF(a_rh, xrl, accr[I]);
+                MF(a_ih, xrl, acci[I]);
+                MF(a_rl, xrh, accr[I]);
+                MF(a_il, xrh, acci[I]);
+                MF(a_ih, xnh, accr[I]);
+                MF(a_rh, xih, acci[I]);
+                MF(a_ih, xnl, accr[I]);
+                MF(a_rh, xil, acci[I]);
+                MF(a_il, xnh, accr[I]);
+                MF(a_rl, xih, acci[I]);
+#undef MF
+                if (q == 0 && sub.first[t]) {
+                    // pair (J, J) closes row block J: q += Re sum conj(x) Y over its rows 4 g + i of
+                    // half p, whose float32 steering values sit at K-slots 8 g + 4 p + i
+                    const float *fb = xf + bcur * kStBlockF + foff;
+                    const f4 xr = *reinterpret_cast<const f4 *>(fb);
+                    const f4 xi = *reinterpret_cast<const f4 *>(fb + kStPlaneF);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        qacc = fmaf(xr[e], accr[J][e], qacc);
+                        qacc = fmaf(xi[e], acci[J][e], qacc);
+                    }
+                    accr[J] = f4{0.f, 0.f, 0.f, 0.f};
+                    acci[J] = f4{0.f, 0.f, 0.f, 0.f};
+                }
+            }
+            };
+            if (((wave & 1) ^ (wave >> 2)) == 0) {
+                stage_gen();
+                mfma();
+            } else {
+                mfma();
+                stage_gen();
+            }
+            if constexpr (J == 0 && sub.last[t]) {
+                // the frequency's q partial of this wave's half and column tile
+                qacc += __shfl_xor(qacc, 16);
+                qacc += __shfl_xor(qacc, 32);
+                if (lane < 16) s_q[((f - f0) & 1) * 2 * kStBN + p * kStBN + c * 16 + lane] = qacc;
+            }
+            __syncthreads();
+        };
+        st_for(substep, std::make_integer_sequence<int, NS>{});
+        par ^= NK & 1;
+    }
+    if (wave == 0) store(f1 - 1);
+}
+
+} // namespace abf

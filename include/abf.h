@@ -71,7 +71,11 @@ enum abf_mfma_kernel {
                                         32-row block (half the LDS reads per MFMA) */
     ABF_MFMA_KERNEL_PIPELINED = 4,   /* alternating, X and U fragments of the next block loaded
                                         before the MFMAs of the current one (split-bf16) */
-    ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS = 5 /* both of the above, one column tile per fragment */
+    ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS = 5, /* both of the above, one column tile per fragment */
+    ABF_MFMA_KERNEL_STREAMED = 6 /* steering streamed through LDS one 32-microphone block at a time,
+                                    its generation overlapped with the MFMAs of the previous block
+                                    (split-bf16, 33..256 mics, reference phase with hardware sin/cos;
+                                    other jobs fall back to ROW_PAIRS) */
 };
 
 /* Steering geometry shared by the grid entry points. r0 = ref_dist if ref_dist > 0, else

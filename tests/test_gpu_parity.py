@@ -412,7 +412,8 @@ def test_batch_equals_alone_bitwise(gpu):
 @pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f16x2', 'mfma_f32'])
 @pytest.mark.parametrize('kernel,tile_cols,tile_freqs', [(1, 0, 0), (1, 16, 1), (1, 64, 3), (2, 0, 0), (2, 64, 2),
                                                          (2, 32, 5), (3, 0, 0), (3, 16, 2), (3, 64, 4), (4, 0, 0),
-                                                         (4, 64, 3), (5, 0, 0), (5, 32, 2)])
+                                                         (4, 64, 3), (5, 0, 0), (5, 32, 2), (6, 0, 0), (6, 64, 3),
+                                                         (6, 16, 1)])
 @pytest.mark.parametrize('nm,steer', [(64, 'true level'), (100, 'classic'), (256, 'inverse'), (33, 'true location')])
 def test_mfma_kernel_variants_and_tiles(gpu, form, kernel, tile_cols, tile_freqs, nm, steer):
     """Every MFMA kernel variant (alternating, warp-specialised, row pairs, pipelined) and explicit tile shapes, including
@@ -436,6 +437,68 @@ def test_mfma_kernel_variants_and_tiles(gpu, form, kernel, tile_cols, tile_freqs
     err = peak_rel_err(out.cpu().numpy(), ref)
     print(f'form={form} kernel={kernel} tile=({tile_cols},{tile_freqs}) nm={nm} steer={steer} err={err:.2e}')
     assert err <= _tol(form)
+
+
+@pytest.mark.parametrize('nm', [33, 64, 65, 100, 128, 160, 192, 224, 256])
+@pytest.mark.parametrize('steer', STEERS)
+def test_streamed_kernel_every_block_count(gpu, nm, steer):
+    """The streamed kernel (one instantiation per 2..8 blocks of 32 microphones): both r_diag
+    settings, a grid whose last tile is partial, a band that does not divide into the frequency
+    tile; the plan reports the streamed kernel."""
+    torch = _lib.torch()
+    freqs = np.array([700.0, 3300.0, 9000.0, 14000.0, 19500.0])
+    grid = aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.2, y_max=0.2, z=0.5, increment=0.03)  # 21 x 14
+    mics, grid, csm = _scene(nm, grid, freqs, z=0.5, seed=nm + 1)
+    sv = aa.SteeringVector(grid=grid, mics=mics, steer_type=steer)
+    gpos, mpos = sv.device_geometry()
+    dcsm = _lib.to_device(csm, torch.complex64)
+    k = torch.from_numpy(2 * np.pi * freqs / 343.0).to(gpu)
+    lib = _lib.lib()
+    for r_diag in (True, False):
+        out = torch.full((len(freqs), grid.size), -7.0, dtype=torch.float32, device=gpu)
+        onorm = torch.empty_like(out)
+        norm = nm / (nm - 1.0) if r_diag else 1.0
+        job = make_job(dcsm, k, gpos, mpos, sv.ref, steer, r_diag, norm, 'mfma_bf16x3', 'reference', out, onorm,
+                       r_diag, fast_trig=True)
+        job.tile_kernel, job.tile_cols, job.tile_freqs = 6, 64, 2
+        plan = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()]
+        _lib.check(lib.abf_beamform_plan(job, *[ctypes.byref(x) for x in plan]))
+        assert plan[0].value == 6
+        ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
+        _lib.check(lib.abf_beamform(job, _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
+        ref = _oracle_maps(csm, freqs, mics, grid, steer, r_diag)
+        err = peak_rel_err(out.cpu().numpy(), ref)
+        print(f'nm={nm} steer={steer} r_diag={r_diag} err={err:.2e}')
+        assert err <= TOL
+        want_norm = np.stack([B.freq_beamformer_full_csm(csm[0].astype(np.complex128), B.steer_r0(grid.pos, sv.ref),
+                                                         B.dist_mat(grid.pos, mics.pos), 1.0, 1.0, r_diag, steer)[1]]
+                             * len(freqs))
+        np.testing.assert_allclose(onorm.cpu().numpy(), want_norm, rtol=1e-5)
+
+
+def test_streamed_kernel_falls_back_outside_its_cases(gpu):
+    """The streamed kernel asked for the float64 phase, the library sin/cos or more than 256
+    microphones runs the row-pair kernel (the plan says so) and stays within the gate."""
+    torch = _lib.torch()
+    lib = _lib.lib()
+    freqs = np.array([2000.0, 17000.0])
+    grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.5, increment=0.04)
+    for nm, phase, fast in ((64, 'fp64', True), (64, 'reference', False), (300, 'reference', True)):
+        mics, grid, csm = _scene(nm, grid, freqs, z=0.5, seed=3)
+        sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='true level')
+        gpos, mpos = sv.device_geometry()
+        out = torch.empty((len(freqs), grid.size), dtype=torch.float32, device=gpu)
+        job = make_job(_lib.to_device(csm, torch.complex64), torch.from_numpy(2 * np.pi * freqs / 343.0).to(gpu),
+                       gpos, mpos, sv.ref, 'true level', True, nm / (nm - 1.0), 'mfma_bf16x3', phase, out, None, True,
+                       fast_trig=fast)
+        job.tile_kernel, job.tile_cols, job.tile_freqs = 6, 64, 2
+        plan = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()]
+        _lib.check(lib.abf_beamform_plan(job, *[ctypes.byref(x) for x in plan]))
+        assert plan[0].value in (1, 3)
+        ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
+        _lib.check(lib.abf_beamform(job, _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
+        ref = _oracle_maps(csm, freqs, mics, grid, 'true level', True, phase)
+        assert peak_rel_err(out.cpu().numpy(), ref) <= TOL
 
 
 def test_mfma_kernel_variant_rejected_when_unknown(gpu):
