@@ -114,6 +114,15 @@ def measured_traffic(form, runner):
     return None if e is None else int(e['bytes_per_launch'])
 
 
+def dominant_kernel_name(form, runner):
+    """Name of the beamform launch's kernel (as rocprofv3 lists it) for the job's launch shape."""
+    if form == 'plain_f32':
+        return 'bf_plain_kernel'
+    from acoular_amd import autotune
+
+    return 'bf_mfma_stream_kernel' if autotune.plan(runner.job)[0] == 6 else 'bf_mfma_kernel'
+
+
 def executed_flops_per_bin(form, nm):
     """Flops the kernels execute per map-bin (the upper block triangle of U only)."""
     if form == 'plain_f32':
@@ -280,6 +289,35 @@ def parity_sample(wl, runner, bins):
     return res
 
 
+def variants_bench(wl, dev, steps=2):
+    """Every form on the 256-mic workload for 'classic' and 'inverse' with and without r_diag:
+    map-bins/s (mean of ``steps`` steps after one warm-up, prepare + beamform) and the max relative
+    map error against the oracle on two bins."""
+    import torch
+
+    from oracle import beamform as B
+
+    bins = [100, 400]
+    out = []
+    for steer in ('classic', 'inverse'):
+        for rd in (True, False):
+            ref = B.beamformer_map(wl['csm'][bins].cpu().numpy().astype(np.complex128), wl['freqs'][bins], wl['gpos'],
+                                   wl['mpos'], steer, rd, ind=np.arange(len(bins)))
+            peak = np.abs(ref).max(axis=1)
+            for form in ('mfma_bf16x3', 'mfma_f32', 'plain_f32', 'mfma_f16x2'):
+                r = Runner(wl, form, steer=steer, r_diag=rd, dev=dev)
+                t = time_steps(r.step, steps, 1)
+                got = r.out[bins].double().cpu().numpy()
+                err = float(np.max(np.max(np.abs(got - ref), axis=1) / peak))
+                out.append({'form': form, 'steer_type': steer, 'r_diag': rd, 'map_bins_per_s': r.map_bins / t,
+                            'ms_per_step': t * 1e3, 'max_rel_err': err, 'err_bins': bins,
+                            'gate': 5e-4 if form == 'mfma_f16x2' else 1e-4})
+                log(f'variant {form:12s} {steer:8s} r_diag={rd!s:5s}: {r.map_bins / t:.4g} map-bins/s, err {err:.2e}')
+                del r
+                torch.cuda.empty_cache()
+    return out
+
+
 def _cpu_model():
     try:
         with open('/proc/cpuinfo') as fh:
@@ -351,7 +389,7 @@ def serving_bench(args):
     from acoular_amd import serving
 
     log('serving: resident CSM pools (analytic, 513 bins, 32..512 mics)')
-    fac = serving.RequestFactory(seed=7, pool=2)
+    fac = serving.RequestFactory(seed=7)
     mfma = dict(hbm_budget=24 << 30, max_batch=64, num_streams=2, keep_results=False, staging_bytes=1 << 30)
     plain = dict(hbm_budget=24 << 30, max_batch=1, num_streams=1, form='plain_f32', batched=False,
                  keep_results=False, staging_bytes=1 << 30)
@@ -1089,6 +1127,8 @@ def main():
     ap.add_argument('--no-tune', action='store_true', help='default launch shapes (no autotuner table, no tuning)')
     ap.add_argument('--tune', action='store_true', help='tune the launch shape now even if the table has it')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--no-extras', action='store_true',
+                    help='headline only: skip the per-variant, fused, large-3D and serving figures of the default line')
     ap.add_argument('--mode', default='headline', choices=['headline', 'copy', 'serving', 'fused', 'large3d', 'eig', 'api',
                                                           'damas', 'cleansc', 'clean', 'rfft', 'c1'],
                     help="'copy': a 1 GiB device copy, the calibration kernel for HBM byte counters; "
@@ -1183,7 +1223,7 @@ def main():
     roofline = {'bound': 'mfma' if args.form != 'plain_f32' else 'valu', 'achieved': round(achieved, 2),
                 'peak': bound_peak, 'unit': 'TFLOP/s', 'frac': round(achieved / bound_peak, 4),
                 'traffic': measured_traffic(args.form, runner),
-                'kernel': 'bf_mfma_kernel' if args.form != 'plain_f32' else 'bf_plain_kernel',
+                'kernel': dominant_kernel_name(args.form, runner),
                 'kernel_ms': round(t_run * 1e3, 3),
                 'algorithmic_flops_per_launch': alg_flops,
                 'executed_mfma_tflops': round(runner.map_bins * executed / t_run / 1e12, 2)}
@@ -1212,9 +1252,63 @@ def main():
         log('CPU baseline (oracle/cpu_ref.c, OpenMP) on a bounded sample')
         cpu = cpu_baseline(wl, args.cpu_seconds)
         log(f"CPU: {cpu['value']:.4g} map-bins/s on {cpu['cores']} threads")
+    num_grid, num_freqs = int(wl['gpos'].shape[1]), len(wl['freqs'])
+    variants = configs = serving = None
+    if not args.no_extras:
+        log('variants: every form x classic/inverse x r_diag on the same workload')
+        variants = variants_bench(wl, dev)
+        del runner
+        wl = None
+        import gc
+
+        gc.collect()
+        torch.cuda.empty_cache()
+        sub = argparse.Namespace(**{**vars(args), 'steps': 3, 'warmup': 1, 'no_compare': False})
+        log('configuration: fused CSM + beamforming (512 mics, 10 s, bs 2048, 128x128, hipGraph)')
+        fz = fused_bench(sub)
+        torch.cuda.empty_cache()
+        log('configuration: large 3D map (1024 mics, 64x64x32, 256 bins)')
+        l3 = large3d_bench(sub)
+        torch.cuda.empty_cache()
+        nm3 = l3['config']['num_mics']
+        configs = {
+            'fused_csm_beamform': {
+                'workload': fz['config']['workload'], 'map_bins_per_s': fz['value'], 'ms_per_replay': fz['ms_per_step'],
+                'max_rel_err': fz['max_rel_err'], 'csm_ms': fz['csm_ms'], 'beamform_ms': fz['beamform_ms'],
+                'roofline': {'kernel': 'bf_mfma (beamforming launch)', 'bound': 'mfma', 'unit': 'TFLOP/s',
+                             'achieved': round(fz['beamform_alg_tflops'], 2), 'peak': MI355X_PEAK['bf16'],
+                             'frac': round(fz['beamform_alg_tflops'] / MI355X_PEAK['bf16'], 4)},
+                'csm_roofline': {'kernel': 'welch_herk_kernel (fp32 MFMA, dense flop count)', 'bound': 'mfma',
+                                 'unit': 'TFLOP/s', 'achieved': round(fz['csm_tflops'], 2),
+                                 'peak': MI355X_PEAK['fp32_mfma'],
+                                 'frac': round(fz['csm_tflops'] / MI355X_PEAK['fp32_mfma'], 4)}},
+            'large3d': {
+                'workload': l3['config']['workload'], 'map_bins_per_s': l3['value'], 'ms_per_step': l3['ms_per_step'],
+                'max_rel_err': l3['max_rel_err'], 'plain_map_bins_per_s': l3['forms']['plain_f32']['map_bins_per_s'],
+                'mfma_over_plain': l3['mfma_over_plain'],
+                'roofline': {'kernel': 'bf_mfma_kernel', 'bound': 'mfma', 'unit': 'TFLOP/s', 'num_mics': nm3,
+                             'achieved': round(l3['forms']['mfma_bf16x3']['alg_tflops'], 2),
+                             'peak': MI355X_PEAK['bf16'],
+                             'frac': round(l3['forms']['mfma_bf16x3']['alg_tflops'] / MI355X_PEAK['bf16'], 4)}},
+        }
+        log('serving: mixed-size requests at 65 % of the measured saturation, against the unbatched plain kernel')
+        ssub = argparse.Namespace(**{**vars(args), 'serve_requests': 2000, 'serve_sat': 300, 'serve_plain': 100,
+                                     'serve_load': 0.65})
+        sv = serving_bench(ssub)
+        serving = {'p50_ms': sv['serving']['p50_ms'], 'p99_ms': sv['serving']['p99_ms'],
+                   'req_per_s': sv['serving']['req_per_s'], 'map_bins_per_s': sv['serving']['map_bins_per_s'],
+                   'requests': sv['config']['requests'], 'offered_req_per_s': sv['config']['offered_req_per_s'],
+                   'saturation_req_per_s': sv['saturation']['req_per_s'],
+                   'mean_batch': sv['serving']['mean_batch'],
+                   'plain_unbatched': {'saturation_req_per_s': sv['plain_unbatched']['saturation']['req_per_s'],
+                                       'p50_ms_same_load': sv['plain_unbatched']['same_offered_load_first_requests'][
+                                           'p50_ms'],
+                                       'p99_ms_same_load': sv['plain_unbatched']['same_offered_load_first_requests'][
+                                           'p99_ms']},
+                   'workload': sv['config']['workload']}
 
     line = {
-        'metric': 'map-bins/s (grid pts x freqs) at <=1e-4 rel err vs FP64',
+        'metric': 'map-bins/s (grid pts\u00d7freqs) at \u22641e-4 rel err vs FP64; serving p50/p99 latency',
         'value': value,
         'unit': 'map-bins/s',
         'n_gpus': 1,
@@ -1230,7 +1324,7 @@ def main():
         'data': 'synthetic (seeded analytic CSM)',
         'config': {'workload': 'plain vs MFMA beamforming: 256 mics (2 m disk), 8 sources (20 dB spread), '
                                'analytic c64 CSM 513 bins, RectGrid 201x201 z=0.5, classic, r_diag',
-                   'num_mics': nm, 'num_grid': int(wl['gpos'].shape[1]), 'num_freqs': len(wl['freqs']),
+                   'num_mics': nm, 'num_grid': num_grid, 'num_freqs': num_freqs,
                    'form': args.form, 'phase_mode': 'reference', 'launch_shape': tiles,
                    'parallelism': 'single GPU'},
         'max_rel_err': err,
@@ -1239,6 +1333,9 @@ def main():
         'forms_map_bins_per_s': forms,
         'roofline': roofline,
         'cpu_baseline': cpu,
+        'variants': variants,
+        'configs': configs,
+        'serving': serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -50,6 +50,27 @@ int main(void)
             CHECK(rc == ABF_OK || rc == ABF_ERR_UNSUPPORTED);
             if (rc == ABF_OK) CHECK(wgs > 0 && cols > 0 && fb > 0);
         }
+    /* every kernel request on every microphone count: the streamed kernel (6) plans for 2..8
+     * blocks of 32 microphones with the hardware sin/cos and the float32 phase, the others fall back */
+    for (int kr = 0; kr <= 6; ++kr)
+        for (int mi = 0; mi < (int)(sizeof mics / sizeof mics[0]); ++mi)
+            for (int fast = 0; fast < 2 && mics[mi] <= 1024; ++fast) {
+                abf_grid_job j = job(40401, mics[mi], 513, ABF_FORM_MFMA_BF16X3, ABF_STEER_TRUE_LEVEL);
+                j.tile_kernel = kr;
+                j.phase_mode = ABF_PHASE_REFERENCE | (fast ? ABF_PHASE_FAST_TRIG : 0);
+                int32_t kern = -1, cols = -1, fb = -1;
+                int64_t wgs = -1;
+                CHECK(abf_beamform_plan(&j, &kern, &cols, &fb, &wgs) == ABF_OK);
+                CHECK(wgs > 0 && cols > 0 && fb > 0 && kern >= 1 && kern <= 6);
+                const int mp = (mics[mi] + 31) / 32 * 32;
+                CHECK((kern == 6) == (kr == 6 && fast && mp >= 64 && mp <= 256));
+                CHECK(abf_beamform_workspace_size(&j) > 0);
+            }
+    {
+        abf_grid_job j = job(100, 64, 2, ABF_FORM_MFMA_BF16X3, ABF_STEER_CLASSIC);
+        j.tile_kernel = 7;
+        CHECK(abf_beamform_plan(&j, NULL, NULL, NULL, NULL) != ABF_OK);
+    }
     /* the C4 shape: a CSM above 2^31 bytes */
     {
         abf_grid_job j = job(131072, 1024, 256, ABF_FORM_MFMA_BF16X3, ABF_STEER_TRUE_LEVEL);
