@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, 'libabf.so')
 STREAM = [('abf_bf_mfma_stream.hip', [f'-DABF_ST_NK={nk}'], f'abf_bf_mfma_stream_{nk}.o') for nk in range(8, 1, -1)]
 SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f16.hip', 'abf_bf_mfma_f32.hip', 'abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip',
            'abf_eig.hip', 'abf_deconv.hip', 'abf_sources.hip', 'abf_clean.hip', 'abf_resample.hip',
-           'abf_api.hip']
+           'abf_welch_fft.hip', 'abf_api.hip']
 # the batched Hermitian eigensolver of the eigenvalue path; the FFTs of the signal resampling
 LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocsolver', '-lrocblas', '-lrocfft']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')

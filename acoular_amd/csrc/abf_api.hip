@@ -976,6 +976,93 @@ int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mic
 // GPU (small arrays, e.g. 64 mics x 65 bins: 130 waves walking 799 blocks each), the blocks of a
 // pass are split into up to 16 slices of at least 32 blocks, whose partial sums the workspace
 // holds (at most 256 MB) and welch_reduce_kernel adds in a fixed order.
+extern "C++" { // launch helpers of the block FFT (templates, inside the C-linkage entry points)
+// Windowed block FFT launch shape: the radix kernel (block sizes 16 ... 4096; ABF_FFT_RADIX=0
+// selects the radix-2 kernel for comparison) with the largest power-of-two pair count P that the
+// channels can use, P bs <= 16 x its threads and LDS within kFftRadixLdsMax; the radix-2 kernel
+// with at most 64 KB of LDS otherwise.
+struct FftShape {
+    bool radix;
+    int P, logP;
+    size_t lds;
+    int bpw_max; // radix kernel: at most this many consecutive blocks per workgroup
+};
+
+static FftShape fft_shape(int bs, int logbs, int M)
+{
+    FftShape f{};
+    f.radix = logbs >= kFftRadixMinLog && logbs <= kFftRadixMaxLog && env_int("ABF_FFT_RADIX", 1) != 0;
+    if (f.radix) {
+        int P = 1, lp = 0;
+        while (2 * P * bs <= 16 * kFftRadixThreads && P < (M + 1) / 2 && welch_fft_lds_bytes(bs, 2 * P) <= kFftRadixLdsMax) {
+            P *= 2;
+            ++lp;
+        }
+        f.P = P;
+        f.logP = lp;
+        f.lds = welch_fft_lds_bytes(bs, P);
+        f.bpw_max = std::max(1, env_int("ABF_FFT_BPW", 8));
+    } else {
+        // measured 2.36 vs 2.97 ms for 128 KB on 499 blocks x 512 channels of 2048 samples;
+        // 16384-sample blocks need 128 KB
+        f.P = std::max(1, std::min({env_int("ABF_FFT_P", 16), 8192 / bs, (M + 1) / 2}));
+        f.logP = 0;
+        f.lds = (size_t)f.P * bs * sizeof(float2);
+    }
+    return f;
+}
+
+template <int LOGN, bool VEC>
+static void launch_fft_radix_vec(unsigned nwg, size_t lds, hipStream_t s, const WelchFftArgs &a)
+{
+    static std::once_flag attr; // kernel attribute set once, safe under concurrent first calls
+    std::call_once(attr, [] {
+        (void)hipFuncSetAttribute((const void *)welch_fft_radix_kernel<LOGN, VEC>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFftRadixLdsMax);
+    });
+    hipLaunchKernelGGL((welch_fft_radix_kernel<LOGN, VEC>), dim3(nwg), dim3(kFftRadixThreads), lds, s, a);
+}
+
+// the 8-byte-load / 16-byte-store form where M is even and the buffers are aligned for it
+template <int LOGN> static void launch_fft_radix_one(unsigned nwg, size_t lds, hipStream_t s, const WelchFftArgs &a)
+{
+    const bool vec = !(a.M & 1) && !(reinterpret_cast<uintptr_t>(a.time) & 7) &&
+                     !(reinterpret_cast<uintptr_t>(a.spec) & 15);
+    if (vec) launch_fft_radix_vec<LOGN, true>(nwg, lds, s, a);
+    else launch_fft_radix_vec<LOGN, false>(nwg, lds, s, a);
+}
+
+// one launch of the block FFT kernel that fft_shape chose over the nbl blocks a.bl0 ... of a pass
+// and ng channel groups.  The radix kernel takes up to bpw_max consecutive blocks per workgroup
+// while at least 2048 workgroups remain (eight per CU).
+static void launch_fft(const FftShape &f, int ng, int nbl, hipStream_t s, WelchFftArgs a)
+{
+    a.nbl = nbl;
+    if (!f.radix) {
+        static std::once_flag attr;
+        std::call_once(attr, [] {
+            (void)hipFuncSetAttribute((const void *)welch_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      131072);
+        });
+        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)((int64_t)ng * nbl)), dim3(512), f.lds, s, a);
+        return;
+    }
+    a.bpw = (int)std::max<int64_t>(1, std::min<int64_t>(f.bpw_max, (int64_t)ng * nbl / 2048));
+    const unsigned nwg = (unsigned)((int64_t)ng * cdiv(nbl, a.bpw));
+    switch (a.logbs) {
+    case 4: launch_fft_radix_one<4>(nwg, f.lds, s, a); break;
+    case 5: launch_fft_radix_one<5>(nwg, f.lds, s, a); break;
+    case 6: launch_fft_radix_one<6>(nwg, f.lds, s, a); break;
+    case 7: launch_fft_radix_one<7>(nwg, f.lds, s, a); break;
+    case 8: launch_fft_radix_one<8>(nwg, f.lds, s, a); break;
+    case 9: launch_fft_radix_one<9>(nwg, f.lds, s, a); break;
+    case 10: launch_fft_radix_one<10>(nwg, f.lds, s, a); break;
+    case 11: launch_fft_radix_one<11>(nwg, f.lds, s, a); break;
+    default: launch_fft_radix_one<12>(nwg, f.lds, s, a); break;
+    }
+}
+} // extern "C++"
+
 static int welch_split(int M, int bs, int blocks_per_pass)
 {
     const int64_t F = bs / 2 + 1;
@@ -1030,13 +1117,8 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
                                reinterpret_cast<float2 *>(csm_out) + (total - 1));
     }
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
-    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
-    std::call_once(attr, [] {
-        (void)hipFuncSetAttribute((const void *)welch_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-    });
-    // channel pairs per workgroup: at most 64 KB of LDS (two workgroups per CU; measured 2.36 vs 2.97 ms
-    // for 128 KB on 499 blocks x 512 channels of 2048 samples); 16384-sample blocks need 128 KB
-    const int P = std::max(1, std::min({env_int("ABF_FFT_P", 16), 8192 / bs, (M + 1) / 2}));
+    const FftShape fsh = fft_shape(bs, logbs, M);
+    const int P = fsh.P;
     const int nT = (M + 63) / 64;
     // 64 x 64 tiles with the next step's loads in flight across the MFMAs (three waves per SIMD)
     // from 256 mics: CSM of the 512-mic, 1025-bin workload 11.8 ms against 13.9 ms for 32 x 64
@@ -1058,12 +1140,12 @@ int abf_welch_csm(const float *time, int64_t num_samples, int32_t num_mics, int3
         a.b0 = b0;
         a.nb = nb;
         a.P = P;
+        a.logP = fsh.logP;
         const int ng = (int)cdiv(M, 2 * P);
         const int per_launch = (int)std::max<int64_t>(1, kFftMaxWorkgroups / ng);
         for (int c0 = 0; c0 < nb; c0 += per_launch) {
             a.bl0 = c0;
-            hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)((int64_t)ng * std::min(per_launch, nb - c0))),
-                               dim3(512), (size_t)P * bs * sizeof(float2), s, a);
+            launch_fft(fsh, ng, std::min(per_launch, nb - c0), s, a);
         }
         // one-wave workgroups: 2^24 per launch stays far below the 2^32 work-items a dispatch counts
         const int64_t tiles = (int64_t)(herk_tr == 2 ? cdiv(M, 32) : nT) * nT * nsplit;
@@ -1111,13 +1193,8 @@ int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, in
     while ((1 << logbs) < bs) ++logbs;
     float2 *tw = reinterpret_cast<float2 *>(workspace);
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)cdiv(bs / 2, 256)), dim3(256), 0, s, tw, bs);
-    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
-    std::call_once(attr, [] {
-        (void)hipFuncSetAttribute((const void *)welch_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-    });
-    // channel pairs per workgroup: at most 64 KB of LDS (two workgroups per CU; measured 2.36 vs 2.97 ms
-    // for 128 KB on 499 blocks x 512 channels of 2048 samples); 16384-sample blocks need 128 KB
-    const int P = std::max(1, std::min({env_int("ABF_FFT_P", 16), 8192 / bs, (M + 1) / 2}));
+    const FftShape fsh = fft_shape(bs, logbs, M);
+    const int P = fsh.P;
     const int ng = (int)cdiv(M, 2 * P);
     const int per_launch = (int)std::min<int64_t>(65535, std::max<int64_t>(1, kFftMaxWorkgroups / ng));
     for (int b0 = 0; b0 < num_blocks; b0 += per_launch) {
@@ -1135,9 +1212,9 @@ int abf_rfft_blocks(const float *time, int64_t num_samples, int32_t num_mics, in
         a.b0 = b0;
         a.nb = std::min(per_launch, num_blocks - b0);
         a.P = P;
+        a.logP = fsh.logP;
         a.block_major = 1;
-        hipLaunchKernelGGL(welch_fft_kernel, dim3((unsigned)((int64_t)ng * a.nb)), dim3(512),
-                           (size_t)P * bs * sizeof(float2), s, a);
+        launch_fft(fsh, ng, a.nb, s, a);
     }
     return hip_status("block FFT launch");
 }

@@ -71,18 +71,6 @@ __global__ __launch_bounds__(256) void calc_csm_c128_kernel(double2 *__restrict_
     *p = v;
 }
 
-// Workgroups are handed to the eight XCDs in turn (workgroup w runs on XCD w % 8), and each XCD
-// has its own L2.  xcd_order(w, n) renumbers the n workgroups of a launch so that XCD x takes one
-// contiguous range of logical indices, x q + min(x, r) + w / 8 with n = 8 q + r: the neighbouring
-// channel groups of a block (which share the 128-byte lines of every time sample and spectrum
-// row) and the next, half-overlapping block then meet in one L2 instead of each XCD fetching
-// whole lines for a quarter of their bytes.  A bijection of [0, n).
-__device__ __forceinline__ int64_t xcd_order(int64_t w, int64_t n)
-{
-    const int64_t q = n >> 3, r = n & 7, x = w & 7;
-    return x * q + (x < r ? x : r) + (w >> 3);
-}
-
 __device__ __forceinline__ int bitrev(int n, int bits) { return (int)(__builtin_bitreverse32((uint32_t)n) >> (32 - bits)); }
 
 // radix-2 decimation-in-time stages over P sequences of bs points in LDS (bit-reversed order on

@@ -79,36 +79,46 @@ class RectGrid(Grid):
         pos[2] = self.z
         return pos
 
+    def _axes(self):
+        """(name, lower bound, upper bound, step) of each axis."""
+        return (('x', self.x_min, self.x_max, self.increment), ('y', self.y_min, self.y_max, self.increment))
+
+    def _snap(self, *coords, half_even=False):
+        """Per-axis step numbers of the grid points nearest to the coordinates (scalars or arrays,
+        one per axis): (c - lower bound) / step rounded half up (2D, ``grids.py:388-414``) or half
+        to even (3D, Python's ``round``, ``grids.py:566-606``).  A coordinate outside its axis
+        range raises ValueError."""
+        steps = []
+        for (name, lo, hi, inc), c in zip(self._axes(), coords):
+            c = np.asarray(c, dtype=float)
+            outside = (c < lo) | (c > hi)
+            if outside.any():
+                msg = f'{name} = {np.ravel(c[outside])[0]:g} lies outside the grid range [{lo:g}, {hi:g}]'
+                raise ValueError(msg)
+            t = (c - lo) / inc
+            steps.append((np.round(t) if half_even else np.floor(t + 0.5)).astype(np.int64))
+        return steps
+
     def index(self, x, y):
         """Indices of the grid point nearest to (x, y) (``grids.py:388-414``)."""
-        if x < self.x_min or x > self.x_max:
-            msg = 'x-value out of range'
-            raise ValueError(msg)
-        if y < self.y_min or y > self.y_max:
-            msg = 'y-value out of range'
-            raise ValueError(msg)
-        xi = int((x - self.x_min) / self.increment + 0.5)
-        yi = int((y - self.y_min) / self.increment + 0.5)
-        return xi, yi
+        return tuple(int(i) for i in self._snap(x, y))
 
     def indices(self, *r):
-        """Rectangle (4 values) or circle (3 values) sub-domain (``grids.py:416-473``)."""
-        if len(r) == 3:
-            xpos = self.pos
-            dr2 = (xpos[0, :] - r[0]) ** 2 + (xpos[1, :] - r[1]) ** 2
-            inds = dr2 <= r[2] ** 2
-            xis, yis = [], []
-            for n_p in np.arange(self.size)[inds]:
-                xi, yi = self.index(xpos[0, n_p], xpos[1, n_p])
-                xis.append(xi)
-                yis.append(yi)
-            if not (xis and yis):
-                return self.index(r[0], r[1])
-            return np.array(xis), np.array(yis)
+        """Rectangle (x1, y1, x2, y2) or circle (cx, cy, radius) sub-domain (``grids.py:416-473``).
+
+        A rectangle gives one slice per axis between the points nearest to its corners; a circle
+        gives the index arrays of every grid point within the radius (the nearest point to the
+        centre when none is)."""
         if len(r) == 4:
-            xi1, yi1 = self.index(min(r[0], r[2]), min(r[1], r[3]))
-            xi2, yi2 = self.index(max(r[0], r[2]), max(r[1], r[3]))
-            return np.s_[xi1 : xi2 + 1], np.s_[yi1 : yi2 + 1]
+            lo = self.index(min(r[0], r[2]), min(r[1], r[3]))
+            hi = self.index(max(r[0], r[2]), max(r[1], r[3]))
+            return tuple(np.s_[a : b + 1] for a, b in zip(lo, hi))
+        if len(r) == 3:
+            pos = self.pos
+            inside = np.flatnonzero((pos[0] - r[0]) ** 2 + (pos[1] - r[1]) ** 2 <= r[2] ** 2)
+            if inside.size == 0:
+                return self.index(r[0], r[1])
+            return tuple(self._snap(pos[0, inside], pos[1, inside]))
         msg = 'polygon sub-domains are not part of this build'
         raise NotImplementedError(msg)
 
@@ -157,29 +167,20 @@ class RectGrid3D(RectGrid):
         pos[2] = np.tile(z, nx * ny)
         return pos
 
+    def _axes(self):
+        return tuple((name, lo, hi, self._inc(a)) for a, (name, lo, hi) in enumerate(
+            (('x', self.x_min, self.x_max), ('y', self.y_min, self.y_max), ('z', self.z_min, self.z_max))))
+
     def index(self, x, y, z):
         """Indices of the grid point nearest to (x, y, z) (``grids.py:566-606``)."""
-        if x < self.x_min or x > self.x_max:
-            msg = f'x-value out of range {x:f} ({self.x_min:f}, {self.x_max:f})'
-            raise ValueError(msg)
-        if y < self.y_min or y > self.y_max:
-            msg = f'y-value out of range {y:f} ({self.y_min:f}, {self.y_max:f})'
-            raise ValueError(msg)
-        if z < self.z_min or z > self.z_max:
-            msg = f'z-value out of range {z:f} ({self.z_min:f}, {self.z_max:f})'
-            raise ValueError(msg)
-        inc = self.increment
-        incx, incy, incz = (inc, inc, inc) if np.isscalar(inc) else tuple(inc)
-        xi = round((x - self.x_min) / incx)
-        yi = round((y - self.y_min) / incy)
-        zi = round((z - self.z_min) / incz)
-        return xi, yi, zi
+        return tuple(int(i) for i in self._snap(x, y, z, half_even=True))
 
     def indices(self, x1, y1, z1, x2, y2, z2):
-        """Box sub-domain (``grids.py:608-650``)."""
-        xi1, yi1, zi1 = self.index(min(x1, x2), min(y1, y2), min(z1, z2))
-        xi2, yi2, zi2 = self.index(max(x1, x2), max(y1, y2), max(z1, z2))
-        return np.s_[xi1 : xi2 + 1], np.s_[yi1 : yi2 + 1], np.s_[zi1 : zi2 + 1]
+        """Box sub-domain: one slice per axis between the points nearest to two opposite corners
+        (``grids.py:608-650``)."""
+        lo = self.index(min(x1, x2), min(y1, y2), min(z1, z2))
+        hi = self.index(max(x1, x2), max(y1, y2), max(z1, z2))
+        return tuple(np.s_[a : b + 1] for a, b in zip(lo, hi))
 
 
 class ImportGrid(Grid):

@@ -79,7 +79,15 @@ class BaseSpectra(HasParams):
 
 
 class PowerSpectra(BaseSpectra):
-    """Welch CSM of multichannel time data (``spectra.py:187-588``)."""
+    """Welch CSM of multichannel time data (``spectra.py:187-588``).
+
+    Precision: the GPU path transforms and accumulates in complex64 (fp32 FFT, fp32 MFMA CSM
+    update) whatever ``precision`` says; ``precision='complex128'`` (the default, as in
+    ``spectra.py:119``) returns that CSM widened to complex128.  Against the float64 Welch
+    estimate the result stays within 1e-5 of the CSM's largest element and within 1e-5 relative
+    on every auto-spectrum (64 microphones, 1 s at 51.2 kHz, block size 128, Hanning, 50 %),
+    and it is exactly Hermitian.
+    """
 
     _params = ('source', 'window', 'overlap', 'block_size', 'precision')
 

@@ -838,7 +838,7 @@ def rfft_bench(args):
     t_k = event_time(step, 5)
     alg_bytes = n * nch * 4.0 + nb * nf * nch * 8.0
     roof = {'bound': 'hbm', 'achieved': round(alg_bytes / t_k / 1e9, 1), 'peak': 8000.0, 'unit': 'GB/s',
-            'kernel': 'welch_fft_kernel (block-major output)', 'algorithmic_bytes_per_launch': alg_bytes,
+            'kernel': 'welch_fft_radix_kernel (block-major output)', 'algorithmic_bytes_per_launch': alg_bytes,
             'traffic': None}
     roof['frac'] = round(roof['achieved'] / roof['peak'], 4)
     values = nb * nf * nch

@@ -174,3 +174,37 @@ def test_calc_csm_rejects_more_mics_than_a_grid_row_holds():
     refused before any device work."""
     rc = _lib.lib().abf_calc_csm(ctypes.c_void_p(8), ctypes.c_void_p(8), 1, 70000, 0, None)
     assert rc == _lib.ABF_ERR_UNSUPPORTED
+
+
+def test_rect_grid_index_and_indices_against_brute_force():
+    """RectGrid.index is the nearest grid point (brute force over every point, off the midway
+    ties); a circle sub-domain holds exactly the points within the radius; a rectangle's slices
+    span the points nearest to its corners; coordinates outside the grid raise ValueError."""
+    g = aa.RectGrid(x_min=-0.3, x_max=0.5, y_min=-0.2, y_max=0.2, z=0.4, increment=0.05)
+    nx, ny = g.shape
+    rng = np.random.default_rng(11)
+    for x, y in rng.uniform([-0.3, -0.2], [0.5, 0.2], (200, 2)):
+        d2 = (g.pos[0] - x) ** 2 + (g.pos[1] - y) ** 2
+        assert g.index(x, y) == divmod(int(np.argmin(d2)), ny)
+    for cx, cy, rad in ((0.1, 0.0, 0.12), (-0.3, -0.2, 0.07), (0.49, 0.19, 0.2), (0.12, 0.03, 0.001)):
+        got = g.indices(cx, cy, rad)
+        flat = np.flatnonzero((g.pos[0] - cx) ** 2 + (g.pos[1] - cy) ** 2 <= rad**2)
+        if flat.size == 0:
+            assert got == g.index(cx, cy)
+            continue
+        ix, iy = got
+        assert ix.dtype.kind == 'i' and sorted(zip(ix.tolist(), iy.tolist())) == sorted(divmod(int(f), ny) for f in flat)
+    sx, sy = g.indices(0.2, -0.1, -0.07, 0.13)
+    assert (sx, sy) == (np.s_[g.index(-0.07, 0.0)[0]: g.index(0.2, 0.0)[0] + 1],
+                        np.s_[g.index(0.0, -0.1)[1]: g.index(0.0, 0.13)[1] + 1])
+    sub = g.pos.reshape(3, nx, ny)[:, sx, sy]
+    assert sub[0].min() >= -0.07 - 0.025 and sub[0].max() <= 0.2 + 0.025
+    assert sub[1].min() >= -0.1 - 0.025 and sub[1].max() <= 0.13 + 0.025
+    for bad in ((0.51, 0.0), (0.0, -0.21)):
+        with pytest.raises(ValueError):
+            g.index(*bad)
+    g3 = aa.RectGrid3D(x_min=0, x_max=1, y_min=0, y_max=2, z_min=-1, z_max=1, increment=0.5)
+    assert g3.index(0.25, 1.75, 0.0) == (0, 4, 2)  # half to even on x (0.5 -> 0), y (3.5 -> 4)
+    assert g3.indices(1.0, 0.0, 1.0, 0.0, 2.0, -1.0) == (np.s_[0:3], np.s_[0:5], np.s_[0:5])
+    with pytest.raises(ValueError):
+        g3.index(0.0, 0.0, 1.5)
