@@ -343,6 +343,27 @@ void launch_mfma_group(int prec, int CW, int TT, bool pf, bool ws, int var, cons
     else launch_mfma_var<kPrF32>(var, CW, TT, pf, ws, dev, n, wgs, lds, s);
 }
 
+// The plain kernel with microphone chunks of JC (ABF_PLAIN_JC = 64, 128 or 256): JC x 512
+// bytes of LDS per workgroup of 64 grid points; grid (ceil(G / 64), F).
+template <int JC> static void launch_plain_jc(const PlainArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    static std::once_flag attr; // kernel attribute set once, safe under concurrent first calls
+    std::call_once(attr, [] {
+        (void)hipFuncSetAttribute((const void *)bf_plain_kernel<JC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  JC * 64 * (int)sizeof(float2));
+    });
+    const size_t lds = (size_t)std::min(JC, a.Mp) * 64 * sizeof(float2);
+    hipLaunchKernelGGL(bf_plain_kernel<JC>, dim3(gx, gy), dim3(256), lds, s, a);
+}
+
+static void launch_plain(const PlainArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    const int jc = env_int("ABF_PLAIN_JC", kPlainDefaultJC);
+    if (jc <= 64) launch_plain_jc<64>(a, gx, gy, s);
+    else if (jc <= 128) launch_plain_jc<128>(a, gx, gy, s);
+    else launch_plain_jc<256>(a, gx, gy, s);
+}
+
 void prep_plain(const abf_grid_job *j, void *ws, hipStream_t s)
 {
     const int M = j->geom.num_mics, Mp = round_up(M, 16), F = j->num_freqs;
@@ -372,12 +393,7 @@ int run_plain_only(const abf_grid_job *j, void *ws, hipStream_t s)
     a.phase_mode = j->phase_mode & kPhaseMask;
     a.fast_trig = (j->phase_mode & kFastTrigFlag) != 0;
     a.clip = j->clip_negative;
-    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
-    std::call_once(attr, [] {
-        (void)hipFuncSetAttribute((const void *)bf_plain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-    });
-    const size_t lds = (size_t)std::min(256, Mp) * 64 * sizeof(float2);
-    hipLaunchKernelGGL(bf_plain_kernel, dim3((unsigned)cdiv(G, 64), F), dim3(256), lds, s, a);
+    launch_plain(a, (unsigned)cdiv(G, 64), (unsigned)F, s);
     return hip_status("plain beamformer launch");
 }
 
@@ -713,12 +729,7 @@ int abf_beamform_table(const void *csm, const double *r0, const double *rm, int6
     a.phase_mode = phase_mode & kPhaseMask;
     a.fast_trig = (phase_mode & kFastTrigFlag) != 0;
     a.clip = 0;
-    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
-    std::call_once(attr, [] {
-        (void)hipFuncSetAttribute((const void *)bf_plain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-    });
-    const size_t lds = (size_t)std::min(256, Mp) * 64 * sizeof(float2);
-    hipLaunchKernelGGL(bf_plain_kernel, dim3((unsigned)cdiv(num_grid, 64), 1), dim3(256), lds, s, a);
+    launch_plain(a, (unsigned)cdiv(num_grid, 64), 1u, s);
     return hip_status("table beamformer launch");
 }
 

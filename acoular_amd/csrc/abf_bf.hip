@@ -10,8 +10,9 @@
 // the upper triangle and the diagonal of C like the reference does.
 //
 // Two forms:
-//  * plain FP32 (bf_plain_kernel): one grid point per lane, U streamed as wave-uniform
-//    (scalar) loads, the steering vectors of 64 grid points staged in LDS per mic chunk;
+//  * plain FP32 (bf_plain_kernel): one grid point per lane, U streamed through the scalar
+//    cache as wave-uniform SGPR operands, the steering vectors of 64 grid points staged in LDS
+//    per mic chunk;
 //  * MFMA complex GEMM (bf_mfma_kernel): Y = U X on 16x16x32 bf16 MFMA with both operands
 //    split hi+lo (3 products per real product) or on 16x16x4 fp32 MFMA; X (the steering
 //    matrix of a grid tile) lives in LDS, U is pre-laid-out in MFMA fragment order by
@@ -47,7 +48,6 @@ __global__ void prep_plain_kernel(const float2 *__restrict__ csm, int M, int Mp,
 // -------------------------------------------------------------------------------------------
 constexpr int kPlainNP = 64;   // grid points per workgroup (one per lane)
 constexpr int kPlainRB = 16;   // rows of U per register block
-constexpr int kPlainJC = 256;  // microphones per LDS chunk
 constexpr int kPlainWaves = 4;
 
 struct PlainPoint {
@@ -71,9 +71,21 @@ __device__ __forceinline__ float2 plain_steer(const PlainArgs &a, const PlainPoi
     return make_float2(w * c, -w * s);
 }
 
+// One grid point per lane, 64 per workgroup of 4 waves.  The microphones are taken in chunks of
+// JC: the steering vectors of a chunk are generated into LDS ([JC][64] float2), then every row
+// block of 16 rows i0 ... with i0 < chunk end accumulates (U v)_i over the chunk's columns
+// j >= i; rows of earlier chunks take their v_i from a fresh evaluation.  The 16 values U_ij of
+// row block i0 and column j (one 128-byte row of U^T) are wave-uniform: scalar loads into
+// SGPRs, the operands of the FMAs, one column ahead of their use.  JC sets the LDS per workgroup
+// (JC x 512 bytes) and so the workgroups per CU: JC = 64 takes 4 per CU at the cost of
+// evaluating the steering vectors of earlier rows again (on 256 mics, 201 x 201, 513 bins:
+// 99.6 ms for JC = 64, 105.8 ms for 128; the same loop with vector loads of U by every lane,
+// which the L1 then serves 64 times over, 172 and 163 ms; scalar loads waited for at once,
+// with JC = 256, 197 ms).
+template <int JC>
 __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
 {
-    extern __shared__ float2 vlds[]; // [kPlainJC][kPlainNP]
+    extern __shared__ float2 vlds[]; // [JC][kPlainNP]
     __shared__ double s_help[kPlainWaves][kPlainNP];
     __shared__ float s_q[kPlainWaves][kPlainNP];
 
@@ -94,8 +106,8 @@ __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
     double help = 0.0;
     float q = 0.f;
 
-    for (int c0 = 0; c0 < a.Mp; c0 += kPlainJC) {
-        const int jn = min(kPlainJC, a.Mp - c0);
+    for (int c0 = 0; c0 < a.Mp; c0 += JC) {
+        const int jn = min(JC, a.Mp - c0);
         __syncthreads();
         for (int mm = wave; mm < jn; mm += kPlainWaves) {
             double h;
@@ -110,34 +122,77 @@ __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
             const int rb = rd * kPlainWaves + ((rd & 1) ? (kPlainWaves - 1 - wave) : wave);
             if (rb >= nrb) continue;
             const int i0 = rb * kPlainRB;
-            const int js = max(i0, c0);
+            const int js = max(i0, c0), je = c0 + jn;
             float ar[kPlainRB], ai[kPlainRB];
 #pragma unroll
             for (int r = 0; r < kPlainRB; ++r) ar[r] = ai[r] = 0.f;
-            for (int j = js; j < c0 + jn; ++j) {
-                const float2 vj = vlds[(j - c0) * kPlainNP + lane];
-                const float2 *__restrict__ u = ut + (int64_t)j * a.Mp + i0;
+            // U through the scalar cache, one column ahead: the wait at the top of a column
+            // completes column j's U row and v_j; column j + 1's loads are issued after it and
+            // complete during column j's 64 FMAs (scalar loads return out of order, so any wait
+            // for one of them is a wait for all: hence the explicit wait before the next loads)
+            // (U^T is read-only in this kernel: read through the constant address space, the
+            // loads stay scalar loads across the explicit waits)
+            typedef const __attribute__((address_space(4))) float *cptr;
+            auto ldu = [&](int j, float2 (&d)[kPlainRB]) __attribute__((always_inline)) {
+                const cptr u = (cptr)(ut + (int64_t)j * a.Mp + i0);
+#pragma unroll
+                for (int r = 0; r < kPlainRB; ++r) d[r] = make_float2(u[2 * r], u[2 * r + 1]);
+            };
+            // ping-pong between two register sets, two columns per step: no copy between them
+            // (a copy of a loaded value would wait for it at once)
+            float2 ua[kPlainRB], ub[kPlainRB];
+            float2 va = make_float2(0.f, 0.f), vb = make_float2(0.f, 0.f);
+            auto fmas = [&](const float2 (&u)[kPlainRB], float2 v) __attribute__((always_inline)) {
 #pragma unroll
                 for (int r = 0; r < kPlainRB; ++r) {
-                    const float2 uu = u[r];
-                    ar[r] = fmaf(uu.x, vj.x, ar[r]);
-                    ar[r] = fmaf(-uu.y, vj.y, ar[r]);
-                    ai[r] = fmaf(uu.x, vj.y, ai[r]);
-                    ai[r] = fmaf(uu.y, vj.x, ai[r]);
+                    ar[r] = fmaf(u[r].x, v.x, ar[r]);
+                    ar[r] = fmaf(-u[r].y, v.y, ar[r]);
+                    ai[r] = fmaf(u[r].x, v.y, ai[r]);
+                    ai[r] = fmaf(u[r].y, v.x, ai[r]);
                 }
+            };
+            if (js < je) {
+                ldu(js, ua);
+                va = vlds[(js - c0) * kPlainNP + lane];
             }
+            for (int j = js; j < je; j += 2) {
+                __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0): column j in ua, va
+                __builtin_amdgcn_sched_barrier(0);
+                const int j1 = min(j + 1, je - 1), j2 = min(j + 2, je - 1);
+                ldu(j1, ub);
+                vb = vlds[(j1 - c0) * kPlainNP + lane];
+                fmas(ua, va);
+                __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_s_waitcnt(0xc07f); // column j + 1 in ub, vb
+                __builtin_amdgcn_sched_barrier(0);
+                ldu(j2, ua);
+                va = vlds[(j2 - c0) * kPlainNP + lane];
+                fmas(ub, j + 1 < je ? vb : make_float2(0.f, 0.f)); // past the end: adds 0
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // a row block lies either inside the chunk (its v_i in LDS) or before it (i0 + 16 <= c0)
+            if (i0 >= c0) {
 #pragma unroll
-            for (int r = 0; r < kPlainRB; ++r) {
-                const int i = i0 + r;
-                float2 vi;
-                if (i >= c0) {
-                    vi = vlds[(i - c0) * kPlainNP + lane];
-                } else {
-                    double h;
-                    vi = plain_steer(a, p, i, k, h);
+                for (int r = 0; r < kPlainRB; ++r) {
+                    const float2 vi = vlds[(i0 + r - c0) * kPlainNP + lane];
+                    q = fmaf(vi.x, ar[r], q);
+                    q = fmaf(vi.y, ai[r], q);
                 }
-                q = fmaf(vi.x, ar[r], q);
-                q = fmaf(vi.y, ai[r], q);
+            } else {
+                // one row at a time (the accumulators shift down), so that one steering
+                // evaluation is live at once
+#pragma unroll 1
+                for (int r = 0; r < kPlainRB; ++r) {
+                    double h;
+                    const float2 vi = plain_steer(a, p, i0 + r, k, h);
+                    q = fmaf(vi.x, ar[0], q);
+                    q = fmaf(vi.y, ai[0], q);
+#pragma unroll
+                    for (int t = 0; t + 1 < kPlainRB; ++t) {
+                        ar[t] = ar[t + 1];
+                        ai[t] = ai[t + 1];
+                    }
+                }
             }
         }
     }
@@ -166,6 +221,10 @@ __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
         if (a.out_norm) a.out_norm[(int64_t)f * a.G + p.g] = (float)nst;
     }
 }
+
+template __global__ void bf_plain_kernel<64>(PlainArgs);
+template __global__ void bf_plain_kernel<128>(PlainArgs);
+template __global__ void bf_plain_kernel<256>(PlainArgs);
 
 // -------------------------------------------------------------------------------------------
 // float64 form: the plain kernel's structure with float64 steering vectors, CSM and sums, for

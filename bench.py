@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MI355X_PEAK = {'bf16': 2500.0, 'fp16': 2500.0, 'fp32_mfma': 157.3, 'fp32': 157.3}  # TFLOP/s, dense
+# FP32 VALU without packed instructions (disabled in this build): 256 CUs x 4 SIMDs x 16 lanes x
+# 2 flops x 2.4 GHz
+FP32_VALU_NO_PACKED = 78.6
 
 
 def log(msg):
@@ -1238,6 +1241,7 @@ def main():
         f"{errs['point_rel_err_above_1pct_peak']:.2e})")
 
     forms = {args.form: value}
+    plain = None
     if not args.no_compare:
         for other in ('mfma_f32', 'plain_f32', 'mfma_bf16x3', 'mfma_f16x2'):
             if other == args.form:
@@ -1246,6 +1250,17 @@ def main():
             t = time_steps(r.step, 2, 1)
             forms[other] = r.map_bins / t
             log(f'form {other}: {forms[other]:.4g} map-bins/s')
+            if other == 'plain_f32':
+                r.prepare()
+                tk = event_time(r.run, 2)
+                ex = r.map_bins * executed_flops_per_bin('plain_f32', nm) / tk / 1e12
+                plain = {'kernel': 'bf_plain_kernel', 'kernel_ms': round(tk * 1e3, 2),
+                         'executed_tflops': round(ex, 2), 'valu_ceiling_tflops': FP32_VALU_NO_PACKED,
+                         'frac_of_valu_ceiling': round(ex / FP32_VALU_NO_PACKED, 4),
+                         'algorithmic_tflops': round(r.map_bins * 8.0 * nm * (nm + 1) / tk / 1e12, 2),
+                         'mfma_over_plain': round(value / forms[other], 2)}
+                log(f'plain kernel {tk * 1e3:.1f} ms: {ex:.1f} TFLOP/s executed '
+                    f'({ex / FP32_VALU_NO_PACKED:.1%} of the non-packed FP32 VALU ceiling)')
             del r
     cpu = None
     if not args.no_cpu:
@@ -1331,6 +1346,7 @@ def main():
         'max_rel_err_vs_fp64_phase': errs['max_rel_err_vs_fp64_phase'],
         'point_rel_err_above_1pct_peak': errs['point_rel_err_above_1pct_peak'],
         'forms_map_bins_per_s': forms,
+        'plain_baseline': plain,
         'roofline': roofline,
         'cpu_baseline': cpu,
         'variants': variants,
