@@ -26,7 +26,7 @@ import threading
 
 from . import _lib
 
-KERNELS = (1, 2, 3, 4, 5, 6)  # enum abf_mfma_kernel (include/abf.h)
+KERNELS = (1, 2, 3, 4, 5, 6, 7, 8)  # enum abf_mfma_kernel (include/abf.h)
 SHIPPED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tiles_gfx950.json')
 TILE_COLS = (16, 32, 64, 128)
 TILE_FREQS = (2, 4, 8, 16, 32, 64)

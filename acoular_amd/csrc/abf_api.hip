@@ -1,6 +1,7 @@
 // C-ABI entry points of libabf.so (declared in include/abf.h).  Host-side planning,
 // validation and launches; the kernels live in abf_bf.hip, abf_steer.hip and abf_csm.hip.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -64,6 +65,8 @@ struct MfPlan {
     bool pf; // software-pipelined fragment loads (split-bf16 alternating kernel)
     bool ws; // warp-specialised kernel (double-buffered steering matrix)
     bool st; // streamed kernel (steering blocks through an LDS ring)
+    bool wide; // streamed kernel with two column tiles per wave
+    bool ps;   // streamed kernel, persistent launch over a work-item counter
     int64_t ntiles, wgs;
     size_t prep_bytes, lds;
     size_t uscale_off; // f16x2: offset of the per-frequency U scales in the job's workspace
@@ -123,19 +126,25 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     p.CT = 0;
     p.ws = false;
     p.st = false;
-    if (kreq == ABF_MFMA_KERNEL_STREAMED) {
+    p.wide = false;
+    p.ps = false;
+    if (kreq == ABF_MFMA_KERNEL_STREAMED || kreq == ABF_MFMA_KERNEL_STREAMED_WIDE ||
+        kreq == ABF_MFMA_KERNEL_STREAMED_PERSISTENT) {
         // split-bf16, 2..8 blocks of 32 microphones, the reference's float32 phase with the hardware
         // sin/cos; anything else runs the row-pair kernel
         if (p.prec == kPrBF16X3 && p.Mp >= 64 && p.Mp <= 256 && (j->phase_mode & kFastTrigFlag) &&
             (j->phase_mode & kPhaseMask) == ABF_PHASE_REFERENCE) {
             p.st = true;
+            p.wide = kreq == ABF_MFMA_KERNEL_STREAMED_WIDE;
+            p.ps = kreq == ABF_MFMA_KERNEL_STREAMED_PERSISTENT;
+            static_assert(kStBN == kWdBN, "both streamed kernels tile 64 grid points");
             p.CT = kStBN / 16;
             p.TT = 1;
             p.CW = 1;
             p.pf = false;
             p.ntiles = cdiv(j->geom.num_grid, kStBN);
             p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
-            p.lds = st_lds_bytes();
+            p.lds = p.wide ? wd_lds_bytes() : st_lds_bytes();
             return ABF_OK;
         }
         kreq = ABF_MFMA_KERNEL_ROW_PAIRS;
@@ -192,7 +201,7 @@ int validate_job(const abf_grid_job *j)
     if (int r = check_steer(j->steer_type)) return r;
     if (int r = check_phase(j->phase_mode)) return r;
     if (j->form < ABF_FORM_PLAIN_F32 || j->form > ABF_FORM_MFMA_F16X2) return fail(ABF_ERR_INVALID, "unknown form");
-    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_STREAMED)
+    if (j->tile_kernel < ABF_MFMA_KERNEL_AUTO || j->tile_kernel > ABF_MFMA_KERNEL_STREAMED_PERSISTENT)
         return fail(ABF_ERR_INVALID, "unknown MFMA kernel variant");
     if (j->tile_cols < 0 || j->tile_freqs < 0) return fail(ABF_ERR_INVALID, "negative tile size");
     return ABF_OK;
@@ -221,12 +230,24 @@ constexpr int kPack = 16;
 struct JobPack {
     MfJob jobs[kPack];
     int n;
+    unsigned *counter; // work-item counter of a persistent launch to zero, or null
 };
 
+// job descriptors (and the zeroed work-item counter of a persistent launch) from kernel arguments:
+// no host copy, no memset node, so the sequence can be captured in a graph
 __global__ void write_jobs_kernel(JobPack pack, MfJob *dst)
 {
     const int i = threadIdx.x;
     if (i < pack.n) dst[i] = pack.jobs[i];
+    if (i == 0 && pack.counter) *pack.counter = 0u;
+}
+
+// batch workspace head: the job descriptors, then one work-item counter per job (a launch group
+// holds at least one job, so there is one per group)
+size_t desc_bytes(int num_jobs)
+{
+    const size_t n = (size_t)std::max(num_jobs, 1);
+    return align256(n * sizeof(MfJob)) + align256(n * sizeof(unsigned));
 }
 
 __global__ void set_double_kernel(double *d, double v)
@@ -312,26 +333,76 @@ void launch_mfma_var(int var, int CW, int TT, bool pf, bool ws, const MfJob *dev
     }
 }
 
-template <int NK, int WK> void launch_stream(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+// compute units of the current device (looked up once per device)
+int cu_count()
+{
+    static std::atomic<int> cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cus[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// counter == null: one work item per workgroup; otherwise a persistent grid of one workgroup per CU
+// (the kernel's LDS allows one) over the wgs items
+template <int NK, int WK, bool PS>
+void launch_stream_ps(const MfJob *dev, int n, int64_t wgs, size_t lds, unsigned *counter, hipStream_t s)
 {
     static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
     std::call_once(attr, [] {
-        (void)hipFuncSetAttribute((const void *)bf_mfma_stream_kernel<NK, WK>,
+        (void)hipFuncSetAttribute((const void *)bf_mfma_stream_kernel<NK, WK, PS>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
     });
-    hipLaunchKernelGGL((bf_mfma_stream_kernel<NK, WK>), dim3((unsigned)wgs), dim3(kStWaves * 64), lds, s, dev, n);
+    const int64_t grid = PS ? std::min<int64_t>(wgs, cu_count()) : wgs;
+    hipLaunchKernelGGL((bf_mfma_stream_kernel<NK, WK, PS>), dim3((unsigned)grid), dim3(kStWaves * 64), lds, s, dev,
+                       n, wgs, counter);
 }
 
-template <int WK> void launch_stream_nk(int nk, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+template <int NK, int WK>
+void launch_stream(const MfJob *dev, int n, int64_t wgs, size_t lds, unsigned *counter, hipStream_t s)
+{
+    if (counter) launch_stream_ps<NK, WK, true>(dev, n, wgs, lds, counter, s);
+    else launch_stream_ps<NK, WK, false>(dev, n, wgs, lds, nullptr, s);
+}
+
+template <int WK>
+void launch_stream_nk(int nk, const MfJob *dev, int n, int64_t wgs, size_t lds, unsigned *counter, hipStream_t s)
 {
     switch (nk) {
-    case 2: launch_stream<2, WK>(dev, n, wgs, lds, s); break;
-    case 3: launch_stream<3, WK>(dev, n, wgs, lds, s); break;
-    case 4: launch_stream<4, WK>(dev, n, wgs, lds, s); break;
-    case 5: launch_stream<5, WK>(dev, n, wgs, lds, s); break;
-    case 6: launch_stream<6, WK>(dev, n, wgs, lds, s); break;
-    case 7: launch_stream<7, WK>(dev, n, wgs, lds, s); break;
-    default: launch_stream<8, WK>(dev, n, wgs, lds, s); break;
+    case 2: launch_stream<2, WK>(dev, n, wgs, lds, counter, s); break;
+    case 3: launch_stream<3, WK>(dev, n, wgs, lds, counter, s); break;
+    case 4: launch_stream<4, WK>(dev, n, wgs, lds, counter, s); break;
+    case 5: launch_stream<5, WK>(dev, n, wgs, lds, counter, s); break;
+    case 6: launch_stream<6, WK>(dev, n, wgs, lds, counter, s); break;
+    case 7: launch_stream<7, WK>(dev, n, wgs, lds, counter, s); break;
+    default: launch_stream<8, WK>(dev, n, wgs, lds, counter, s); break;
+    }
+}
+
+template <int NK, int WK> void launch_wide(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+{
+    static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
+    std::call_once(attr, [] {
+        (void)hipFuncSetAttribute((const void *)bf_mfma_wide_kernel<NK, WK>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
+    });
+    hipLaunchKernelGGL((bf_mfma_wide_kernel<NK, WK>), dim3((unsigned)wgs), dim3(kWdWaves * 64), lds, s, dev, n);
+}
+
+template <int WK> void launch_wide_nk(int nk, const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
+{
+    switch (nk) {
+    case 2: launch_wide<2, WK>(dev, n, wgs, lds, s); break;
+    case 3: launch_wide<3, WK>(dev, n, wgs, lds, s); break;
+    case 4: launch_wide<4, WK>(dev, n, wgs, lds, s); break;
+    case 5: launch_wide<5, WK>(dev, n, wgs, lds, s); break;
+    case 6: launch_wide<6, WK>(dev, n, wgs, lds, s); break;
+    case 7: launch_wide<7, WK>(dev, n, wgs, lds, s); break;
+    default: launch_wide<8, WK>(dev, n, wgs, lds, s); break;
     }
 }
 
@@ -471,7 +542,9 @@ int abf_beamform_plan(const abf_grid_job *job, int32_t *tile_kernel, int32_t *ti
     } else {
         MfPlan p;
         if (int r = plan_mfma(job, p)) return r;
-        if (p.st) k = ABF_MFMA_KERNEL_STREAMED;
+        if (p.st)
+            k = p.wide ? ABF_MFMA_KERNEL_STREAMED_WIDE
+                       : (p.ps ? ABF_MFMA_KERNEL_STREAMED_PERSISTENT : ABF_MFMA_KERNEL_STREAMED);
         else if (p.ws) k = ABF_MFMA_KERNEL_SPECIALISED;
         else if (p.pf) k = p.TT == 2 ? ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS : ABF_MFMA_KERNEL_PIPELINED;
         else k = p.TT == 2 ? ABF_MFMA_KERNEL_ROW_PAIRS : ABF_MFMA_KERNEL_ALTERNATING;
@@ -488,7 +561,7 @@ int abf_beamform_plan(const abf_grid_job *job, int32_t *tile_kernel, int32_t *ti
 
 size_t abf_beamform_batch_workspace_size(const abf_grid_job *jobs, int32_t num_jobs, size_t *offsets)
 {
-    size_t off = align256((size_t)std::max(num_jobs, 1) * sizeof(MfJob));
+    size_t off = desc_bytes(num_jobs);
     for (int i = 0; i < num_jobs; ++i) {
         if (validate_job(&jobs[i])) return 0;
         if (offsets) offsets[i] = off;
@@ -523,7 +596,7 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         bool pf, ws;
         int var;
         std::vector<int> idx;
-        bool st;
+        bool st, wide, ps;
         int nk; // streamed kernel: blocks of 32 microphones
     };
     std::vector<Group> groups;
@@ -546,18 +619,22 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
                                [&](const Group &g) {
                                    return g.prec == plans[i].prec && g.CW == plans[i].CW && g.TT == plans[i].TT &&
                                           g.pf == plans[i].pf && g.ws == plans[i].ws && g.st == plans[i].st &&
-                                          g.nk == nk && g.var == mfma_variant(&jobs[i]);
+                                          g.wide == plans[i].wide && g.ps == plans[i].ps && g.nk == nk &&
+                                          g.var == mfma_variant(&jobs[i]);
                                });
         if (it == groups.end()) {
             groups.push_back({plans[i].prec, plans[i].CW, plans[i].TT, plans[i].pf, plans[i].ws,
-                              mfma_variant(&jobs[i]), {}, plans[i].st, nk});
+                              mfma_variant(&jobs[i]), {}, plans[i].st, plans[i].wide, plans[i].ps, nk});
             it = groups.end() - 1;
         }
         it->idx.push_back(i);
     }
     if (int r = hip_status("MFMA re-layout launch")) return r;
     int desc_base = 0;
-    for (const Group &gr : groups) {
+    unsigned *counters = reinterpret_cast<unsigned *>(ws + align256((size_t)std::max(num_jobs, 1) * sizeof(MfJob)));
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const Group &gr = groups[gi];
+        unsigned *counter = gr.ps ? counters + gi : nullptr;
         int64_t wg = 0;
         size_t lds = 0;
         std::vector<MfJob> descs;
@@ -569,14 +646,22 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         for (size_t b = 0; b < descs.size(); b += kPack) {
             JobPack pk{};
             pk.n = (int)std::min<size_t>(kPack, descs.size() - b);
+            pk.counter = b == 0 ? counter : nullptr;
             for (int q = 0; q < pk.n; ++q) pk.jobs[q] = descs[b + q];
             hipLaunchKernelGGL(write_jobs_kernel, dim3(1), dim3(64), 0, s, pk, dev_jobs + desc_base + b);
         }
         if (gr.st) {
             const int wk = gr.var >> 1;
-            if (wk == 0) launch_stream_nk<0>(gr.nk, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
-            else if (wk == 1) launch_stream_nk<1>(gr.nk, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
-            else launch_stream_nk<2>(gr.nk, dev_jobs + desc_base, (int)descs.size(), wg, lds, s);
+            const int nd = (int)descs.size();
+            if (gr.wide) {
+                if (wk == 0) launch_wide_nk<0>(gr.nk, dev_jobs + desc_base, nd, wg, lds, s);
+                else if (wk == 1) launch_wide_nk<1>(gr.nk, dev_jobs + desc_base, nd, wg, lds, s);
+                else launch_wide_nk<2>(gr.nk, dev_jobs + desc_base, nd, wg, lds, s);
+            } else {
+                if (wk == 0) launch_stream_nk<0>(gr.nk, dev_jobs + desc_base, nd, wg, lds, counter, s);
+                else if (wk == 1) launch_stream_nk<1>(gr.nk, dev_jobs + desc_base, nd, wg, lds, counter, s);
+                else launch_stream_nk<2>(gr.nk, dev_jobs + desc_base, nd, wg, lds, counter, s);
+            }
         } else {
             launch_mfma_group(gr.prec, gr.CW, gr.TT, gr.pf, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg,
                               lds, s);
@@ -663,7 +748,7 @@ int abf_beamform_noalloc(const abf_grid_job *job, void *stream)
     // bins per chunk: descriptors + the re-laid-out CSM of the chunk must fit the scratch array
     abf_grid_job one = *job;
     one.num_freqs = 1;
-    const size_t desc = align256(sizeof(MfJob));
+    const size_t desc = desc_bytes(1);
     const size_t per_bin = job_workspace(&one);
     if (per_bin == 0) return g_err.empty() ? fail(ABF_ERR_INVALID, "invalid job") : ABF_ERR_INVALID;
     if (desc + per_bin > (size_t)ABF_NOALLOC_SCRATCH_BYTES)

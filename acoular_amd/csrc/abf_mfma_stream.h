@@ -26,7 +26,8 @@
 //    step, instead of flipping the U fragment of every pair.
 //
 // Supported: split-bf16 form, 64 <= Mp <= 256 (NK = Mp / 32 blocks), hardware sin/cos with the
-// reference's float32 phase; other cases use the alternating kernels.
+// reference's float32 phase; other cases use the alternating kernels.  The kernel runs either one
+// work item (tile, frequency band) per workgroup or persistently (see bf_mfma_stream_kernel).
 #pragma once
 
 #include <utility>
@@ -42,11 +43,11 @@ template <class Fn, int... S> __device__ __forceinline__ void st_for(Fn &&fn, st
 }
 
 // The sub-steps of a frequency: step J (descending) covers the pairs I = J, J - 1, .., 0 (the
-// pair (J, J) that closes row block J first), cut into ceil((J + 1) / kStGS) chunks of nearly
+// pair (J, J) that closes row block J first), cut into ceil((J + 1) / GS) chunks of nearly
 // equal size [I0, I1) counted downwards; the top step takes one more chunk when the count would
 // be odd (an even count keeps the register and LDS double buffers of the U blocks in phase from
 // one frequency to the next).
-template <int NK> struct StSub {
+template <int NK, int GS = kStGS> struct StSub {
     static constexpr int kMax = 2 * NK + 2;
     int n;
     int J[kMax], I0[kMax], I1[kMax];
@@ -54,11 +55,11 @@ template <int NK> struct StSub {
     constexpr StSub() : n(0), J(), I0(), I1(), first(), last()
     {
         int tot = 0;
-        for (int j = NK - 1; j >= 0; --j) tot += (j + kStGS) / kStGS;
+        for (int j = NK - 1; j >= 0; --j) tot += (j + GS) / GS;
         const int extra = tot & 1;
         for (int j = NK - 1; j >= 0; --j) {
             const int np = j + 1;
-            const int nc = (np + kStGS - 1) / kStGS + (j == NK - 1 ? extra : 0);
+            const int nc = (np + GS - 1) / GS + (j == NK - 1 ? extra : 0);
             int done = 0;
             for (int q = 0; q < nc; ++q) {
                 const int sz = (np - done) / (nc - q) + ((np - done) % (nc - q) ? 1 : 0);
@@ -74,11 +75,11 @@ template <int NK> struct StSub {
     }
 };
 
+// One work item: the tile and frequency band that workgroup wg of the one-item-per-workgroup
+// launch computes.  Starts and ends with all LDS reads of the previous item done (barriers).
 template <int NK, int WK>
-__global__ __launch_bounds__(kStWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void bf_mfma_stream_kernel(
-    const MfJob *__restrict__ jobs, int njobs)
+__device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int njobs, int64_t wg, char *smem)
 {
-    extern __shared__ __attribute__((aligned(16))) char smem[];
     uint16_t *xs = reinterpret_cast<uint16_t *>(smem);                   // steering, bf16 [2][4][BN][KSB]
     float *xf = reinterpret_cast<float *>(smem + kStOffXf);              // steering, f32 [2][2][BN][KSF]
     char *us = smem + kStOffU;                                           // U ring [2][kStGS][8 KiB]
@@ -91,7 +92,6 @@ __global__ __launch_bounds__(kStWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t wg = blockIdx.x;
     const MfJob &jb = jobs[find_job(jobs, njobs, wg)];
     const int M = jb.M, F = jb.F, FB = jb.FB;
     const int64_t G = jb.G;
@@ -357,6 +357,30 @@ __global__ __launch_bounds__(kStWaves * 64) __attribute__((amdgpu_waves_per_eu(2
         par ^= NK & 1;
     }
     if (wave == 0) store(f1 - 1);
+}
+
+// PS = false: one work item per workgroup (workgroup index = item).  PS = true (persistent): a
+// grid of at most one workgroup per CU takes items 0 .. total - 1 in order from a device counter
+// (zeroed before the launch), one at a time, until none is left; every workgroup leaves the loop
+// once the counter passes total, so the grid drains whatever the item sizes.
+template <int NK, int WK, bool PS>
+__global__ __launch_bounds__(kStWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void bf_mfma_stream_kernel(
+    const MfJob *__restrict__ jobs, int njobs, int64_t total, unsigned *counter)
+{
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    if constexpr (!PS) {
+        stream_item<NK, WK>(jobs, njobs, blockIdx.x, smem);
+    } else {
+        unsigned *s_item = reinterpret_cast<unsigned *>(smem + kStOffItem);
+        for (;;) {
+            if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1u);
+            __syncthreads();
+            const int64_t item = *s_item;
+            __syncthreads();
+            if (item >= total) break;
+            stream_item<NK, WK>(jobs, njobs, item, smem);
+        }
+    }
 }
 
 } // namespace abf

@@ -72,10 +72,16 @@ enum abf_mfma_kernel {
     ABF_MFMA_KERNEL_PIPELINED = 4,   /* alternating, X and U fragments of the next block loaded
                                         before the MFMAs of the current one (split-bf16) */
     ABF_MFMA_KERNEL_PIPELINED_ROW_PAIRS = 5, /* both of the above, one column tile per fragment */
-    ABF_MFMA_KERNEL_STREAMED = 6 /* steering streamed through LDS one 32-microphone block at a time,
-                                    its generation overlapped with the MFMAs of the previous block
-                                    (split-bf16, 33..256 mics, reference phase with hardware sin/cos;
-                                    other jobs fall back to ROW_PAIRS) */
+    ABF_MFMA_KERNEL_STREAMED = 6, /* steering streamed through LDS one 32-microphone block at a time,
+                                     its generation overlapped with the MFMAs of the previous block
+                                     (split-bf16, 33..256 mics, reference phase with hardware sin/cos;
+                                     other jobs fall back to ROW_PAIRS) */
+    ABF_MFMA_KERNEL_STREAMED_WIDE = 7, /* STREAMED with 4 waves of two column tiles each (every U
+                                          fragment read from LDS feeds twice the MFMAs); same cases
+                                          and fallback */
+    ABF_MFMA_KERNEL_STREAMED_PERSISTENT = 8 /* STREAMED as a persistent grid (one workgroup per CU)
+                                               taking (job, tile, band) items from a device counter;
+                                               same cases and fallback */
 };
 
 /* Steering geometry shared by the grid entry points. r0 = ref_dist if ref_dist > 0, else

@@ -52,7 +52,7 @@ int main(void)
         }
     /* every kernel request on every microphone count: the streamed kernel (6) plans for 2..8
      * blocks of 32 microphones with the hardware sin/cos and the float32 phase, the others fall back */
-    for (int kr = 0; kr <= 6; ++kr)
+    for (int kr = 0; kr <= 8; ++kr)
         for (int mi = 0; mi < (int)(sizeof mics / sizeof mics[0]); ++mi)
             for (int fast = 0; fast < 2 && mics[mi] <= 1024; ++fast) {
                 abf_grid_job j = job(40401, mics[mi], 513, ABF_FORM_MFMA_BF16X3, ABF_STEER_TRUE_LEVEL);
@@ -61,14 +61,16 @@ int main(void)
                 int32_t kern = -1, cols = -1, fb = -1;
                 int64_t wgs = -1;
                 CHECK(abf_beamform_plan(&j, &kern, &cols, &fb, &wgs) == ABF_OK);
-                CHECK(wgs > 0 && cols > 0 && fb > 0 && kern >= 1 && kern <= 6);
+                CHECK(wgs > 0 && cols > 0 && fb > 0 && kern >= 1 && kern <= 8);
                 const int mp = (mics[mi] + 31) / 32 * 32;
-                CHECK((kern == 6) == (kr == 6 && fast && mp >= 64 && mp <= 256));
+                const int streamed = kr >= 6 && fast && mp >= 64 && mp <= 256;
+                CHECK((kern >= 6) == streamed);
+                CHECK(!streamed || kern == kr);
                 CHECK(abf_beamform_workspace_size(&j) > 0);
             }
     {
         abf_grid_job j = job(100, 64, 2, ABF_FORM_MFMA_BF16X3, ABF_STEER_CLASSIC);
-        j.tile_kernel = 7;
+        j.tile_kernel = 9;
         CHECK(abf_beamform_plan(&j, NULL, NULL, NULL, NULL) != ABF_OK);
     }
     /* the C4 shape: a CSM above 2^31 bytes */

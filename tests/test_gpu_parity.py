@@ -433,11 +433,77 @@ def test_batch_equals_alone_bitwise(gpu):
         np.testing.assert_array_equal(o.cpu().numpy(), a)
 
 
+def test_persistent_streamed_kernel_equals_one_item_per_workgroup(gpu):
+    """The persistent streamed launch (kernel 8: one workgroup per CU taking (job, tile, band) items
+    from a device counter) gives the maps of the one-item-per-workgroup launch (kernel 6) bit for
+    bit: a batch of jobs in two launch groups (64 and 128 microphones, two jobs sharing a group),
+    more items than CUs, run eagerly twice in a row (the counter is zeroed by every call) and
+    replayed from a captured graph with other GPU work between the replays."""
+    torch = _lib.torch()
+    lib = _lib.lib()
+    specs = [(64, 0.01, [900.0, 2000.0, 3100.0, 5000.0, 7700.0]), (128, 0.02, [1500.0, 6000.0]),
+             (64, 0.03, [12000.0, 400.0, 8000.0])]
+    tensors, outs = [], {6: [], 8: []}
+    jobs = {6: [], 8: []}
+    for i, (nm, inc, freqs) in enumerate(specs):
+        grid = aa.RectGrid(x_min=-0.5, x_max=0.5, y_min=-0.4, y_max=0.4, z=0.5, increment=inc)
+        mics, grid, csm = _scene(nm, grid, np.array(freqs), seed=30 + i)
+        sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='true level')
+        gpos, mpos = sv.device_geometry()
+        dcsm = _lib.to_device(csm, torch.complex64)
+        k = torch.from_numpy(2 * np.pi * np.array(freqs) / 343.0).to(gpu)
+        tensors += [dcsm, k, gpos, mpos]
+        for kern in (6, 8):
+            out = torch.full((len(freqs), grid.size), -3.0, dtype=torch.float32, device=gpu)
+            job = make_job(dcsm, k, gpos, mpos, sv.ref, 'true level', True, nm / (nm - 1.0), 'mfma_bf16x3',
+                           'reference', out, None, True, fast_trig=True)
+            job.tile_kernel, job.tile_cols, job.tile_freqs = kern, 64, 2
+            jobs[kern].append(job)
+            outs[kern].append(out)
+    plan = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()]
+    _lib.check(lib.abf_beamform_plan(jobs[8][0], *[ctypes.byref(x) for x in plan]))
+    assert plan[0].value == 8 and plan[3].value > 256  # more items than the persistent grid
+    arrs = {kern: (_lib.AbfGridJob * len(js))(*js) for kern, js in jobs.items()}
+    ws = {kern: _lib.workspace(lib.abf_beamform_batch_workspace_size(a, len(jobs[kern]), None))
+          for kern, a in arrs.items()}
+    stream = _lib.stream_handle()
+    for kern in (6, 8):
+        _lib.check(lib.abf_beamform_batch(arrs[kern], len(jobs[kern]), _lib.ptr(ws[kern]), ws[kern].numel(), stream))
+    want = [o.cpu().numpy() for o in outs[6]]
+    for o, w in zip(outs[8], want):
+        np.testing.assert_array_equal(o.cpu().numpy(), w)
+    # a second eager call (the counter starts from zero again)
+    for o in outs[8]:
+        o.fill_(-5.0)
+    _lib.check(lib.abf_beamform_batch(arrs[8], len(jobs[8]), _lib.ptr(ws[8]), ws[8].numel(), stream))
+    for o, w in zip(outs[8], want):
+        np.testing.assert_array_equal(o.cpu().numpy(), w)
+    # the first job's run (descriptors, counter reset, persistent launch) captured and replayed
+    job, out = jobs[8][0], outs[8][0]
+    wsj = _lib.workspace(lib.abf_beamform_workspace_size(job))
+    _lib.check(lib.abf_beamform_prepare(job, _lib.ptr(wsj), wsj.numel(), stream))
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            _lib.check(lib.abf_beamform_run(job, _lib.ptr(wsj), wsj.numel(), _lib.stream_handle()))
+    torch.cuda.current_stream().wait_stream(side)
+    for rep in range(3):
+        out.fill_(float(-rep))
+        junk = torch.randn(1 << 20, device=gpu).sum()  # other eager GPU work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        np.testing.assert_array_equal(out.cpu().numpy(), want[0])
+
+
 @pytest.mark.parametrize('form', ['mfma_bf16x3', 'mfma_f16x2', 'mfma_f32'])
 @pytest.mark.parametrize('kernel,tile_cols,tile_freqs', [(1, 0, 0), (1, 16, 1), (1, 64, 3), (2, 0, 0), (2, 64, 2),
                                                          (2, 32, 5), (3, 0, 0), (3, 16, 2), (3, 64, 4), (4, 0, 0),
                                                          (4, 64, 3), (5, 0, 0), (5, 32, 2), (6, 0, 0), (6, 64, 3),
-                                                         (6, 16, 1)])
+                                                         (6, 16, 1), (7, 0, 0), (7, 64, 3), (8, 0, 0), (8, 64, 3)])
 @pytest.mark.parametrize('nm,steer', [(64, 'true level'), (100, 'classic'), (256, 'inverse'), (33, 'true location')])
 def test_mfma_kernel_variants_and_tiles(gpu, form, kernel, tile_cols, tile_freqs, nm, steer):
     """Every MFMA kernel variant (alternating, warp-specialised, row pairs, pipelined) and explicit tile shapes, including
@@ -463,12 +529,13 @@ def test_mfma_kernel_variants_and_tiles(gpu, form, kernel, tile_cols, tile_freqs
     assert err <= _tol(form)
 
 
+@pytest.mark.parametrize('kernel', [6, 7, 8])
 @pytest.mark.parametrize('nm', [33, 64, 65, 100, 128, 160, 192, 224, 256])
 @pytest.mark.parametrize('steer', STEERS)
-def test_streamed_kernel_every_block_count(gpu, nm, steer):
-    """The streamed kernel (one instantiation per 2..8 blocks of 32 microphones): both r_diag
+def test_streamed_kernel_every_block_count(gpu, kernel, nm, steer):
+    """The streamed kernels (one instantiation per 2..8 blocks of 32 microphones): both r_diag
     settings, a grid whose last tile is partial, a band that does not divide into the frequency
-    tile; the plan reports the streamed kernel."""
+    tile; the plan reports the kernel asked for."""
     torch = _lib.torch()
     freqs = np.array([700.0, 3300.0, 9000.0, 14000.0, 19500.0])
     grid = aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.2, y_max=0.2, z=0.5, increment=0.03)  # 21 x 14
@@ -484,15 +551,15 @@ def test_streamed_kernel_every_block_count(gpu, nm, steer):
         norm = nm / (nm - 1.0) if r_diag else 1.0
         job = make_job(dcsm, k, gpos, mpos, sv.ref, steer, r_diag, norm, 'mfma_bf16x3', 'reference', out, onorm,
                        r_diag, fast_trig=True)
-        job.tile_kernel, job.tile_cols, job.tile_freqs = 6, 64, 2
+        job.tile_kernel, job.tile_cols, job.tile_freqs = kernel, 64, 2
         plan = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()]
         _lib.check(lib.abf_beamform_plan(job, *[ctypes.byref(x) for x in plan]))
-        assert plan[0].value == 6
+        assert plan[0].value == kernel
         ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
         _lib.check(lib.abf_beamform(job, _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
         ref = _oracle_maps(csm, freqs, mics, grid, steer, r_diag)
         err = peak_rel_err(out.cpu().numpy(), ref)
-        print(f'nm={nm} steer={steer} r_diag={r_diag} err={err:.2e}')
+        print(f'kernel={kernel} nm={nm} steer={steer} r_diag={r_diag} err={err:.2e}')
         assert err <= TOL
         want_norm = np.stack([B.freq_beamformer_full_csm(csm[0].astype(np.complex128), B.steer_r0(grid.pos, sv.ref),
                                                          B.dist_mat(grid.pos, mics.pos), 1.0, 1.0, r_diag, steer)[1]]
@@ -500,9 +567,10 @@ def test_streamed_kernel_every_block_count(gpu, nm, steer):
         np.testing.assert_allclose(onorm.cpu().numpy(), want_norm, rtol=1e-5)
 
 
-def test_streamed_kernel_falls_back_outside_its_cases(gpu):
-    """The streamed kernel asked for the float64 phase, the library sin/cos or more than 256
-    microphones runs the row-pair kernel (the plan says so) and stays within the gate."""
+@pytest.mark.parametrize('kernel', [6, 7, 8])
+def test_streamed_kernel_falls_back_outside_its_cases(gpu, kernel):
+    """The streamed kernels asked for the float64 phase, the library sin/cos or more than 256
+    microphones run the row-pair kernel (the plan says so) and stay within the gate."""
     torch = _lib.torch()
     lib = _lib.lib()
     freqs = np.array([2000.0, 17000.0])
@@ -515,7 +583,7 @@ def test_streamed_kernel_falls_back_outside_its_cases(gpu):
         job = make_job(_lib.to_device(csm, torch.complex64), torch.from_numpy(2 * np.pi * freqs / 343.0).to(gpu),
                        gpos, mpos, sv.ref, 'true level', True, nm / (nm - 1.0), 'mfma_bf16x3', phase, out, None, True,
                        fast_trig=fast)
-        job.tile_kernel, job.tile_cols, job.tile_freqs = 6, 64, 2
+        job.tile_kernel, job.tile_cols, job.tile_freqs = kernel, 64, 2
         plan = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()]
         _lib.check(lib.abf_beamform_plan(job, *[ctypes.byref(x) for x in plan]))
         assert plan[0].value in (1, 3)
@@ -536,7 +604,7 @@ def test_mfma_kernel_variant_rejected_when_unknown(gpu):
     k = torch.from_numpy(2 * np.pi * freqs / 343.0).to(gpu)
     out = torch.empty((1, grid.size), dtype=torch.float32, device=gpu)
     job = make_job(dcsm, k, gpos, mpos, sv.ref, 'classic', False, 1.0, 'mfma_bf16x3', 'reference', out)
-    job.tile_kernel = 7
+    job.tile_kernel = 9
     with pytest.raises(ValueError):
         _lib.check(_lib.lib().abf_beamform(job, None, 0, _lib.stream_handle()))
 
