@@ -26,6 +26,9 @@ ABF_ERR_WORKSPACE = -4
 FORMS = {'plain_f32': 0, 'mfma_bf16x3': 1, 'mfma_f32': 2, 'mfma_f16x2': 3}
 PHASE_MODES = {'reference': 0, 'fp64': 1}
 FAST_TRIG = 0x100
+# enum abf_mfma_kernel (include/abf.h) values named by the host code
+MFMA_KERNEL_STREAMED = 6
+MFMA_KERNEL_STREAMED_PERSISTENT = 8
 STEER_CODES = {'classic': 1, 'inverse': 2, 'true level': 3, 'true location': 4}
 
 _c_dp = ctypes.c_void_p

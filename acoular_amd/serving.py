@@ -138,13 +138,17 @@ class GpuExecutor:
     maps copied to pinned host memory behind it, completion polled through a HIP event."""
 
     def __init__(self, form: str = 'mfma_bf16x3', phase_mode: str = 'reference', num_streams: int = 2,
-                 staging_bytes: int = 0):
+                 staging_bytes: int = 0, persistent: bool | None = True):
         """``staging_bytes`` > 0 gives every stream a reusable pinned host buffer of that size for the
-        device-to-host copies instead of a pinned allocation per launch."""
+        device-to-host copies instead of a pinned allocation per launch.  ``persistent=True`` runs the
+        jobs whose launch shape is the streamed kernel as its persistent launch (one workgroup per CU
+        taking work items from a device counter; same tiles, same maps bit for bit), ``False`` as one
+        work item per workgroup, ``None`` as the tile table says."""
         torch = _lib.torch()
         self.dev = _lib.device()
         self.form = form
         self.phase_mode = phase_mode
+        self.persistent = persistent
         # default-priority streams only
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(num_streams)]
         self.busy = [False] * num_streams
@@ -189,8 +193,13 @@ class GpuExecutor:
                 out = torch.empty((b1 - b0, r.num_grid), dtype=torch.float32, device=self.dev)
                 nm = r.num_mics
                 norm = nm / (nm - 1.0) if r.r_diag else 1.0
-                jobs.append(make_job(r.csm[b0:b1], r.wavenumber[b0:b1], r.gpos, r.mpos, r.ref, r.steer_type,
-                                     r.r_diag, norm, self.form, self.phase_mode, out, None, clip=r.r_diag))
+                job = make_job(r.csm[b0:b1], r.wavenumber[b0:b1], r.gpos, r.mpos, r.ref, r.steer_type, r.r_diag,
+                               norm, self.form, self.phase_mode, out, None, clip=r.r_diag)
+                if self.persistent and job.tile_kernel == _lib.MFMA_KERNEL_STREAMED:
+                    job.tile_kernel = _lib.MFMA_KERNEL_STREAMED_PERSISTENT
+                elif self.persistent is False and job.tile_kernel == _lib.MFMA_KERNEL_STREAMED_PERSISTENT:
+                    job.tile_kernel = _lib.MFMA_KERNEL_STREAMED
+                jobs.append(job)
                 outs.append(out)
             arr = (_lib.AbfGridJob * len(jobs))(*jobs)
             nb = _lib.lib().abf_beamform_batch_workspace_size(arr, len(jobs), None)
@@ -236,10 +245,12 @@ class BeamformServer:
     def __init__(self, hbm_budget: int = 32 << 30, max_batch: int = 64, num_streams: int = 2,
                  form: str = 'mfma_bf16x3', batched: bool = True, phase_mode: str = 'reference',
                  keep_results: bool = True, staging_bytes: int = 0, max_bins_per_launch: int | None = None,
-                 max_retries: int = 3, executor=None):
+                 max_retries: int = 3, executor=None, persistent: bool | None = True):
         """``keep_results=False`` drops each map once it has reached host memory (long benchmark
-        runs, latency only); ``max_bins_per_launch`` launches long bands in chunks of bins."""
-        self.executor = executor if executor is not None else GpuExecutor(form, phase_mode, num_streams, staging_bytes)
+        runs, latency only); ``max_bins_per_launch`` launches long bands in chunks of bins;
+        ``persistent`` as in ``GpuExecutor``."""
+        self.executor = executor if executor is not None else GpuExecutor(form, phase_mode, num_streams, staging_bytes,
+                                                                          persistent)
         self.budget = int(hbm_budget)
         self.max_batch = max_batch if batched else 1
         self.max_bins = max_bins_per_launch

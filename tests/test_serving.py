@@ -170,6 +170,38 @@ def test_results_identical_alone_and_in_mix(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('persistent', [True, False])
+def test_persistent_serving_matches_alone(gpu, persistent):
+    """Serving with the streamed jobs as persistent launches (the default) or as one work item per
+    workgroup gives each request the maps of its launch shape run alone, bit for bit (the two
+    launches of the streamed kernel compute every item alike); the mix holds streamed jobs."""
+    import torch
+
+    from acoular_amd import _lib
+    from acoular_amd.fbeamform import make_job
+    from acoular_amd.serving import BeamformServer, RequestFactory
+
+    fac = RequestFactory(seed=11, g_min=3000, g_max=9000, f_max=40, mic_counts=(64, 128, 256))
+    reqs = fac.make(12, qps=None)
+    srv = BeamformServer(hbm_budget=1 << 30, max_batch=6, num_streams=2, persistent=persistent)
+    srv.run(reqs, realtime=False)
+    lib = _lib.lib()
+    streamed = 0
+    for r in reqs:
+        gpos = r.grid_fn()
+        csm = r.csm_fn()
+        out = torch.empty((r.num_freqs, gpos.shape[1]), dtype=torch.float32, device=gpu)
+        nm = r.num_mics
+        job = make_job(csm, r.wavenumber, gpos, r.mpos, r.ref, r.steer_type, True, nm / (nm - 1.0),
+                       'mfma_bf16x3', 'reference', out, None, clip=True)
+        streamed += job.tile_kernel in (_lib.MFMA_KERNEL_STREAMED, _lib.MFMA_KERNEL_STREAMED_PERSISTENT)
+        ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
+        _lib.check(lib.abf_beamform(job, _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
+        np.testing.assert_array_equal(out.cpu().numpy(), r.result)
+    assert streamed > 0
+
+
+@pytest.mark.gpu
 def test_serving_results_match_oracle(gpu):
     from acoular_amd.serving import BeamformServer, RequestFactory
     from oracle import beamform as B
