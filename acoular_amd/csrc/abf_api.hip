@@ -66,7 +66,7 @@ struct MfPlan {
     bool ws; // warp-specialised kernel (double-buffered steering matrix)
     bool st; // streamed kernel (steering blocks through an LDS ring)
     bool wide; // streamed kernel with two column tiles per wave
-    bool ps;   // streamed kernel, persistent launch over a work-item counter
+    bool ps;   // streamed kernel, persistent launch (one workgroup per CU walking the work items)
     int64_t ntiles, wgs;
     size_t prep_bytes, lds;
     size_t uscale_off; // f16x2: offset of the per-frequency U scales in the job's workspace
@@ -230,25 +230,18 @@ constexpr int kPack = 16;
 struct JobPack {
     MfJob jobs[kPack];
     int n;
-    unsigned *counter; // work-item counter of a persistent launch to zero, or null
 };
 
-// job descriptors (and the zeroed work-item counter of a persistent launch) from kernel arguments:
-// no host copy, no memset node, so the sequence can be captured in a graph
+// job descriptors from kernel arguments: no host copy, no memset node, so the sequence can be
+// captured in a graph
 __global__ void write_jobs_kernel(JobPack pack, MfJob *dst)
 {
     const int i = threadIdx.x;
     if (i < pack.n) dst[i] = pack.jobs[i];
-    if (i == 0 && pack.counter) *pack.counter = 0u;
 }
 
-// batch workspace head: the job descriptors, then one work-item counter per job (a launch group
-// holds at least one job, so there is one per group)
-size_t desc_bytes(int num_jobs)
-{
-    const size_t n = (size_t)std::max(num_jobs, 1);
-    return align256(n * sizeof(MfJob)) + align256(n * sizeof(unsigned));
-}
+// batch workspace head: the job descriptors
+size_t desc_bytes(int num_jobs) { return align256((size_t)std::max(num_jobs, 1) * sizeof(MfJob)); }
 
 __global__ void set_double_kernel(double *d, double v)
 {
@@ -347,10 +340,9 @@ int cu_count()
     return n;
 }
 
-// counter == null: one work item per workgroup; otherwise a persistent grid of one workgroup per CU
-// (the kernel's LDS allows one) over the wgs items
-template <int NK, int WK, bool PS>
-void launch_stream_ps(const MfJob *dev, int n, int64_t wgs, size_t lds, unsigned *counter, hipStream_t s)
+// PS = false: one work item per workgroup; true: a persistent grid of one workgroup per CU (the
+// kernel's LDS allows one) walking the wgs items
+template <int NK, int WK, bool PS> void launch_stream_ps(const MfJob *dev, int n, int64_t wgs, size_t lds, hipStream_t s)
 {
     static std::once_flag attr; // kernel attributes set once, safe under concurrent first calls
     std::call_once(attr, [] {
@@ -359,27 +351,26 @@ void launch_stream_ps(const MfJob *dev, int n, int64_t wgs, size_t lds, unsigned
     });
     const int64_t grid = PS ? std::min<int64_t>(wgs, cu_count()) : wgs;
     hipLaunchKernelGGL((bf_mfma_stream_kernel<NK, WK, PS>), dim3((unsigned)grid), dim3(kStWaves * 64), lds, s, dev,
-                       n, wgs, counter);
+                       n, wgs);
 }
 
-template <int NK, int WK>
-void launch_stream(const MfJob *dev, int n, int64_t wgs, size_t lds, unsigned *counter, hipStream_t s)
+template <int NK, int WK> void launch_stream(const MfJob *dev, int n, int64_t wgs, size_t lds, bool ps, hipStream_t s)
 {
-    if (counter) launch_stream_ps<NK, WK, true>(dev, n, wgs, lds, counter, s);
-    else launch_stream_ps<NK, WK, false>(dev, n, wgs, lds, nullptr, s);
+    if (ps) launch_stream_ps<NK, WK, true>(dev, n, wgs, lds, s);
+    else launch_stream_ps<NK, WK, false>(dev, n, wgs, lds, s);
 }
 
 template <int WK>
-void launch_stream_nk(int nk, const MfJob *dev, int n, int64_t wgs, size_t lds, unsigned *counter, hipStream_t s)
+void launch_stream_nk(int nk, const MfJob *dev, int n, int64_t wgs, size_t lds, bool ps, hipStream_t s)
 {
     switch (nk) {
-    case 2: launch_stream<2, WK>(dev, n, wgs, lds, counter, s); break;
-    case 3: launch_stream<3, WK>(dev, n, wgs, lds, counter, s); break;
-    case 4: launch_stream<4, WK>(dev, n, wgs, lds, counter, s); break;
-    case 5: launch_stream<5, WK>(dev, n, wgs, lds, counter, s); break;
-    case 6: launch_stream<6, WK>(dev, n, wgs, lds, counter, s); break;
-    case 7: launch_stream<7, WK>(dev, n, wgs, lds, counter, s); break;
-    default: launch_stream<8, WK>(dev, n, wgs, lds, counter, s); break;
+    case 2: launch_stream<2, WK>(dev, n, wgs, lds, ps, s); break;
+    case 3: launch_stream<3, WK>(dev, n, wgs, lds, ps, s); break;
+    case 4: launch_stream<4, WK>(dev, n, wgs, lds, ps, s); break;
+    case 5: launch_stream<5, WK>(dev, n, wgs, lds, ps, s); break;
+    case 6: launch_stream<6, WK>(dev, n, wgs, lds, ps, s); break;
+    case 7: launch_stream<7, WK>(dev, n, wgs, lds, ps, s); break;
+    default: launch_stream<8, WK>(dev, n, wgs, lds, ps, s); break;
     }
 }
 
@@ -631,10 +622,7 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
     }
     if (int r = hip_status("MFMA re-layout launch")) return r;
     int desc_base = 0;
-    unsigned *counters = reinterpret_cast<unsigned *>(ws + align256((size_t)std::max(num_jobs, 1) * sizeof(MfJob)));
-    for (size_t gi = 0; gi < groups.size(); ++gi) {
-        const Group &gr = groups[gi];
-        unsigned *counter = gr.ps ? counters + gi : nullptr;
+    for (const Group &gr : groups) {
         int64_t wg = 0;
         size_t lds = 0;
         std::vector<MfJob> descs;
@@ -646,7 +634,6 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
         for (size_t b = 0; b < descs.size(); b += kPack) {
             JobPack pk{};
             pk.n = (int)std::min<size_t>(kPack, descs.size() - b);
-            pk.counter = b == 0 ? counter : nullptr;
             for (int q = 0; q < pk.n; ++q) pk.jobs[q] = descs[b + q];
             hipLaunchKernelGGL(write_jobs_kernel, dim3(1), dim3(64), 0, s, pk, dev_jobs + desc_base + b);
         }
@@ -658,9 +645,9 @@ int beamform_batch_impl(const abf_grid_job *jobs, int32_t num_jobs, void *worksp
                 else if (wk == 1) launch_wide_nk<1>(gr.nk, dev_jobs + desc_base, nd, wg, lds, s);
                 else launch_wide_nk<2>(gr.nk, dev_jobs + desc_base, nd, wg, lds, s);
             } else {
-                if (wk == 0) launch_stream_nk<0>(gr.nk, dev_jobs + desc_base, nd, wg, lds, counter, s);
-                else if (wk == 1) launch_stream_nk<1>(gr.nk, dev_jobs + desc_base, nd, wg, lds, counter, s);
-                else launch_stream_nk<2>(gr.nk, dev_jobs + desc_base, nd, wg, lds, counter, s);
+                if (wk == 0) launch_stream_nk<0>(gr.nk, dev_jobs + desc_base, nd, wg, lds, gr.ps, s);
+                else if (wk == 1) launch_stream_nk<1>(gr.nk, dev_jobs + desc_base, nd, wg, lds, gr.ps, s);
+                else launch_stream_nk<2>(gr.nk, dev_jobs + desc_base, nd, wg, lds, gr.ps, s);
             }
         } else {
             launch_mfma_group(gr.prec, gr.CW, gr.TT, gr.pf, gr.ws, gr.var, dev_jobs + desc_base, (int)descs.size(), wg,

@@ -10,11 +10,11 @@
 
 namespace abf {
 
-template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 0, false>(const MfJob *, int, int64_t, unsigned *);
-template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 1, false>(const MfJob *, int, int64_t, unsigned *);
-template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 2, false>(const MfJob *, int, int64_t, unsigned *);
-template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 0, true>(const MfJob *, int, int64_t, unsigned *);
-template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 1, true>(const MfJob *, int, int64_t, unsigned *);
-template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 2, true>(const MfJob *, int, int64_t, unsigned *);
+template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 0, false>(const MfJob *, int, int64_t);
+template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 1, false>(const MfJob *, int, int64_t);
+template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 2, false>(const MfJob *, int, int64_t);
+template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 0, true>(const MfJob *, int, int64_t);
+template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 1, true>(const MfJob *, int, int64_t);
+template __global__ void bf_mfma_stream_kernel<ABF_ST_NK, 2, true>(const MfJob *, int, int64_t);
 
 } // namespace abf

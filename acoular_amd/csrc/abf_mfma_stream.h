@@ -360,25 +360,19 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
 }
 
 // PS = false: one work item per workgroup (workgroup index = item).  PS = true (persistent): a
-// grid of at most one workgroup per CU takes items 0 .. total - 1 in order from a device counter
-// (zeroed before the launch), one at a time, until none is left; every workgroup leaves the loop
-// once the counter passes total, so the grid drains whatever the item sizes.
+// grid of at most one workgroup per CU walks the items 0 .. total - 1, workgroup b taking items
+// b, b + grid, b + 2 grid, ... in order; every workgroup leaves once its next item passes total.
 template <int NK, int WK, bool PS>
 __global__ __launch_bounds__(kStWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void bf_mfma_stream_kernel(
-    const MfJob *__restrict__ jobs, int njobs, int64_t total, unsigned *counter)
+    const MfJob *__restrict__ jobs, int njobs, int64_t total)
 {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     if constexpr (!PS) {
         stream_item<NK, WK>(jobs, njobs, blockIdx.x, smem);
     } else {
-        unsigned *s_item = reinterpret_cast<unsigned *>(smem + kStOffItem);
-        for (;;) {
-            if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1u);
-            __syncthreads();
-            const int64_t item = *s_item;
-            __syncthreads();
-            if (item >= total) break;
+        for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
             stream_item<NK, WK>(jobs, njobs, item, smem);
+            __syncthreads();
         }
     }
 }

@@ -138,12 +138,12 @@ class GpuExecutor:
     maps copied to pinned host memory behind it, completion polled through a HIP event."""
 
     def __init__(self, form: str = 'mfma_bf16x3', phase_mode: str = 'reference', num_streams: int = 2,
-                 staging_bytes: int = 0, persistent: bool | None = True):
+                 staging_bytes: int = 0, persistent: bool | None = None):
         """``staging_bytes`` > 0 gives every stream a reusable pinned host buffer of that size for the
         device-to-host copies instead of a pinned allocation per launch.  ``persistent=True`` runs the
         jobs whose launch shape is the streamed kernel as its persistent launch (one workgroup per CU
-        taking work items from a device counter; same tiles, same maps bit for bit), ``False`` as one
-        work item per workgroup, ``None`` as the tile table says."""
+        walking the work items round robin; same tiles, same maps bit for bit), ``False`` as one
+        work item per workgroup, ``None`` (default) as the tile table says."""
         torch = _lib.torch()
         self.dev = _lib.device()
         self.form = form
@@ -245,7 +245,7 @@ class BeamformServer:
     def __init__(self, hbm_budget: int = 32 << 30, max_batch: int = 64, num_streams: int = 2,
                  form: str = 'mfma_bf16x3', batched: bool = True, phase_mode: str = 'reference',
                  keep_results: bool = True, staging_bytes: int = 0, max_bins_per_launch: int | None = None,
-                 max_retries: int = 3, executor=None, persistent: bool | None = True):
+                 max_retries: int = 3, executor=None, persistent: bool | None = None):
         """``keep_results=False`` drops each map once it has reached host memory (long benchmark
         runs, latency only); ``max_bins_per_launch`` launches long bands in chunks of bins;
         ``persistent`` as in ``GpuExecutor``."""

@@ -408,9 +408,9 @@ def serving_bench(args):
     reqs = fac.make(args.serve_requests, qps, seed_offset=2)
     run = _serve(mfma, reqs, True)
     log(f"p50 {run['p50_ms']:.1f} ms, p99 {run['p99_ms']:.1f} ms, {run['map_bins_per_s']:.3g} map-bins/s")
-    # the same request stream with the streamed jobs launched one work item per workgroup
-    prs = _serve({**mfma, 'persistent': False}, fac.make(args.serve_requests, qps, seed_offset=2), True)
-    log(f"one work item per workgroup: p50 {prs['p50_ms']:.1f} ms, p99 {prs['p99_ms']:.1f} ms, "
+    # the same request stream with every streamed job as a persistent launch
+    prs = _serve({**mfma, 'persistent': True}, fac.make(args.serve_requests, qps, seed_offset=2), True)
+    log(f"persistent launches: p50 {prs['p50_ms']:.1f} ms, p99 {prs['p99_ms']:.1f} ms, "
         f"{prs['map_bins_per_s']:.3g} map-bins/s")
     log('baseline: unbatched plain kernel')
     psat = _serve(plain, fac.make(args.serve_plain, None, seed_offset=1), False)
@@ -427,7 +427,7 @@ def serving_bench(args):
                    'requests': args.serve_requests, 'offered_req_per_s': qps, 'load_fraction': args.serve_load,
                    'hbm_budget_bytes': mfma['hbm_budget'], 'max_batch': mfma['max_batch'],
                    'streams': mfma['num_streams'], 'parallelism': 'single GPU'},
-        'serving': run, 'serving_one_item_per_workgroup': prs, 'saturation': sat,
+        'serving': run, 'serving_persistent': prs, 'saturation': sat,
         'plain_unbatched': {'saturation': psat, 'same_offered_load_first_requests': prun},
     }
 
@@ -1321,11 +1321,11 @@ def main():
                    'requests': sv['config']['requests'], 'offered_req_per_s': sv['config']['offered_req_per_s'],
                    'saturation_req_per_s': sv['saturation']['req_per_s'],
                    'mean_batch': sv['serving']['mean_batch'],
-                   'launches': 'streamed jobs as persistent launches (one workgroup per CU over a work-item counter)',
-                   'one_item_per_workgroup': {'p50_ms': sv['serving_one_item_per_workgroup']['p50_ms'],
-                                              'p99_ms': sv['serving_one_item_per_workgroup']['p99_ms'],
-                                              'map_bins_per_s': sv['serving_one_item_per_workgroup'][
-                                                  'map_bins_per_s']},
+                   'persistent_launches': {'p50_ms': sv['serving_persistent']['p50_ms'],
+                                           'p99_ms': sv['serving_persistent']['p99_ms'],
+                                           'map_bins_per_s': sv['serving_persistent']['map_bins_per_s'],
+                                           'launches': 'every streamed job as a persistent launch (one workgroup '
+                                                       'per CU walking the work items round robin)'},
                    'plain_unbatched': {'saturation_req_per_s': sv['plain_unbatched']['saturation']['req_per_s'],
                                        'p50_ms_same_load': sv['plain_unbatched']['same_offered_load_first_requests'][
                                            'p50_ms'],

@@ -80,8 +80,8 @@ enum abf_mfma_kernel {
                                           fragment read from LDS feeds twice the MFMAs); same cases
                                           and fallback */
     ABF_MFMA_KERNEL_STREAMED_PERSISTENT = 8 /* STREAMED as a persistent grid (one workgroup per CU)
-                                               taking (job, tile, band) items from a device counter;
-                                               same cases and fallback */
+                                               walking the (job, tile, band) items in a fixed
+                                               round-robin order; same cases and fallback */
 };
 
 /* Steering geometry shared by the grid entry points. r0 = ref_dist if ref_dist > 0, else

@@ -434,11 +434,11 @@ def test_batch_equals_alone_bitwise(gpu):
 
 
 def test_persistent_streamed_kernel_equals_one_item_per_workgroup(gpu):
-    """The persistent streamed launch (kernel 8: one workgroup per CU taking (job, tile, band) items
-    from a device counter) gives the maps of the one-item-per-workgroup launch (kernel 6) bit for
-    bit: a batch of jobs in two launch groups (64 and 128 microphones, two jobs sharing a group),
-    more items than CUs, run eagerly twice in a row (the counter is zeroed by every call) and
-    replayed from a captured graph with other GPU work between the replays."""
+    """The persistent streamed launch (kernel 8: one workgroup per CU walking the (job, tile, band)
+    items round robin) gives the maps of the one-item-per-workgroup launch (kernel 6) bit for bit:
+    a batch of jobs in two launch groups (64 and 128 microphones, two jobs sharing a group), more
+    items than CUs, run eagerly twice in a row and replayed from a captured graph with other GPU
+    work between the replays."""
     torch = _lib.torch()
     lib = _lib.lib()
     specs = [(64, 0.01, [900.0, 2000.0, 3100.0, 5000.0, 7700.0]), (128, 0.02, [1500.0, 6000.0]),
@@ -472,13 +472,13 @@ def test_persistent_streamed_kernel_equals_one_item_per_workgroup(gpu):
     want = [o.cpu().numpy() for o in outs[6]]
     for o, w in zip(outs[8], want):
         np.testing.assert_array_equal(o.cpu().numpy(), w)
-    # a second eager call (the counter starts from zero again)
+    # a second eager call
     for o in outs[8]:
         o.fill_(-5.0)
     _lib.check(lib.abf_beamform_batch(arrs[8], len(jobs[8]), _lib.ptr(ws[8]), ws[8].numel(), stream))
     for o, w in zip(outs[8], want):
         np.testing.assert_array_equal(o.cpu().numpy(), w)
-    # the first job's run (descriptors, counter reset, persistent launch) captured and replayed
+    # the first job's run (descriptors, persistent launch) captured and replayed
     job, out = jobs[8][0], outs[8][0]
     wsj = _lib.workspace(lib.abf_beamform_workspace_size(job))
     _lib.check(lib.abf_beamform_prepare(job, _lib.ptr(wsj), wsj.numel(), stream))
@@ -590,7 +590,12 @@ def test_streamed_kernel_falls_back_outside_its_cases(gpu, kernel):
         ws = _lib.workspace(lib.abf_beamform_workspace_size(job))
         _lib.check(lib.abf_beamform(job, _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
         ref = _oracle_maps(csm, freqs, mics, grid, 'true level', True, phase)
-        assert peak_rel_err(out.cpu().numpy(), ref) <= TOL
+        got = out.cpu().numpy()
+        err = peak_rel_err(got, ref)
+        bad = np.abs(got - ref) > TOL * np.abs(ref).max(axis=1, keepdims=True)
+        print(f'kernel={kernel} nm={nm} phase={phase} fast={fast} plan={[x.value for x in plan]} err={err:.2e} '
+              f'bad points per bin {bad.sum(axis=1).tolist()} first {np.argwhere(bad)[:4].tolist()}')
+        assert err <= TOL
 
 
 def test_mfma_kernel_variant_rejected_when_unknown(gpu):
