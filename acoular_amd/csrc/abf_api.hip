@@ -406,24 +406,29 @@ void launch_mfma_group(int prec, int CW, int TT, bool pf, bool ws, int var, cons
 }
 
 // The plain kernel with microphone chunks of JC (ABF_PLAIN_JC = 64, 128 or 256): JC x 512
-// bytes of LDS per workgroup of 64 grid points; grid (ceil(G / 64), F).
-template <int JC> static void launch_plain_jc(const PlainArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+// bytes of LDS per workgroup of 64 grid points (LU: plus 4 x JC x 128 bytes of U blocks); grid
+// (ceil(G / 64), F).
+template <int JC, bool LU> static void launch_plain_jc(const PlainArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
+    constexpr size_t kMaxLds = JC * 64 * sizeof(float2) + (LU ? 4 * JC * 16 * sizeof(float2) : 0);
     static std::once_flag attr; // kernel attribute set once, safe under concurrent first calls
     std::call_once(attr, [] {
-        (void)hipFuncSetAttribute((const void *)bf_plain_kernel<JC>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  JC * 64 * (int)sizeof(float2));
+        (void)hipFuncSetAttribute((const void *)bf_plain_kernel<JC, LU>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kMaxLds);
     });
-    const size_t lds = (size_t)std::min(JC, a.Mp) * 64 * sizeof(float2);
-    hipLaunchKernelGGL(bf_plain_kernel<JC>, dim3(gx, gy), dim3(256), lds, s, a);
+    const size_t lds = LU ? kMaxLds : (size_t)std::min(JC, a.Mp) * 64 * sizeof(float2);
+    hipLaunchKernelGGL((bf_plain_kernel<JC, LU>), dim3(gx, gy), dim3(256), lds, s, a);
 }
 
+// ABF_PLAIN_LDSU=1 stages U in LDS (measured slower than the scalar-cache broadcast, kept for
+// comparison; 64-microphone chunks only)
 static void launch_plain(const PlainArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
     const int jc = env_int("ABF_PLAIN_JC", kPlainDefaultJC);
-    if (jc <= 64) launch_plain_jc<64>(a, gx, gy, s);
-    else if (jc <= 128) launch_plain_jc<128>(a, gx, gy, s);
-    else launch_plain_jc<256>(a, gx, gy, s);
+    if (env_int("ABF_PLAIN_LDSU", 0)) launch_plain_jc<64, true>(a, gx, gy, s);
+    else if (jc <= 64) launch_plain_jc<64, false>(a, gx, gy, s);
+    else if (jc <= 128) launch_plain_jc<128, false>(a, gx, gy, s);
+    else launch_plain_jc<256, false>(a, gx, gy, s);
 }
 
 void prep_plain(const abf_grid_job *j, void *ws, hipStream_t s)

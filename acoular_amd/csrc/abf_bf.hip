@@ -81,11 +81,15 @@ __device__ __forceinline__ float2 plain_steer(const PlainArgs &a, const PlainPoi
 // evaluating the steering vectors of earlier rows again (on 256 mics, 201 x 201, 513 bins:
 // 99.6 ms for JC = 64, 105.8 ms for 128; the same loop with vector loads of U by every lane,
 // which the L1 then serves 64 times over, 172 and 163 ms; scalar loads waited for at once,
-// with JC = 256, 197 ms).
-template <int JC>
+// with JC = 256, 197 ms).  LU = true stages the U block of a row block and chunk in LDS instead
+// (per wave [JC][16] float2, one 128-byte column per lane loaded with vector loads), every lane
+// reading each value there (one broadcast address per read); the accumulation order is the same
+// (115.6 ms against 99.3 ms on the workload above: each LDS read returns 16 bytes to all 64 lanes
+// through the LDS data path, where one scalar load puts 64 bytes into SGPRs for the whole wave).
+template <int JC, bool LU>
 __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
 {
-    extern __shared__ float2 vlds[]; // [JC][kPlainNP]
+    extern __shared__ float2 vlds[]; // [JC][kPlainNP], then (LU) [kPlainWaves][JC][kPlainRB]
     __shared__ double s_help[kPlainWaves][kPlainNP];
     __shared__ float s_q[kPlainWaves][kPlainNP];
 
@@ -151,6 +155,28 @@ __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
                     ai[r] = fmaf(u[r].y, v.x, ai[r]);
                 }
             };
+            if constexpr (LU) {
+                float2 *uw = vlds + JC * kPlainNP + wave * (JC * kPlainRB);
+                const int jl = js + lane;
+                if (jl < je) {
+                    const float4 *src = reinterpret_cast<const float4 *>(ut + (int64_t)jl * a.Mp + i0);
+                    float4 *dst = reinterpret_cast<float4 *>(uw + (jl - js) * kPlainRB);
+#pragma unroll
+                    for (int t = 0; t < kPlainRB / 2; ++t) dst[t] = src[t];
+                }
+                // the LDS operations of one wave complete in order: these reads see the writes
+                for (int j = js; j < je; ++j) {
+                    const float4 *u4 = reinterpret_cast<const float4 *>(uw + (j - js) * kPlainRB);
+                    float2 u[kPlainRB];
+#pragma unroll
+                    for (int t = 0; t < kPlainRB / 2; ++t) {
+                        const float4 w = u4[t];
+                        u[2 * t] = make_float2(w.x, w.y);
+                        u[2 * t + 1] = make_float2(w.z, w.w);
+                    }
+                    fmas(u, vlds[(j - c0) * kPlainNP + lane]);
+                }
+            } else {
             if (js < je) {
                 ldu(js, ua);
                 va = vlds[(js - c0) * kPlainNP + lane];
@@ -169,6 +195,7 @@ __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
                 va = vlds[(j2 - c0) * kPlainNP + lane];
                 fmas(ub, j + 1 < je ? vb : make_float2(0.f, 0.f)); // past the end: adds 0
                 __builtin_amdgcn_sched_barrier(0);
+            }
             }
             // a row block lies either inside the chunk (its v_i in LDS) or before it (i0 + 16 <= c0)
             if (i0 >= c0) {
@@ -222,9 +249,10 @@ __global__ __launch_bounds__(256) void bf_plain_kernel(PlainArgs a)
     }
 }
 
-template __global__ void bf_plain_kernel<64>(PlainArgs);
-template __global__ void bf_plain_kernel<128>(PlainArgs);
-template __global__ void bf_plain_kernel<256>(PlainArgs);
+template __global__ void bf_plain_kernel<64, false>(PlainArgs);
+template __global__ void bf_plain_kernel<128, false>(PlainArgs);
+template __global__ void bf_plain_kernel<256, false>(PlainArgs);
+template __global__ void bf_plain_kernel<64, true>(PlainArgs);
 
 // -------------------------------------------------------------------------------------------
 // float64 form: the plain kernel's structure with float64 steering vectors, CSM and sums, for

@@ -103,7 +103,8 @@ struct CustomArgs {
 __global__ void prep_plain_kernel(const float2 *csm, int M, int Mp, int r_diag, float2 *ut);
 // microphones per LDS chunk of the plain kernel unless ABF_PLAIN_JC says otherwise
 constexpr int kPlainDefaultJC = 64;
-template <int JC> __global__ void bf_plain_kernel(PlainArgs a);
+// LU: U staged in LDS (read by every lane) instead of broadcast through scalar loads
+template <int JC, bool LU> __global__ void bf_plain_kernel(PlainArgs a);
 __global__ void prep_plain64_kernel(const double2 *csm, int M, int Mp, int r_diag, double2 *ut);
 __global__ void bf_plain64_kernel(Plain64Args a);
 __global__ void prep_mfma_kernel(const float2 *csm, int M, int Mp, int r_diag, int prec, const double *uscale,

@@ -122,6 +122,28 @@ def test_many_mics(gpu, form):
     assert err <= _tol(form)
 
 
+@pytest.mark.parametrize('env', [{'ABF_PLAIN_JC': '128'}, {'ABF_PLAIN_JC': '256'}, {'ABF_PLAIN_LDSU': '1'}])
+def test_plain_kernel_variants(gpu, monkeypatch, env):
+    """The plain kernel's other microphone chunk sizes (partial sums split differently: within
+    1e-6 of the peak of the default kernel's maps) and its LDS-staged U form (the same accumulation
+    order: bit for bit) against the default kernel, and all within the gate: a ragged microphone
+    count (more than one chunk, a partial last chunk), both r_diag settings."""
+    freqs = np.array([1500.0, 8000.0, 15000.0])
+    grid = aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.3, y_max=0.3, z=0.5, increment=0.02)
+    mics, grid, csm = _scene(300, grid, freqs, z=0.5, seed=8)
+    for steer, r_diag in (('inverse', True), ('true level', False)):
+        want = _gpu_maps(csm, freqs, mics, grid, steer, r_diag, 'plain_f32')
+        with monkeypatch.context() as m:
+            for k, v in env.items():
+                m.setenv(k, v)
+            got = _gpu_maps(csm, freqs, mics, grid, steer, r_diag, 'plain_f32')
+        if 'ABF_PLAIN_LDSU' in env:
+            np.testing.assert_array_equal(got, want)
+        else:
+            assert peak_rel_err(got, want) <= 1e-6
+        assert peak_rel_err(got, _oracle_maps(csm, freqs, mics, grid, steer, r_diag)) <= TOL
+
+
 def test_reference_point_and_distance(gpu):
     freqs = np.array([3000.0])
     grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.4, increment=0.05)
