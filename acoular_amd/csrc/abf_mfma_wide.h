@@ -300,15 +300,6 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(1
                     if (lane < 16) s_q[((f - f0) & 1) * 2 * kWdBN + p * kWdBN + (2 * cp + c) * 16 + lane] = v;
                 }
             }
-#ifdef ABF_WD_VPM
-            if constexpr (sub.first[t]) {
-                constexpr int nmf = (sub.I0[t] - sub.I1[t]) * 24;
-                st_for([&](auto) __attribute__((always_inline)) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x002, ABF_WD_VPM, 0);
-                }, std::make_integer_sequence<int, nmf>{});
-            }
-#endif
             __syncthreads();
         };
         st_for(substep, std::make_integer_sequence<int, NS>{});
