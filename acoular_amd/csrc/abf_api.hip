@@ -1010,12 +1010,13 @@ int abf_dist_mat(const double *gpos, const double *mpos, int64_t num_grid, int32
     if (num_grid == 0) return ABF_OK;
     if (!gpos || !mpos || !rm) return fail(ABF_ERR_INVALID, "null pointer");
     const int64_t nmb = cdiv((int64_t)num_mics, 512);
-    const int64_t per = std::max<int64_t>(1, (int64_t(1) << 22) / nmb); // points per launch
+    // points per launch: at most 2^22 workgroups of kDistPts points
+    const int64_t per = std::max<int64_t>(1, (int64_t(1) << 22) / nmb) * kDistPts;
     // two distances per 16-byte store when every row starts 16-byte aligned
     const int vec2 = !(num_mics & 1) && aligned16(rm);
     for (int64_t g0 = 0; g0 < num_grid; g0 += per)
-        hipLaunchKernelGGL(dist_mat_kernel, dim3((unsigned)(std::min(per, num_grid - g0) * nmb)), dim3(256), 0,
-                           S(stream), gpos, mpos, num_grid, (int)num_mics, g0, vec2, rm);
+        hipLaunchKernelGGL(dist_mat_kernel, dim3((unsigned)(cdiv(std::min(per, num_grid - g0), kDistPts) * nmb)),
+                           dim3(256), 0, S(stream), gpos, mpos, num_grid, (int)num_mics, g0, vec2, rm);
     return hip_status("dist_mat launch");
 }
 

@@ -5,28 +5,37 @@
 
 namespace abf {
 
-// grid: (points g0 ... of this launch) x ceil(M / 512) workgroups; workgroup (g, mb) writes
-// rm[g][512 mb ...], two microphones per thread (one 16-byte store when M is even, every row then
-// starting on an even element); one division per workgroup, none per element
+// grid: (groups of kDistPts points from g0 on) x ceil(M / 512) workgroups; workgroup (gp, mb)
+// writes rm[g][512 mb ...] for the kDistPts points g of group gp, two microphones per thread (one
+// 16-byte store when M is even, every row then starting on an even element).  The microphones'
+// coordinates stay in registers across the points (the grid coordinates are wave-uniform); one
+// division per workgroup, none per element.
 __global__ __launch_bounds__(256) void dist_mat_kernel(const double *__restrict__ gpos, const double *__restrict__ mpos,
                                                        int64_t G, int M, int64_t g0, int vec2,
                                                        double *__restrict__ rm)
 {
     const int nmb = (M + 511) / 512;
-    const int64_t g = g0 + blockIdx.x / nmb;
+    const int64_t gb = g0 + (int64_t)(blockIdx.x / nmb) * kDistPts;
     const int m = (int)(blockIdx.x % nmb) * 512 + 2 * threadIdx.x;
     if (m >= M) return;
-    const double gx = gpos[g], gy = gpos[G + g], gz = gpos[2 * G + g];
-    const double d0 = dist3(gx, gy, gz, mpos[m], mpos[M + m], mpos[2 * M + m]);
-    if (m + 1 < M) {
-        const double d1 = dist3(gx, gy, gz, mpos[m + 1], mpos[M + m + 1], mpos[2 * M + m + 1]);
-        if (vec2) {
-            *reinterpret_cast<double2 *>(rm + g * M + m) = make_double2(d0, d1);
-            return;
+    const bool two = m + 1 < M;
+    const double mx0 = mpos[m], my0 = mpos[M + m], mz0 = mpos[2 * M + m];
+    const double mx1 = two ? mpos[m + 1] : 0.0, my1 = two ? mpos[M + m + 1] : 0.0, mz1 = two ? mpos[2 * M + m + 1] : 0.0;
+    const int np = (int)min((int64_t)kDistPts, G - gb);
+    for (int i = 0; i < np; ++i) {
+        const int64_t g = gb + i;
+        const double gx = gpos[g], gy = gpos[G + g], gz = gpos[2 * G + g];
+        const double d0 = dist3(gx, gy, gz, mx0, my0, mz0);
+        if (two) {
+            const double d1 = dist3(gx, gy, gz, mx1, my1, mz1);
+            if (vec2) {
+                *reinterpret_cast<double2 *>(rm + g * M + m) = make_double2(d0, d1);
+                continue;
+            }
+            rm[g * M + m + 1] = d1;
         }
-        rm[g * M + m + 1] = d1;
+        rm[g * M + m] = d0;
     }
-    rm[g * M + m] = d0;
 }
 
 // One wave per grid point, the lanes over the microphones (rows of rm read and of out written
