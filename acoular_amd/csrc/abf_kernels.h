@@ -124,20 +124,16 @@ constexpr int kStKSB = 40;               // LDS row stride of a steering block, 
 constexpr int kStPlane = kStBN * kStKSB; // elements per plane of a block
 constexpr int kStBlock = 4 * kStPlane;   // elements per block (re hi, re lo, im hi, im lo)
 constexpr int kStGS = 4;                 // U blocks (pairs) per sub-step
-constexpr int kStKSF = 36;               // LDS row stride of the float32 steering values (16-byte skew)
-constexpr int kStPlaneF = kStBN * kStKSF;
-constexpr int kStBlockF = 2 * kStPlaneF; // re, im
-// LDS layout: steering bf16 [2][4][BN][KSB], steering f32 [2][2][BN][KSF], U ring [2][kStGS][8 KiB],
-// q partials [2 parities][2 halves][BN] f32, normalisation factor [BN] f64, steering norm [BN] f32,
-// r0 [BN] f64, steering-sum partials [8][BN] f64
-constexpr size_t kStOffXf = 2 * (size_t)kStBlock * 2;
-constexpr size_t kStOffU = kStOffXf + 2 * (size_t)kStBlockF * 4;
+// LDS layout: steering bf16 [2][4][BN][KSB], U ring [2][kStGS][8 KiB], q partials [2 parities]
+// [2 halves][BN] f32, normalisation factor [BN] f64, steering norm [BN] f32, r0 [BN] f64,
+// steering-sum partials [2][BN] f64
+constexpr size_t kStOffU = 2 * (size_t)kStBlock * 2;
 constexpr size_t kStOffQ = kStOffU + 2 * (size_t)kStGS * 8192;
 constexpr size_t kStOffFac = kStOffQ + 4 * kStBN * 4;
 constexpr size_t kStOffNst = kStOffFac + kStBN * 8;
 constexpr size_t kStOffR0 = kStOffNst + kStBN * 4;
 constexpr size_t kStOffHelp = kStOffR0 + kStBN * 8;
-constexpr size_t st_lds_bytes() { return kStOffHelp + 8 * kStBN * 8; }
+constexpr size_t st_lds_bytes() { return kStOffHelp + 2 * kStBN * 8; }
 // PS: persistent launch (a grid of at most one workgroup per CU walking the work items)
 template <int NK, int WK, bool PS> __global__ void bf_mfma_stream_kernel(const MfJob *jobs, int njobs, int64_t total);
 

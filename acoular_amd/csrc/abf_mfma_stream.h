@@ -15,7 +15,10 @@
 //    column tiles x 2 halves, 64 grid points per workgroup), so all waves carry the same MFMA
 //    work in every step;
 //  * each thread generates 4 phasors of the next block per step from float64 distances it keeps
-//    in registers for the whole band (4 per block);
+//    in registers for the whole band (4 per block): the 4 values of its own column and row half
+//    that its epilogue needs, whose float32 form it keeps in registers until then (16.27 ms on the
+//    256-mic, 201 x 201, 513-bin workload with a float32 copy of the block in LDS instead, 15.66
+//    ms so);
 //  * the U blocks (both halves, 8 KiB) of the (I, J) pairs enter the workgroup once: a step is
 //    cut into sub-steps of at most 4 pairs, every wave loads its 1 KiB share of the blocks of
 //    sub-step t + 2 into registers and writes the share it loaded one sub-step earlier into a
@@ -81,13 +84,12 @@ template <int NK, int WK>
 __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int njobs, int64_t wg, char *smem)
 {
     uint16_t *xs = reinterpret_cast<uint16_t *>(smem);                   // steering, bf16 [2][4][BN][KSB]
-    float *xf = reinterpret_cast<float *>(smem + kStOffXf);              // steering, f32 [2][2][BN][KSF]
     char *us = smem + kStOffU;                                           // U ring [2][kStGS][8 KiB]
     float *s_q = reinterpret_cast<float *>(smem + kStOffQ);              // q partials [2][2][BN]
     double *s_fac = reinterpret_cast<double *>(smem + kStOffFac);        // normalisation factor [BN]
     float *s_nst = reinterpret_cast<float *>(smem + kStOffNst);          // steering normalisation [BN]
     double *s_r0 = reinterpret_cast<double *>(smem + kStOffR0);          // [BN]
-    double *s_help = reinterpret_cast<double *>(smem + kStOffHelp);      // steering-sum partials [8][BN]
+    double *s_help = reinterpret_cast<double *>(smem + kStOffHelp);      // steering-sum partials [2][BN]
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -100,8 +102,9 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     const int f0 = (int)(local / jb.ntiles) * FB;
     const int f1 = min(f0 + FB, F);
 
-    // ---- generation role: grid point col = lane, K-slots 4 sg .. 4 sg + 3 of every block ----
-    const int col = lane, sgp = wave;
+    // ---- generation role: the values the thread's own epilogue needs: grid point col = 16 (wave & 3)
+    // + (lane & 15), K-slots 8 gq + 4 pg + i (i < 4) of every block, gq = lane >> 4, pg = wave >> 2 ----
+    const int col = 16 * (wave & 3) + (lane & 15), gq = lane >> 4, pg = wave >> 2;
     const int64_t g = (int64_t)tile * kStBN + col;
     const bool valid = g < G;
     double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -117,7 +120,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     for (int J = 0; J < NK; ++J)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int m = 32 * J + 4 * (sgp >> 1) + i + 16 * (sgp & 1);
+            const int m = 32 * J + 4 * gq + i + 16 * pg;
             double r = 1.0;
             if (valid && m < M) {
                 r = dist3(gx, gy, gz, mp[m], mp[M + m], mp[2 * M + m]);
@@ -125,17 +128,28 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
             }
             rr[J][i] = r;
         }
-    s_help[sgp * kStBN + col] = help;
-    if (sgp == 0) {
+    // the four lane groups of a column, then the two waves (pg) through LDS
+    help += __shfl_xor(help, 16);
+    help += __shfl_xor(help, 32);
+    if (gq == 0) s_help[pg * kStBN + col] = help;
+    if (wave == 0) {
+        const int64_t g0 = (int64_t)tile * kStBN + lane;
         double r0 = jb.ref_dist;
-        if (!(r0 > 0.0)) r0 = dist3(gx, gy, gz, jb.ref[0], jb.ref[1], jb.ref[2]);
-        s_r0[col] = r0;
+        if (!(r0 > 0.0))
+            r0 = g0 < G ? dist3(gp(jb.gpos)[g0], gp(jb.gpos)[G + g0], gp(jb.gpos)[2 * G + g0], jb.ref[0], jb.ref[1],
+                                jb.ref[2])
+                        : 1.0;
+        s_r0[lane] = r0;
     }
 
+    // float32 steering values of the epilogue, kept in registers from generation to epilogue: block
+    // J in set es(J) (J & 1; the last block in a third set when NK is odd, since it is generated
+    // for the next frequency while block 0 still waits for its epilogue)
+    float ev[3][8];
     // phasors of a block (distances rb) for wavenumber k into steering buffer b: K-slots
-    // 4 sg .. 4 sg + 3 of column col, as bf16 hi/lo planes (MFMA operand) and float32 (epilogue)
-    auto generate = [&](const double (&rb)[4], double k, int b) __attribute__((always_inline)) {
-        float xr[4], xi[4];
+    // 8 gq + 4 pg .. + 3 of column col, as bf16 hi/lo planes (MFMA operand) and float32 (epilogue)
+    auto generate = [&](const double (&rb)[4], double k, int b, float (&xe)[8]) __attribute__((always_inline)) {
+        float *xr = xe, *xi = xe + 4;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             double r = rb[i];
@@ -172,20 +186,17 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
                 l[2 * d + q] = __builtin_bit_cast(uint32_t, lp);
             }
         }
-        uint16_t *dst = xs + b * kStBlock + col * kStKSB + 4 * sgp;
+        uint16_t *dst = xs + b * kStBlock + col * kStKSB + 8 * gq + 4 * pg;
         *reinterpret_cast<uint2 *>(dst) = make_uint2(h[0], h[1]);
         *reinterpret_cast<uint2 *>(dst + kStPlane) = make_uint2(l[0], l[1]);
         *reinterpret_cast<uint2 *>(dst + 2 * kStPlane) = make_uint2(h[2], h[3]);
         *reinterpret_cast<uint2 *>(dst + 3 * kStPlane) = make_uint2(l[2], l[3]);
-        float *fd = xf + b * kStBlockF + col * kStKSF + 4 * sgp;
-        *reinterpret_cast<f4 *>(fd) = f4{xr[0], xr[1], xr[2], xr[3]};
-        *reinterpret_cast<f4 *>(fd + kStPlaneF) = f4{xi[0], xi[1], xi[2], xi[3]};
     };
+    constexpr auto es = [](int J) constexpr { return (NK & 1) && J == NK - 1 ? 2 : (J & 1); };
 
     // ---- MFMA role: column tile c, row half p of every row block ----
     const int c = wave & 3, p = wave >> 2;
     const int xoff = (c * 16 + (lane & 15)) * kStKSB + (lane >> 4) * 8;             // B fragment
-    const int foff = (c * 16 + (lane & 15)) * kStKSF + (lane >> 4) * 8 + p * 4;     // epilogue values
     const ABF_GLOBAL char *prep_base = gp(reinterpret_cast<const char *>(jb.prep));
     constexpr int64_t kFreqBytes = (int64_t)(NK * (NK + 1) / 2) * 8192;
     constexpr StSub<NK> sub;
@@ -221,7 +232,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     u4 stage[2][kStGS];
 
     // first block of the band; U of its sub-step 0 into LDS buffer 0, of sub-step 1 into registers
-    generate(rr[NK - 1], gp(jb.wavenumber)[f0], 0);
+    generate(rr[NK - 1], gp(jb.wavenumber)[f0], 0, ev[es(NK - 1)]);
     load_sub(std::integral_constant<int, 0>{}, f0, stage[0]);
     load_sub(std::integral_constant<int, 1>{}, f0, stage[1]);
     write_sub(std::integral_constant<int, 0>{}, 0, stage[0]);
@@ -230,7 +241,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     // result = q * fac, normalizeSteer = nst
     if (wave == 0) {
         double hs = 0.0;
-        for (int q = 0; q < 8; ++q) hs += s_help[q * kStBN + lane];
+        for (int q = 0; q < 2; ++q) hs += s_help[q * kStBN + lane];
         double fac, nst;
         normalise(jb.steer_type, 1.0, s_r0[lane], hs, (double)M, jb.norm, fac, nst);
         s_fac[lane] = fac;
@@ -290,9 +301,9 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
             if constexpr (sub.first[t]) {
                 // next block: J - 1 of this frequency, or the last block of the next one
                 if constexpr (J > 0)
-                    generate(rr[J > 0 ? J - 1 : 0], kcur, bcur ^ 1);
+                    generate(rr[J > 0 ? J - 1 : 0], kcur, bcur ^ 1, ev[es(J > 0 ? J - 1 : 0)]);
                 else
-                    generate(rr[NK - 1], knext, bcur ^ 1);
+                    generate(rr[NK - 1], knext, bcur ^ 1, ev[es(NK - 1)]);
             }
             };
             auto mfma = [&]() __attribute__((always_inline)) {
@@ -324,14 +335,13 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
 #undef MF
                 if (q == 0 && sub.first[t]) {
                     // pair (J, J) closes row block J: q += Re sum conj(x) Y over its rows 4 g + i of
-                    // half p, whose float32 steering values sit at K-slots 8 g + 4 p + i
-                    const float *fb = xf + bcur * kStBlockF + foff;
-                    const f4 xr = *reinterpret_cast<const f4 *>(fb);
-                    const f4 xi = *reinterpret_cast<const f4 *>(fb + kStPlaneF);
+                    // half p, whose float32 steering values (K-slots 8 g + 4 p + i) this thread
+                    // generated itself
+                    const float *xe = ev[es(J)];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        qacc = fmaf(xr[e], accr[J][e], qacc);
-                        qacc = fmaf(xi[e], acci[J][e], qacc);
+                        qacc = fmaf(xe[e], accr[J][e], qacc);
+                        qacc = fmaf(xe[4 + e], acci[J][e], qacc);
                     }
                     accr[J] = f4{0.f, 0.f, 0.f, 0.f};
                     acci[J] = f4{0.f, 0.f, 0.f, 0.f};
