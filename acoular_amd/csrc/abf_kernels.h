@@ -123,7 +123,10 @@ constexpr int kStBN = 64;                // grid points per workgroup
 constexpr int kStKSB = 40;               // LDS row stride of a steering block, bf16 elements (32 + 16-byte skew)
 constexpr int kStPlane = kStBN * kStKSB; // elements per plane of a block
 constexpr int kStBlock = 4 * kStPlane;   // elements per block (re hi, re lo, im hi, im lo)
-constexpr int kStGS = 4;                 // U blocks (pairs) per sub-step
+#ifndef ABF_ST_GS
+#define ABF_ST_GS 4
+#endif
+constexpr int kStGS = ABF_ST_GS;         // U blocks (pairs) per sub-step (3, 5: 2 % slower, 6 spills)
 // LDS layout: steering bf16 [2][4][BN][KSB], U ring [2][kStGS][8 KiB], q partials [2 parities]
 // [2 halves][BN] f32, normalisation factor [BN] f64, steering norm [BN] f32, r0 [BN] f64,
 // steering-sum partials [2][BN] f64
