@@ -174,7 +174,7 @@ def api_bench(args):
     got = csm[[3, 700]].cpu().numpy()
     iu = np.triu_indices(M)
     err_csm = float(np.abs(got[:, iu[0], iu[1]] - want[:, iu[0], iu[1]]).max() / np.abs(want).max())
-    t_csm = event_time(step_csm, 20)
+    t_csm = event_time(step_csm, 20, warm=3)
     b_csm = F * (M * (M + 1) // 2) * 16.0 + F * M * 8.0
     log(f'calcCSM {F} x {M} x {M} complex64: {t_csm * 1e3:.3f} ms per call, {b_csm / t_csm / 1e9:.0f} GB/s, '
         f'max error {err_csm:.1e}')
@@ -193,7 +193,7 @@ def api_bench(args):
     a = B.calc_transfer(r0.cpu().numpy()[sel], rm.cpu().numpy()[sel], k)
     want_tr = B.steer_vector(a, 'true level')
     err_tr = float(np.abs(tr.cpu().numpy()[sel] - want_tr).max() / np.abs(want_tr).max())
-    t_tr = event_time(step_tr, 20)
+    t_tr = event_time(step_tr, 20, warm=3)
     b_tr = G * 8.0 + G * Mt * 8.0 + G * Mt * 16.0
     log(f'steer_vector {G} x {Mt} complex128: {t_tr * 1e3:.3f} ms, {b_tr / t_tr / 1e9:.0f} GB/s, max error {err_tr:.1e}')
     # dist_mat
@@ -209,7 +209,7 @@ def api_bench(args):
     seld = np.arange(0, Gd, 4099)
     want_dm = B.dist_mat(gpos.cpu().numpy()[:, seld], mpos.cpu().numpy())
     err_dm = float(np.abs(rmd.cpu().numpy()[seld] - want_dm).max() / want_dm.max())
-    t_dm = event_time(step_dm, 20)
+    t_dm = event_time(step_dm, 20, warm=3)
     b_dm = Gd * Md * 8.0 + (Gd + Md) * 24.0
     log(f'dist_mat {Gd} x {Md}: {t_dm * 1e3:.3f} ms, {b_dm / t_dm / 1e9:.0f} GB/s, max error {err_dm:.1e}')
     roof = {'bound': 'hbm', 'achieved': round(b_csm / t_csm / 1e9, 1), 'peak': 8000.0, 'unit': 'GB/s',
@@ -253,13 +253,16 @@ def time_steps(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def event_time(fn, reps):
-    """Average device time of ``fn`` from HIP events on the current (launch) stream."""
+def event_time(fn, reps, warm=0):
+    """Average device time of ``fn`` from HIP events on the current (launch) stream, after
+    ``warm`` untimed calls (the GPU idles while the host checks results before a timing)."""
     import torch
 
     st = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    for _ in range(warm):
+        fn()
     torch.cuda.synchronize()
     e0.record(st)
     for _ in range(reps):
