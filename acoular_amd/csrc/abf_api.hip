@@ -1047,24 +1047,6 @@ int abf_calc_csm(void *csm, const void *spec, int32_t num_freqs, int32_t num_mic
     const int M = num_mics;
     // complex64: two columns per 16-byte access when every row starts 16-byte aligned
     const int vec2 = !(M & 1) && aligned16(csm) && aligned16(spec);
-    if ((is_c128 || vec2) && env_int("ABF_CSM_ROWS", 0) == 0) {
-        // row pairs of equal length, at least one unit per thread (ABF_CSM_ROWS=1: one workgroup
-        // per row, for comparison)
-        const int64_t npairs = (M + 1) / 2, per = is_c128 ? M + 1 : M / 2 + 1;
-        const int FC = (int)std::max<int64_t>(1, 256 / per);
-        const int64_t span = (int64_t)FC * 65535;
-        for (int64_t f0 = 0; f0 < num_freqs; f0 += span) {
-            const int nf = (int)std::min<int64_t>(span, num_freqs - f0);
-            const dim3 grid((unsigned)npairs, (unsigned)cdiv(nf, FC));
-            if (is_c128)
-                hipLaunchKernelGGL(calc_csm_pairs_kernel<true>, grid, dim3(256), 0, S(stream), csm, spec, (int)f0, nf,
-                                   M, FC);
-            else
-                hipLaunchKernelGGL(calc_csm_pairs_kernel<false>, grid, dim3(256), 0, S(stream), csm, spec, (int)f0,
-                                   nf, M, FC);
-        }
-        return hip_status("calc_csm launch");
-    }
     for (int f0 = 0; f0 < num_freqs; f0 += 65535) {
         const unsigned nf = (unsigned)std::min(65535, num_freqs - f0);
         if (is_c128)

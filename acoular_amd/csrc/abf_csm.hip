@@ -23,20 +23,6 @@ __global__ void fill_zero_kernel(float4 *__restrict__ p, int64_t n4)
     if (i < n4) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// csm element (vx, vy) += conj(spec[c]) spec[r] for column value (cx, cy) and row value (rx, ry),
-// in one fixed order of fused multiply-adds shared by every calcCSM launch shape (the shapes give
-// identical bits)
-template <class T> __device__ __forceinline__ void csm_acc(T &vx, T &vy, T cx, T cy, T rx, T ry)
-{
-    vx = __builtin_fma(cx, rx, __builtin_fma(cy, ry, vx));
-    vy = __builtin_fma(cx, ry, __builtin_fma(-cy, rx, vy));
-}
-__device__ __forceinline__ void csm_acc(float &vx, float &vy, float cx, float cy, float rx, float ry)
-{
-    vx = __builtin_fmaf(cx, rx, __builtin_fmaf(cy, ry, vx));
-    vy = __builtin_fmaf(cx, ry, __builtin_fmaf(-cy, rx, vy));
-}
-
 // calcCSM update, csm[f][r][c] += conj(spec[f][c]) spec[f][r] for r <= c.  grid: (ceil(M / 512)
 // column blocks, M rows, frequencies f0 ... of this launch): two adjacent columns per thread, as
 // one 16-byte access when M is even (every row and spectrum then starts on an even element);
@@ -54,8 +40,10 @@ __global__ __launch_bounds__(256) void calc_csm_c64_kernel(float2 *__restrict__ 
     if (vec2 && c >= r) { // both columns on or above the diagonal, c + 1 < M
         float4 v = *reinterpret_cast<const float4 *>(row + c);
         const float4 sc = *reinterpret_cast<const float4 *>(sp + c);
-        csm_acc(v.x, v.y, sc.x, sc.y, sr.x, sr.y);
-        csm_acc(v.z, v.w, sc.z, sc.w, sr.x, sr.y);
+        v.x += sc.x * sr.x + sc.y * sr.y;
+        v.y += sc.x * sr.y - sc.y * sr.x;
+        v.z += sc.z * sr.x + sc.w * sr.y;
+        v.w += sc.z * sr.y - sc.w * sr.x;
         *reinterpret_cast<float4 *>(row + c) = v;
         return;
     }
@@ -63,7 +51,8 @@ __global__ __launch_bounds__(256) void calc_csm_c64_kernel(float2 *__restrict__ 
         if (cc < r) continue;
         const float2 sc = sp[cc];
         float2 v = row[cc];
-        csm_acc(v.x, v.y, sc.x, sc.y, sr.x, sr.y);
+        v.x += sc.x * sr.x + sc.y * sr.y;
+        v.y += sc.x * sr.y - sc.y * sr.x;
         row[cc] = v;
     }
 }
@@ -77,104 +66,10 @@ __global__ __launch_bounds__(256) void calc_csm_c128_kernel(double2 *__restrict_
     const double2 sr = spec[f * M + r], sc = spec[f * M + c];
     double2 *p = csm + (f * M + r) * M + c;
     double2 v = *p;
-    csm_acc(v.x, v.y, sc.x, sc.y, sr.x, sr.y);
+    v.x += sc.x * sr.x + sc.y * sr.y;
+    v.y += sc.x * sr.y - sc.y * sr.x;
     *p = v;
 }
-
-// kCsmUnroll units of 16 bytes of the calcCSM update: all loads first, then the stores.  Unit k
-// starts at element off[k] of csm (row element roff[k] and column element soff[k] of spec); a
-// complex64 unit with half[k] set straddles the diagonal (columns row - 1 and row) and updates
-// its upper element only.
-template <bool C128>
-__device__ __forceinline__ void csm_update_units(void *__restrict__ csm_, const void *__restrict__ spec_,
-                                                 const int64_t *off, const int64_t *soff, const int64_t *roff,
-                                                 const bool *half, const bool *ok)
-{
-    if constexpr (C128) {
-        double2 *csm = reinterpret_cast<double2 *>(csm_);
-        const double2 *spec = reinterpret_cast<const double2 *>(spec_);
-        double2 val[kCsmUnroll], sc[kCsmUnroll], sr[kCsmUnroll];
-#pragma unroll
-        for (int k = 0; k < kCsmUnroll; ++k)
-            if (ok[k]) {
-                val[k] = csm[off[k]];
-                sc[k] = spec[soff[k]];
-                sr[k] = spec[roff[k]];
-            }
-#pragma unroll
-        for (int k = 0; k < kCsmUnroll; ++k)
-            if (ok[k]) {
-                csm_acc(val[k].x, val[k].y, sc[k].x, sc[k].y, sr[k].x, sr[k].y);
-                csm[off[k]] = val[k];
-            }
-    } else {
-        float2 *csm = reinterpret_cast<float2 *>(csm_);
-        const float2 *spec = reinterpret_cast<const float2 *>(spec_);
-        float4 val[kCsmUnroll], sc[kCsmUnroll];
-        float2 sr[kCsmUnroll];
-#pragma unroll
-        for (int k = 0; k < kCsmUnroll; ++k)
-            if (ok[k]) {
-                val[k] = *reinterpret_cast<const float4 *>(csm + off[k]);
-                sc[k] = *reinterpret_cast<const float4 *>(spec + soff[k]);
-                sr[k] = spec[roff[k]];
-            }
-#pragma unroll
-        for (int k = 0; k < kCsmUnroll; ++k)
-            if (ok[k]) {
-                float4 v = val[k];
-                csm_acc(v.z, v.w, sc[k].z, sc[k].w, sr[k].x, sr[k].y);
-                if (half[k]) {
-                    *reinterpret_cast<float2 *>(csm + off[k] + 1) = make_float2(v.z, v.w);
-                } else {
-                    csm_acc(v.x, v.y, sc[k].x, sc[k].y, sr[k].x, sr[k].y);
-                    *reinterpret_cast<float4 *>(csm + off[k]) = v;
-                }
-            }
-    }
-}
-
-// calcCSM update with equal work per workgroup: workgroup x takes row r = x from its diagonal to
-// the end and then row s = M - 1 - r the same way, together M / 2 + 1 units of 16 bytes (complex64,
-// M even: two columns per unit, rows 16-byte aligned) or M + 1 units (complex128: one column), for
-// the FC frequencies of blockIdx.y; consecutive threads take consecutive units of a row.  On 1025
-// bins x 512 mics, complex64: 0.48-0.50 ms against 0.50 ms for one workgroup per row (about half
-// of its threads exit at once) and 0.52 ms for workgroups of 1024 consecutive units in memory
-// order; FC = 2, 4, 16: 0.52, 0.54, 0.57 ms; complex128: 0.97 ms in both launch shapes.  An
-// in-place scale of a dense 1 GiB array reaches 5.97 TB/s on the same GPU, this update 4.3-4.5.
-template <bool C128>
-__global__ __launch_bounds__(256) void calc_csm_pairs_kernel(void *__restrict__ csm_, const void *__restrict__ spec_,
-                                                             int f0, int nf, int M, int FC)
-{
-    const int r = blockIdx.x, s = M - 1 - r;
-    const int W = C128 ? M : M / 2;
-    const int ar = C128 ? r : r >> 1, as = C128 ? s : s >> 1; // first unit of each row
-    const int ur = W - ar, per = ur + (s == r ? 0 : W - as);
-    const int fb = f0 + (int)blockIdx.y * FC;
-    const int total = (min(fb + FC, f0 + nf) - fb) * per;
-    for (int v0 = threadIdx.x; v0 < total; v0 += 256 * kCsmUnroll) {
-        int64_t off[kCsmUnroll], soff[kCsmUnroll], roff[kCsmUnroll];
-        bool half[kCsmUnroll], ok[kCsmUnroll];
-#pragma unroll
-        for (int k = 0; k < kCsmUnroll; ++k) {
-            const int v = v0 + 256 * k;
-            ok[k] = v < total;
-            const int fi = v / per, w = v - fi * per;
-            const int64_t f = fb + fi;
-            const int row = w < ur ? r : s;
-            const int u = w < ur ? ar + w : as + (w - ur);
-            const int c = C128 ? u : 2 * u; // first column of the unit
-            half[k] = !C128 && c < row;
-            off[k] = (f * M + row) * M + c;
-            soff[k] = f * M + c;
-            roff[k] = f * M + row;
-        }
-        csm_update_units<C128>(csm_, spec_, off, soff, roff, half, ok);
-    }
-}
-template __global__ void calc_csm_pairs_kernel<false>(void *, const void *, int, int, int, int);
-template __global__ void calc_csm_pairs_kernel<true>(void *, const void *, int, int, int, int);
-
 
 __device__ __forceinline__ int bitrev(int n, int bits) { return (int)(__builtin_bitreverse32((uint32_t)n) >> (32 - bits)); }
 

@@ -51,11 +51,6 @@ constexpr size_t kFftRadixLdsMax = 160 * 1024;
 __global__ void fill_zero_kernel(float4 *p, int64_t n4);
 __global__ void calc_csm_c64_kernel(float2 *csm, const float2 *spec, int f0, int M, int vec2);
 __global__ void calc_csm_c128_kernel(double2 *csm, const double2 *spec, int f0, int M);
-template <bool C128>
-__global__ void calc_csm_pairs_kernel(void *csm, const void *spec, int f0, int nf, int M, int FC);
-// units of 16 bytes per thread and loop step of the row-pair calcCSM update (1025 bins x 512 mics,
-// complex64: 1, 2, 4, 8 units 0.57, 0.52, 0.48-0.50, 0.92 ms)
-constexpr int kCsmUnroll = 4;
 __global__ void welch_fft_kernel(WelchFftArgs a);
 template <int LOGN, bool VEC> __global__ void welch_fft_radix_kernel(WelchFftArgs a);
 template <int TR>

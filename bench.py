@@ -213,7 +213,7 @@ def api_bench(args):
     b_dm = Gd * Md * 8.0 + (Gd + Md) * 24.0
     log(f'dist_mat {Gd} x {Md}: {t_dm * 1e3:.3f} ms, {b_dm / t_dm / 1e9:.0f} GB/s, max error {err_dm:.1e}')
     roof = {'bound': 'hbm', 'achieved': round(b_csm / t_csm / 1e9, 1), 'peak': 8000.0, 'unit': 'GB/s',
-            'kernel': 'calc_csm_pairs_kernel<false>', 'algorithmic_bytes_per_launch': b_csm, 'traffic': None}
+            'kernel': 'calc_csm_c64_kernel', 'algorithmic_bytes_per_launch': b_csm, 'traffic': None}
     roof['frac'] = round(roof['achieved'] / roof['peak'], 4)
     cpu = None
     if not args.no_cpu:

@@ -243,27 +243,6 @@ def test_calc_csm_dropin(gpu, dtype, nm):
     np.testing.assert_allclose(got, want, rtol=0, atol=tol * np.abs(want).max())
 
 
-@pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
-@pytest.mark.parametrize('nm', [1, 2, 64, 65])
-def test_calc_csm_launch_shapes(gpu, dtype, nm, monkeypatch):
-    """The row-pair launch (several bins per workgroup below 512 mics, a ragged last group of
-    bins: 301 bins) against the oracle, and bit for bit against the one-row-per-workgroup launch."""
-    rng = np.random.default_rng(5)
-    nf = 301
-    spec = (rng.standard_normal((nf, nm)) + 1j * rng.standard_normal((nf, nm))).astype(dtype)
-    csm = (rng.standard_normal((nf, nm, nm)) + 1j * rng.standard_normal((nf, nm, nm))).astype(dtype)
-    want = S.calc_csm_update(csm.astype(np.complex128).copy(), spec.astype(np.complex128))
-    got = {}
-    for shape, env in (('pairs', None), ('rows', 'ABF_CSM_ROWS')):
-        with monkeypatch.context() as m:
-            if env:
-                m.setenv(env, '1')
-            got[shape] = aa.calcCSM(csm.copy(), spec)
-    tol = 1e-12 if dtype == np.complex128 else 2e-6
-    np.testing.assert_allclose(got['pairs'], want, rtol=0, atol=tol * np.abs(want).max())
-    np.testing.assert_array_equal(got['pairs'], got['rows'])
-
-
 def test_even_counts_on_views_that_start_off_16_bytes(gpu):
     """calcCSM (complex64) and dist_mat with an even count on caller views that start 8 bytes
     past a 16-byte boundary: the library takes the one-column path and the results stay exact."""
