@@ -602,9 +602,12 @@ def test_streamed_kernel_falls_back_outside_its_cases(gpu, kernel):
         sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='true level')
         gpos, mpos = sv.device_geometry()
         out = torch.empty((len(freqs), grid.size), dtype=torch.float32, device=gpu)
-        job = make_job(_lib.to_device(csm, torch.complex64), torch.from_numpy(2 * np.pi * freqs / 343.0).to(gpu),
-                       gpos, mpos, sv.ref, 'true level', True, nm / (nm - 1.0), 'mfma_bf16x3', phase, out, None, True,
-                       fast_trig=fast)
+        # the job holds raw pointers: the device CSM and wavenumbers must stay referenced until the
+        # launch is done, or the workspace allocated below may reuse their memory
+        dcsm = _lib.to_device(csm, torch.complex64)
+        k = torch.from_numpy(2 * np.pi * freqs / 343.0).to(gpu)
+        job = make_job(dcsm, k, gpos, mpos, sv.ref, 'true level', True, nm / (nm - 1.0), 'mfma_bf16x3', phase, out,
+                       None, True, fast_trig=fast)
         job.tile_kernel, job.tile_cols, job.tile_freqs = kernel, 64, 2
         plan = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()]
         _lib.check(lib.abf_beamform_plan(job, *[ctypes.byref(x) for x in plan]))
