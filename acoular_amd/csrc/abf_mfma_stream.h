@@ -20,11 +20,13 @@
 //    256-mic, 201 x 201, 513-bin workload with a float32 copy of the block in LDS instead, 15.66
 //    ms so);
 //  * the U blocks (both halves, 8 KiB) of the (I, J) pairs enter the workgroup once: a step is
-//    cut into sub-steps of at most 4 pairs, every wave loads its 1 KiB share of the blocks of
-//    sub-step t + 2 into registers and writes the share it loaded one sub-step earlier into a
-//    three-deep LDS ring, and one barrier per sub-step hands the ring over.  The four waves of a
-//    half read their fragments from LDS (global loads of the same fragment by four waves would
-//    go through the CU's L1 four times, the bound of the first form of this kernel);
+//    cut into sub-steps of at most 4 pairs; at the start of sub-step t every wave issues its
+//    1 KiB share of the blocks of sub-step t + 1 as direct global-to-LDS loads
+//    (global_load_lds_dwordx4, no registers) into the other buffer of a two-slot LDS ring, and
+//    the barrier that ends the sub-step (which waits for them) hands the ring over (15.52 against
+//    15.76-15.84 ms with the shares staged through registers two sub-steps ahead).  The four
+//    waves of a half read their fragments from LDS (global loads of the same fragment by four
+//    waves would go through the CU's L1 four times, the bound of the first form of this kernel);
 //  * the sign of -Ui Xi is taken from a negated copy of the steering fragment, made once per
 //    step, instead of flipping the U fragment of every pair.
 //
@@ -202,26 +204,6 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     constexpr StSub<NK> sub;
     constexpr int NS = sub.n;
     static_assert(NS % 2 == 0, "an even number of sub-steps per frequency");
-    // this wave's 1 KiB share (half w >> 2, plane w & 3) of U block (I, J) of frequency f
-    auto load_share = [&](int f, int I, int J) __attribute__((always_inline)) {
-        const int ui = I * NK - (I * (I - 1)) / 2 + (J - I);
-        return *reinterpret_cast<const ABF_GLOBAL u4 *>(prep_base + (int64_t)f * kFreqBytes + (int64_t)ui * 8192 +
-                                                         wave * 1024 + lane * 16);
-    };
-    // shares of the blocks of sub-step ts (of frequency ff) into a register set; the pairs of a
-    // sub-step are I = I0 - q for q = 0 .. n - 1 (descending)
-    auto load_sub = [&](auto tsc, int ff, u4 (&st)[kStGS]) __attribute__((always_inline)) {
-        constexpr int ts = decltype(tsc)::value;
-#pragma unroll
-        for (int q = 0; q < sub.I0[ts] - sub.I1[ts]; ++q) st[q] = load_share(ff, sub.I0[ts] - q, sub.J[ts]);
-    };
-    auto write_sub = [&](auto tsc, int buf, const u4 (&st)[kStGS]) __attribute__((always_inline)) {
-        constexpr int ts = decltype(tsc)::value;
-#pragma unroll
-        for (int q = 0; q < sub.I0[ts] - sub.I1[ts]; ++q)
-            *reinterpret_cast<u4 *>(us + (buf * kStGS + q) * 8192 + wave * 1024 + lane * 16) = st[q];
-    };
-
     f4 accr[NK], acci[NK];
 #pragma unroll
     for (int I = 0; I < NK; ++I) {
@@ -229,13 +211,26 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
         acci[I] = f4{0.f, 0.f, 0.f, 0.f};
     }
     const u4 sg = u4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-    u4 stage[2][kStGS];
+    // this wave's 1 KiB share (half w >> 2, plane w & 3) of the U blocks of sub-step ts (pairs
+    // I = I0 - q, descending) of frequency ff, straight from global memory into LDS buffer buf:
+    // each lane loads 16 bytes, which land at the wave-uniform LDS address + 16 x lane
+    auto dma_sub = [&](auto tsc, int ff, int buf) __attribute__((always_inline)) {
+        constexpr int ts = decltype(tsc)::value;
+#pragma unroll
+        for (int q = 0; q < sub.I0[ts] - sub.I1[ts]; ++q) {
+            const int I = sub.I0[ts] - q, J = sub.J[ts];
+            const int ui = I * NK - (I * (I - 1)) / 2 + (J - I);
+            const ABF_GLOBAL char *src =
+                prep_base + (int64_t)ff * kFreqBytes + (int64_t)ui * 8192 + wave * 1024 + lane * 16;
+            char *dst = us + (buf * kStGS + q) * 8192 + wave * 1024;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                             (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+        }
+    };
 
-    // first block of the band; U of its sub-step 0 into LDS buffer 0, of sub-step 1 into registers
+    // first block of the band; U of its sub-step 0 into LDS buffer 0
     generate(rr[NK - 1], gp(jb.wavenumber)[f0], 0, ev[es(NK - 1)]);
-    load_sub(std::integral_constant<int, 0>{}, f0, stage[0]);
-    load_sub(std::integral_constant<int, 1>{}, f0, stage[1]);
-    write_sub(std::integral_constant<int, 0>{}, 0, stage[0]);
+    dma_sub(std::integral_constant<int, 0>{}, f0, 0);
     __syncthreads();
     // frequency-independent normalisation of every grid point (fastFuncs.py:293-310):
     // result = q * fac, normalizeSteer = nst
@@ -286,18 +281,11 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
                 xnh = xih ^ sg;
                 xnl = xil ^ sg;
             }
-            // the VALU work (U staging, steering generation) and the MFMA work of a sub-step in
-            // opposite orders in the two waves that share a SIMD (waves w and w + 4, or w and w ^ 1,
+            // the VALU work (steering generation) and the MFMA work of a sub-step in opposite
+            // orders in the two waves that share a SIMD (waves w and w + 4, or w and w ^ 1,
             // depending on how the workgroup is laid out over the SIMDs: the phase bit below differs
             // within both pairs), so that one issues MFMAs while the other runs VALU
             auto stage_gen = [&]() __attribute__((always_inline)) {
-            // U of sub-step t + 1 (loaded one sub-step ago) into the LDS ring; loads of t + 2
-            {
-                constexpr int t1 = t + 1 < NS ? t + 1 : t + 1 - NS;
-                constexpr int t2 = t + 2 < NS ? t + 2 : t + 2 - NS;
-                write_sub(std::integral_constant<int, t1>{}, ub ^ 1, stage[(t + 1) & 1]);
-                load_sub(std::integral_constant<int, t2>{}, t + 2 < NS ? f : fn, stage[t & 1]);
-            }
             if constexpr (sub.first[t]) {
                 // next block: J - 1 of this frequency, or the last block of the next one
                 if constexpr (J > 0)
@@ -348,6 +336,8 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
                 }
             }
             };
+            // U of sub-step t + 1 (of the next frequency after the last one) into the other buffer
+            dma_sub(std::integral_constant<int, (t + 1 < NS ? t + 1 : 0)>{}, t + 1 < NS ? f : fn, ub ^ 1);
             if (((wave & 1) ^ (wave >> 2)) == 0) {
                 stage_gen();
                 mfma();
