@@ -144,10 +144,10 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
         s_r0[lane] = r0;
     }
 
-    // float32 steering values of the epilogue, kept in registers from generation to epilogue: block
-    // J in set es(J) (J & 1; the last block in a third set when NK is odd, since it is generated
-    // for the next frequency while block 0 still waits for its epilogue)
-    float ev[3][8];
+    // float32 steering values of the epilogue, kept in registers from generation to epilogue: one
+    // set, since the epilogue of block J runs before block J - 1 is generated (both in the first
+    // sub-step of step J, MFMAs first)
+    float ev[8];
     // phasors of a block (distances rb) for wavenumber k into steering buffer b: K-slots
     // 8 gq + 4 pg .. + 3 of column col, as bf16 hi/lo planes (MFMA operand) and float32 (epilogue)
     auto generate = [&](const double (&rb)[4], double k, int b, float (&xe)[8]) __attribute__((always_inline)) {
@@ -194,7 +194,6 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
         *reinterpret_cast<uint2 *>(dst + 2 * kStPlane) = make_uint2(h[2], h[3]);
         *reinterpret_cast<uint2 *>(dst + 3 * kStPlane) = make_uint2(l[2], l[3]);
     };
-    constexpr auto es = [](int J) constexpr { return (NK & 1) && J == NK - 1 ? 2 : (J & 1); };
 
     // ---- MFMA role: column tile c, row half p of every row block ----
     const int c = wave & 3, p = wave >> 2;
@@ -229,7 +228,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     };
 
     // first block of the band; U of its sub-step 0 into LDS buffer 0
-    generate(rr[NK - 1], gp(jb.wavenumber)[f0], 0, ev[es(NK - 1)]);
+    generate(rr[NK - 1], gp(jb.wavenumber)[f0], 0, ev);
     dma_sub(std::integral_constant<int, 0>{}, f0, 0);
     __syncthreads();
     // frequency-independent normalisation of every grid point (fastFuncs.py:293-310):
@@ -281,17 +280,14 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
                 xnh = xih ^ sg;
                 xnl = xil ^ sg;
             }
-            // the VALU work (steering generation) and the MFMA work of a sub-step in opposite
-            // orders in the two waves that share a SIMD (waves w and w + 4, or w and w ^ 1,
-            // depending on how the workgroup is laid out over the SIMDs: the phase bit below differs
-            // within both pairs), so that one issues MFMAs while the other runs VALU
+            // the steering generation of the next block, issued after this sub-step's MFMAs
             auto stage_gen = [&]() __attribute__((always_inline)) {
             if constexpr (sub.first[t]) {
                 // next block: J - 1 of this frequency, or the last block of the next one
                 if constexpr (J > 0)
-                    generate(rr[J > 0 ? J - 1 : 0], kcur, bcur ^ 1, ev[es(J > 0 ? J - 1 : 0)]);
+                    generate(rr[J > 0 ? J - 1 : 0], kcur, bcur ^ 1, ev);
                 else
-                    generate(rr[NK - 1], knext, bcur ^ 1, ev[es(NK - 1)]);
+                    generate(rr[NK - 1], knext, bcur ^ 1, ev);
             }
             };
             auto mfma = [&]() __attribute__((always_inline)) {
@@ -325,7 +321,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
                     // pair (J, J) closes row block J: q += Re sum conj(x) Y over its rows 4 g + i of
                     // half p, whose float32 steering values (K-slots 8 g + 4 p + i) this thread
                     // generated itself
-                    const float *xe = ev[es(J)];
+                    const float *xe = ev;
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         qacc = fmaf(xe[e], accr[J][e], qacc);
@@ -338,13 +334,11 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
             };
             // U of sub-step t + 1 (of the next frequency after the last one) into the other buffer
             dma_sub(std::integral_constant<int, (t + 1 < NS ? t + 1 : 0)>{}, t + 1 < NS ? f : fn, ub ^ 1);
-            if (((wave & 1) ^ (wave >> 2)) == 0) {
-                stage_gen();
-                mfma();
-            } else {
-                mfma();
-                stage_gen();
-            }
+            // every wave issues its MFMAs first and generates while they drain: 14.14-14.66 ms on
+            // the 256-mic, 201 x 201, 513-bin workload, against 15.22-15.89 ms with the two waves of
+            // a SIMD in opposite orders and 15.24-15.30 ms with the generation first everywhere
+            mfma();
+            stage_gen();
             if constexpr (J == 0 && sub.last[t]) {
                 // the frequency's q partial of this wave's half and column tile
                 qacc += __shfl_xor(qacc, 16);
