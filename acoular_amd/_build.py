@@ -10,8 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libabf.so')
 # (source, extra flags, object name); the streamed kernels' files build once per block count
-STREAM = [(f'abf_bf_mfma_{kind}.hip', [f'-DABF_ST_NK={nk}'], f'abf_bf_mfma_{kind}_{nk}.o') for nk in range(8, 1, -1)
-          for kind in ('stream', 'wide')]
+STREAM = [(f'abf_bf_mfma_{kind}.hip', [f'-DABF_ST_NK={nk}'], f'abf_bf_mfma_{kind}_{nk}.o') for nk in range(16, 1, -1)
+          for kind in ('stream', 'wide') if kind == 'wide' or nk <= 8]
 SOURCES = ['abf_bf_mfma_bf16.hip', 'abf_bf_mfma_f16.hip', 'abf_bf_mfma_f32.hip', 'abf_bf.hip', 'abf_steer.hip', 'abf_csm.hip',
            'abf_eig.hip', 'abf_deconv.hip', 'abf_sources.hip', 'abf_clean.hip', 'abf_resample.hip',
            'abf_welch_fft.hip', 'abf_api.hip']

@@ -130,19 +130,20 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
     p.ps = false;
     if (kreq == ABF_MFMA_KERNEL_STREAMED || kreq == ABF_MFMA_KERNEL_STREAMED_WIDE ||
         kreq == ABF_MFMA_KERNEL_STREAMED_PERSISTENT) {
-        // split-bf16, 2..8 blocks of 32 microphones, the reference's float32 phase with the hardware
-        // sin/cos; anything else runs the row-pair kernel
-        if (p.prec == kPrBF16X3 && p.Mp >= 64 && p.Mp <= 256 && (j->phase_mode & kFastTrigFlag) &&
-            (j->phase_mode & kPhaseMask) == ABF_PHASE_REFERENCE) {
+        // split-bf16, 2..8 blocks of 32 microphones (2..16 for the wide kernel), the reference's
+        // float32 phase with the hardware sin/cos; anything else runs the row-pair kernel
+        const bool wide = kreq == ABF_MFMA_KERNEL_STREAMED_WIDE;
+        if (p.prec == kPrBF16X3 && p.Mp >= 64 && p.Mp <= (wide ? 32 * kWdMaxNK : 256) &&
+            (j->phase_mode & kFastTrigFlag) && (j->phase_mode & kPhaseMask) == ABF_PHASE_REFERENCE) {
             p.st = true;
-            p.wide = kreq == ABF_MFMA_KERNEL_STREAMED_WIDE;
+            p.wide = wide;
             p.ps = kreq == ABF_MFMA_KERNEL_STREAMED_PERSISTENT;
-            static_assert(kStBN == kWdBN, "both streamed kernels tile 64 grid points");
-            p.CT = kStBN / 16;
+            const int bn = wide ? kWdBN : kStBN;
+            p.CT = bn / 16;
             p.TT = 1;
             p.CW = 1;
             p.pf = false;
-            p.ntiles = cdiv(j->geom.num_grid, kStBN);
+            p.ntiles = cdiv(j->geom.num_grid, bn);
             p.wgs = p.ntiles * cdiv(j->num_freqs, p.FB);
             p.lds = p.wide ? wd_lds_bytes() : st_lds_bytes();
             return ABF_OK;
@@ -393,7 +394,15 @@ template <int WK> void launch_wide_nk(int nk, const MfJob *dev, int n, int64_t w
     case 5: launch_wide<5, WK>(dev, n, wgs, lds, s); break;
     case 6: launch_wide<6, WK>(dev, n, wgs, lds, s); break;
     case 7: launch_wide<7, WK>(dev, n, wgs, lds, s); break;
-    default: launch_wide<8, WK>(dev, n, wgs, lds, s); break;
+    case 8: launch_wide<8, WK>(dev, n, wgs, lds, s); break;
+    case 9: launch_wide<9, WK>(dev, n, wgs, lds, s); break;
+    case 10: launch_wide<10, WK>(dev, n, wgs, lds, s); break;
+    case 11: launch_wide<11, WK>(dev, n, wgs, lds, s); break;
+    case 12: launch_wide<12, WK>(dev, n, wgs, lds, s); break;
+    case 13: launch_wide<13, WK>(dev, n, wgs, lds, s); break;
+    case 14: launch_wide<14, WK>(dev, n, wgs, lds, s); break;
+    case 15: launch_wide<15, WK>(dev, n, wgs, lds, s); break;
+    default: launch_wide<16, WK>(dev, n, wgs, lds, s); break;
     }
 }
 

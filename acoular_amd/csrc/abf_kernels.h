@@ -140,27 +140,25 @@ constexpr size_t st_lds_bytes() { return kStOffHelp + 2 * kStBN * 8; }
 // PS: persistent launch (a grid of at most one workgroup per CU walking the work items)
 template <int NK, int WK, bool PS> __global__ void bf_mfma_stream_kernel(const MfJob *jobs, int njobs, int64_t total);
 
-// wide streamed split-bf16 kernel (abf_mfma_wide.h): 4 waves, each 2 column tiles x one row half
-constexpr int kWdWaves = 4;
-constexpr int kWdBN = 64;                // grid points per workgroup
+// streamed split-bf16 kernel for up to 16 blocks of 32 microphones (abf_mfma_wide.h): 8 waves over
+// 32 grid points, each one 16-column tile x one row half of the row blocks of one parity
+constexpr int kWdWaves = 8;
+constexpr int kWdBN = 32;                // grid points per workgroup
+constexpr int kWdMaxNK = 16;             // blocks of 32 microphones (512 microphones)
 constexpr int kWdKSB = 40;               // LDS row stride of a steering block, bf16 elements (32 + 16-byte skew)
 constexpr int kWdPlane = kWdBN * kWdKSB; // elements per plane of a block
 constexpr int kWdBlock = 4 * kWdPlane;   // elements per block (re hi, re lo, im hi, im lo)
 constexpr int kWdGS = 4;                 // U blocks (pairs) per sub-step
-constexpr int kWdKSF = 36;               // LDS row stride of the float32 steering values (16-byte skew)
-constexpr int kWdPlaneF = kWdBN * kWdKSF;
-constexpr int kWdBlockF = 2 * kWdPlaneF; // re, im
-// LDS layout: steering bf16 [2][4][BN][KSB], steering f32 [2][2][BN][KSF], U ring [2][kWdGS][8 KiB],
-// q partials [2 parities][2 halves][BN] f32, normalisation factor [BN] f64, steering norm [BN] f32,
-// r0 [BN] f64, steering-sum partials [4][BN] f64
-constexpr size_t kWdOffXf = 2 * (size_t)kWdBlock * 2;
-constexpr size_t kWdOffU = kWdOffXf + 2 * (size_t)kWdBlockF * 4;
+// LDS layout: steering bf16 [2][4][BN][KSB], U ring [2][kWdGS][8 KiB], q partials [2 parities]
+// [4 wave groups][BN] f32, normalisation factor [BN] f64, steering norm [BN] f32, r0 [BN] f64,
+// steering-sum partials [4][BN] f64
+constexpr size_t kWdOffU = 2 * (size_t)kWdBlock * 2;
 constexpr size_t kWdOffQ = kWdOffU + 2 * (size_t)kWdGS * 8192;
-constexpr size_t kWdOffFac = kWdOffQ + 4 * kWdBN * 4;
+constexpr size_t kWdOffFac = kWdOffQ + 2 * 4 * kWdBN * 4;
 constexpr size_t kWdOffNst = kWdOffFac + kWdBN * 8;
 constexpr size_t kWdOffR0 = kWdOffNst + kWdBN * 4;
 constexpr size_t kWdOffHelp = kWdOffR0 + kWdBN * 8;
-constexpr size_t wd_lds_bytes() { return kWdOffHelp + kWdWaves * kWdBN * 8; }
+constexpr size_t wd_lds_bytes() { return kWdOffHelp + 4 * kWdBN * 8; }
 template <int NK, int WK> __global__ void bf_mfma_wide_kernel(const MfJob *jobs, int njobs);
 __global__ void bf_eig_kernel(EigArgs a);
 __global__ void bf_eig_grid_kernel(EigGridArgs a);

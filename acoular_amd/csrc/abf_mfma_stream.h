@@ -53,7 +53,7 @@ template <class Fn, int... S> __device__ __forceinline__ void st_for(Fn &&fn, st
 // be odd (an even count keeps the register and LDS double buffers of the U blocks in phase from
 // one frequency to the next).
 template <int NK, int GS = kStGS> struct StSub {
-    static constexpr int kMax = 2 * NK + 2;
+    static constexpr int kMax = NK * (NK + 1) / 2 + 2; // at least one pair per sub-step, one extra
     int n;
     int J[kMax], I0[kMax], I1[kMax];
     bool first[kMax], last[kMax];

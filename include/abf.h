@@ -76,9 +76,10 @@ enum abf_mfma_kernel {
                                      its generation overlapped with the MFMAs of the previous block
                                      (split-bf16, 33..256 mics, reference phase with hardware sin/cos;
                                      other jobs fall back to ROW_PAIRS) */
-    ABF_MFMA_KERNEL_STREAMED_WIDE = 7, /* STREAMED with 4 waves of two column tiles each (every U
-                                          fragment read from LDS feeds twice the MFMAs); same cases
-                                          and fallback */
+    ABF_MFMA_KERNEL_STREAMED_WIDE = 7, /* STREAMED over 32 grid points, the row blocks split
+                                          between two wave groups by parity (half the accumulator
+                                          and distance registers per wave): 33..512 mics; same
+                                          other cases and fallback */
     ABF_MFMA_KERNEL_STREAMED_PERSISTENT = 8 /* STREAMED as a persistent grid (one workgroup per CU)
                                                walking the (job, tile, band) items in a fixed
                                                round-robin order; same cases and fallback */

@@ -50,8 +50,9 @@ int main(void)
             CHECK(rc == ABF_OK || rc == ABF_ERR_UNSUPPORTED);
             if (rc == ABF_OK) CHECK(wgs > 0 && cols > 0 && fb > 0);
         }
-    /* every kernel request on every microphone count: the streamed kernel (6) plans for 2..8
-     * blocks of 32 microphones with the hardware sin/cos and the float32 phase, the others fall back */
+    /* every kernel request on every microphone count: the streamed kernels (6, 8) plan for 2..8
+     * blocks of 32 microphones (7: 2..16) with the hardware sin/cos and the float32 phase, the
+     * others fall back */
     for (int kr = 0; kr <= 8; ++kr)
         for (int mi = 0; mi < (int)(sizeof mics / sizeof mics[0]); ++mi)
             for (int fast = 0; fast < 2 && mics[mi] <= 1024; ++fast) {
@@ -63,7 +64,7 @@ int main(void)
                 CHECK(abf_beamform_plan(&j, &kern, &cols, &fb, &wgs) == ABF_OK);
                 CHECK(wgs > 0 && cols > 0 && fb > 0 && kern >= 1 && kern <= 8);
                 const int mp = (mics[mi] + 31) / 32 * 32;
-                const int streamed = kr >= 6 && fast && mp >= 64 && mp <= 256;
+                const int streamed = kr >= 6 && fast && mp >= 64 && mp <= (kr == 7 ? 512 : 256);
                 CHECK((kern >= 6) == streamed);
                 CHECK(!streamed || kern == kr);
                 CHECK(abf_beamform_workspace_size(&j) > 0);

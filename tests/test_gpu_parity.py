@@ -551,13 +551,13 @@ def test_mfma_kernel_variants_and_tiles(gpu, form, kernel, tile_cols, tile_freqs
     assert err <= _tol(form)
 
 
-@pytest.mark.parametrize('kernel', [6, 7, 8])
-@pytest.mark.parametrize('nm', [33, 64, 65, 100, 128, 160, 192, 224, 256])
+@pytest.mark.parametrize('kernel, nm', [(kn, nm) for kn in (6, 7, 8) for nm in (33, 64, 65, 100, 128, 160, 192, 224, 256)]
+                         + [(7, nm) for nm in (257, 300, 352, 384, 416, 449, 480, 512)])
 @pytest.mark.parametrize('steer', STEERS)
 def test_streamed_kernel_every_block_count(gpu, kernel, nm, steer):
-    """The streamed kernels (one instantiation per 2..8 blocks of 32 microphones): both r_diag
-    settings, a grid whose last tile is partial, a band that does not divide into the frequency
-    tile; the plan reports the kernel asked for."""
+    """The streamed kernels (one instantiation per 2..8 blocks of 32 microphones, 2..16 for the
+    wide kernel): both r_diag settings, a grid whose last tile is partial, a band that does not
+    divide into the frequency tile; the plan reports the kernel asked for."""
     torch = _lib.torch()
     freqs = np.array([700.0, 3300.0, 9000.0, 14000.0, 19500.0])
     grid = aa.RectGrid(x_min=-0.3, x_max=0.3, y_min=-0.2, y_max=0.2, z=0.5, increment=0.03)  # 21 x 14
@@ -592,12 +592,14 @@ def test_streamed_kernel_every_block_count(gpu, kernel, nm, steer):
 @pytest.mark.parametrize('kernel', [6, 7, 8])
 def test_streamed_kernel_falls_back_outside_its_cases(gpu, kernel):
     """The streamed kernels asked for the float64 phase, the library sin/cos or more than 256
-    microphones run the row-pair kernel (the plan says so) and stay within the gate."""
+    microphones (512 for the wide kernel) run the row-pair kernel (the plan says so) and stay
+    within the gate."""
     torch = _lib.torch()
     lib = _lib.lib()
     freqs = np.array([2000.0, 17000.0])
     grid = aa.RectGrid(x_min=-0.2, x_max=0.2, y_min=-0.2, y_max=0.2, z=0.5, increment=0.04)
-    for nm, phase, fast in ((64, 'fp64', True), (64, 'reference', False), (300, 'reference', True)):
+    big = 520 if kernel == 7 else 300
+    for nm, phase, fast in ((64, 'fp64', True), (64, 'reference', False), (big, 'reference', True)):
         mics, grid, csm = _scene(nm, grid, freqs, z=0.5, seed=3)
         sv = aa.SteeringVector(grid=grid, mics=mics, steer_type='true level')
         gpos, mpos = sv.device_geometry()
