@@ -127,11 +127,18 @@ constexpr int kStBlock = 4 * kStPlane;   // elements per block (re hi, re lo, im
 #define ABF_ST_GS 4
 #endif
 constexpr int kStGS = ABF_ST_GS;         // U blocks (pairs) per sub-step (3, 5: 2 % slower, 6 spills)
-// LDS layout: steering bf16 [2][4][BN][KSB], U ring [2][kStGS][8 KiB], q partials [2 parities]
+#ifndef ABF_ST_RING
+#define ABF_ST_RING 3
+#endif
+// slots of the LDS ring of U sub-steps: the loads of sub-step t + kStRing - 1 are issued in sub-step t
+// (3: 14.29 against 14.62 ms with 2 on the 256-mic, 201 x 201, 513-bin workload)
+constexpr int kStRing = ABF_ST_RING;
+static_assert(kStRing == 2 || kStRing == 3, "a U ring of two or three sub-steps");
+// LDS layout: steering bf16 [2][4][BN][KSB], U ring [kStRing][kStGS][8 KiB], q partials [2 parities]
 // [2 halves][BN] f32, normalisation factor [BN] f64, steering norm [BN] f32, r0 [BN] f64,
 // steering-sum partials [2][BN] f64
 constexpr size_t kStOffU = 2 * (size_t)kStBlock * 2;
-constexpr size_t kStOffQ = kStOffU + 2 * (size_t)kStGS * 8192;
+constexpr size_t kStOffQ = kStOffU + kStRing * (size_t)kStGS * 8192;
 constexpr size_t kStOffFac = kStOffQ + 4 * kStBN * 4;
 constexpr size_t kStOffNst = kStOffFac + kStBN * 8;
 constexpr size_t kStOffR0 = kStOffNst + kStBN * 4;

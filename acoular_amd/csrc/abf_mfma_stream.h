@@ -21,10 +21,11 @@
 //    ms so);
 //  * the U blocks (both halves, 8 KiB) of the (I, J) pairs enter the workgroup once: a step is
 //    cut into sub-steps of at most 4 pairs; at the start of sub-step t every wave issues its
-//    1 KiB share of the blocks of sub-step t + 1 as direct global-to-LDS loads
-//    (global_load_lds_dwordx4, no registers) into the other buffer of a two-slot LDS ring, and
-//    the barrier that ends the sub-step (which waits for them) hands the ring over (15.52 against
-//    15.76-15.84 ms with the shares staged through registers two sub-steps ahead).  The four
+//    1 KiB share of the blocks of sub-step t + 2 as direct global-to-LDS loads
+//    (global_load_lds_dwordx4, no registers) into a three-slot LDS ring, and the barrier that ends
+//    the sub-step hands over the slot of sub-step t + 1 after waiting for its loads only (15.52
+//    against 15.76-15.84 ms with the shares staged through registers two sub-steps ahead; 14.29
+//    against 14.62 ms with two slots, the loads of sub-step t + 1 issued in sub-step t).  The four
 //    waves of a half read their fragments from LDS (global loads of the same fragment by four
 //    waves would go through the CU's L1 four times, the bound of the first form of this kernel);
 //  * the sign of -Ui Xi is taken from a negated copy of the steering fragment, made once per
@@ -49,10 +50,10 @@ template <class Fn, int... S> __device__ __forceinline__ void st_for(Fn &&fn, st
 
 // The sub-steps of a frequency: step J (descending) covers the pairs I = J, J - 1, .., 0 (the
 // pair (J, J) that closes row block J first), cut into ceil((J + 1) / GS) chunks of nearly
-// equal size [I0, I1) counted downwards; the top step takes one more chunk when the count would
-// be odd (an even count keeps the register and LDS double buffers of the U blocks in phase from
-// one frequency to the next).
-template <int NK, int GS = kStGS> struct StSub {
+// equal size [I0, I1) counted downwards; the top step takes up to R - 1 more chunks so that the
+// count is a multiple of R (which keeps the R slots of the LDS ring of U blocks in phase from one
+// frequency to the next).
+template <int NK, int GS = kStGS, int R = 2> struct StSub {
     static constexpr int kMax = NK * (NK + 1) / 2 + 2; // at least one pair per sub-step, one extra
     int n;
     int J[kMax], I0[kMax], I1[kMax];
@@ -61,7 +62,7 @@ template <int NK, int GS = kStGS> struct StSub {
     {
         int tot = 0;
         for (int j = NK - 1; j >= 0; --j) tot += (j + GS) / GS;
-        const int extra = tot & 1;
+        const int extra = (R - tot % R) % R;
         for (int j = NK - 1; j >= 0; --j) {
             const int np = j + 1;
             const int nc = (np + GS - 1) / GS + (j == NK - 1 ? extra : 0);
@@ -86,7 +87,7 @@ template <int NK, int WK>
 __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int njobs, int64_t wg, char *smem)
 {
     uint16_t *xs = reinterpret_cast<uint16_t *>(smem);                   // steering, bf16 [2][4][BN][KSB]
-    char *us = smem + kStOffU;                                           // U ring [2][kStGS][8 KiB]
+    char *us = smem + kStOffU;                                           // U ring [kStRing][kStGS][8 KiB]
     float *s_q = reinterpret_cast<float *>(smem + kStOffQ);              // q partials [2][2][BN]
     double *s_fac = reinterpret_cast<double *>(smem + kStOffFac);        // normalisation factor [BN]
     float *s_nst = reinterpret_cast<float *>(smem + kStOffNst);          // steering normalisation [BN]
@@ -200,9 +201,9 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     const int xoff = (c * 16 + (lane & 15)) * kStKSB + (lane >> 4) * 8;             // B fragment
     const ABF_GLOBAL char *prep_base = gp(reinterpret_cast<const char *>(jb.prep));
     constexpr int64_t kFreqBytes = (int64_t)(NK * (NK + 1) / 2) * 8192;
-    constexpr StSub<NK> sub;
+    constexpr StSub<NK, kStGS, kStRing> sub;
     constexpr int NS = sub.n;
-    static_assert(NS % 2 == 0, "an even number of sub-steps per frequency");
+    static_assert(NS % kStRing == 0, "sub-steps per frequency a multiple of the ring slots");
     f4 accr[NK], acci[NK];
 #pragma unroll
     for (int I = 0; I < NK; ++I) {
@@ -227,9 +228,21 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
         }
     };
 
-    // first block of the band; U of its sub-step 0 into LDS buffer 0
-    generate(rr[NK - 1], gp(jb.wavenumber)[f0], 0, ev);
+    // the wavenumbers of the band (at most 64 frequencies), lane i holding that of frequency
+    // f0 + i: read with readlane where needed, since the first use of a vector load in the band
+    // loop would wait for every U load issued before it
+    const double kband = f0 + lane < f1 ? gp(jb.wavenumber)[f0 + lane] : 0.0;
+    auto kof = [&](int ff) __attribute__((always_inline)) {
+        const int i = ff - f0;
+        const uint64_t b = __builtin_bit_cast(uint64_t, kband);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, i);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), i);
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    };
+    // first block of the band; U of its first kStRing - 1 sub-steps into LDS slots 0 ..
+    generate(rr[NK - 1], kof(f0), 0, ev);
     dma_sub(std::integral_constant<int, 0>{}, f0, 0);
+    if constexpr (kStRing == 3) dma_sub(std::integral_constant<int, 1>{}, f0, 1);
     __syncthreads();
     // frequency-independent normalisation of every grid point (fastFuncs.py:293-310):
     // result = q * fac, normalizeSteer = nst
@@ -259,8 +272,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     int par = 0; // parity of the steering buffer in use at the first step of frequency f
     for (int f = f0; f < f1; ++f) {
         const int fn = min(f + 1, f1 - 1); // next frequency (clamped: the loads stay in bounds)
-        const double knext = gp(jb.wavenumber)[fn];
-        const double kcur = gp(jb.wavenumber)[f];
+        const double kcur = kof(f), knext = kof(fn);
         // maps of the previous frequency: its q partials were written before the last barrier
         if (f > f0 && wave == 0) store(f - 1);
         float qacc = 0.f;
@@ -268,7 +280,9 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
         auto substep = [&](auto tc) __attribute__((always_inline)) {
             constexpr int t = decltype(tc)::value;
             constexpr int J = sub.J[t];
-            constexpr int ub = t & 1; // LDS U buffer of this sub-step
+            constexpr int ub = t % kStRing;                 // LDS U slot of this sub-step
+            constexpr int tp = (t + kStRing - 1) % NS;      // sub-step whose U loads are issued here
+            constexpr int up = (t + kStRing - 1) % kStRing; // ... into this slot
             const int bcur = (par + (NK - 1 - J)) & 1;
             if constexpr (sub.first[t]) {
                 // steering fragment of block J, column tile c (K-slots of lane group lane >> 4)
@@ -333,7 +347,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
             }
             };
             // U of sub-step t + 1 (of the next frequency after the last one) into the other buffer
-            dma_sub(std::integral_constant<int, (t + 1 < NS ? t + 1 : 0)>{}, t + 1 < NS ? f : fn, ub ^ 1);
+            dma_sub(std::integral_constant<int, tp>{}, t + kStRing - 1 < NS ? f : fn, up);
             // every wave issues its MFMAs first and generates while they drain: 14.14-14.66 ms on
             // the 256-mic, 201 x 201, 513-bin workload, against 15.22-15.89 ms with the two waves of
             // a SIMD in opposite orders and 15.24-15.30 ms with the generation first everywhere
@@ -345,11 +359,22 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
                 qacc += __shfl_xor(qacc, 32);
                 if (lane < 16) s_q[((f - f0) & 1) * 2 * kStBN + p * kStBN + c * 16 + lane] = qacc;
             }
-            __syncthreads();
+            if constexpr (kStRing == 2) {
+                __syncthreads();
+            } else {
+                // the U loads of sub-step t + 1 (issued one sub-step earlier) have landed when at
+                // most the ones issued in this sub-step are outstanding (a wave's loads complete in
+                // order); the LDS writes and reads of this sub-step are done (lgkmcnt); then the
+                // barrier.  The memory clobber keeps the compiler from moving LDS accesses across.
+                constexpr int nd = sub.I0[tp] - sub.I1[tp];
+                asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(nd) : "memory");
+            }
         };
         st_for(substep, std::make_integer_sequence<int, NS>{});
         par ^= NK & 1;
     }
+    // the U loads issued ahead for a next frequency land before the workgroup's LDS is released
+    if constexpr (kStRing == 3) __syncthreads();
     if (wave == 0) store(f1 - 1);
 }
 
