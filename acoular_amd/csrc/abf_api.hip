@@ -137,8 +137,8 @@ int plan_mfma(const abf_grid_job *j, MfPlan &p)
             (j->phase_mode & kFastTrigFlag) && (j->phase_mode & kPhaseMask) == ABF_PHASE_REFERENCE) {
             p.st = true;
             p.wide = wide;
-            // kernels 6 and 8 hold the wavenumbers of a band in the 64 lanes of a register
-            if (!wide) p.FB = std::min(p.FB, 64);
+            // the streamed kernels hold the wavenumbers of a band in the 64 lanes of a register
+            p.FB = std::min(p.FB, 64);
             p.ps = kreq == ABF_MFMA_KERNEL_STREAMED_PERSISTENT;
             const int bn = wide ? kWdBN : kStBN;
             p.CT = bn / 16;

@@ -156,11 +156,12 @@ constexpr int kWdKSB = 40;               // LDS row stride of a steering block, 
 constexpr int kWdPlane = kWdBN * kWdKSB; // elements per plane of a block
 constexpr int kWdBlock = 4 * kWdPlane;   // elements per block (re hi, re lo, im hi, im lo)
 constexpr int kWdGS = 4;                 // U blocks (pairs) per sub-step
-// LDS layout: steering bf16 [2][4][BN][KSB], U ring [2][kWdGS][8 KiB], q partials [2 parities]
+constexpr int kWdRing = kStRing;         // slots of the LDS ring of U sub-steps
+// LDS layout: steering bf16 [2][4][BN][KSB], U ring [kWdRing][kWdGS][8 KiB], q partials [2 parities]
 // [4 wave groups][BN] f32, normalisation factor [BN] f64, steering norm [BN] f32, r0 [BN] f64,
 // steering-sum partials [4][BN] f64
 constexpr size_t kWdOffU = 2 * (size_t)kWdBlock * 2;
-constexpr size_t kWdOffQ = kWdOffU + 2 * (size_t)kWdGS * 8192;
+constexpr size_t kWdOffQ = kWdOffU + kWdRing * (size_t)kWdGS * 8192;
 constexpr size_t kWdOffFac = kWdOffQ + 2 * 4 * kWdBN * 4;
 constexpr size_t kWdOffNst = kWdOffFac + kWdBN * 8;
 constexpr size_t kWdOffR0 = kWdOffNst + kWdBN * 4;

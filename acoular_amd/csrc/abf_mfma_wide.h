@@ -4,7 +4,8 @@
 // The schedule of the streamed kernel (abf_mfma_stream.h): the K-blocks J of a frequency in
 // descending order; at step J every row block I <= J receives U(I, J) X(J), so row block J is
 // complete at step J and its epilogue runs there; the U blocks of a step arrive by direct
-// global-to-LDS loads into a two-slot ring one sub-step ahead; the steering of the next block is
+// global-to-LDS loads into a three-slot ring two sub-steps ahead, each wave waiting before the
+// barrier of a sub-step only for the loads of the next one; the steering of the next block is
 // generated into the other of two LDS buffers after the MFMAs of the sub-step are issued.  The
 // streamed kernel gives every wave one 16-row half of every row block, NK blocks x 8 accumulator
 // registers plus NK x 8 registers of float64 distances: 256 registers at 512 microphones, more
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     constexpr int NH = (NK + 1) / 2; // row blocks of a parity (at most)
     extern __shared__ __attribute__((aligned(16))) char smem[];
     uint16_t *xs = reinterpret_cast<uint16_t *>(smem);              // steering, bf16 [2][4][BN][KSB]
-    char *us = smem + kWdOffU;                                      // U ring [2][kWdGS][8 KiB]
+    char *us = smem + kWdOffU;                                      // U ring [kWdRing][kWdGS][8 KiB]
     float *s_q = reinterpret_cast<float *>(smem + kWdOffQ);         // q partials [2][4][BN]
     double *s_fac = reinterpret_cast<double *>(smem + kWdOffFac);   // normalisation factor [BN]
     float *s_nst = reinterpret_cast<float *>(smem + kWdOffNst);     // steering normalisation [BN]
@@ -154,9 +155,9 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     const int xoff = (c * 16 + (lane & 15)) * kWdKSB + (lane >> 4) * 8; // B fragment
     const ABF_GLOBAL char *prep_base = gp(reinterpret_cast<const char *>(jb.prep));
     constexpr int64_t kFreqBytes = (int64_t)(NK * (NK + 1) / 2) * 8192;
-    constexpr StSub<NK, kWdGS> sub;
+    constexpr StSub<NK, kWdGS, kWdRing> sub;
     constexpr int NS = sub.n;
-    static_assert(NS % 2 == 0, "an even number of sub-steps per frequency");
+    static_assert(NS % kWdRing == 0, "sub-steps per frequency a multiple of the ring slots");
     f4 accr[NH], acci[NH];
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
@@ -181,9 +182,22 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
         }
     };
 
-    // first block of the band (by the waves of its parity); U of its sub-step 0 into LDS buffer 0
-    if (rgw == ((NK - 1) & 1)) generate(rr[(NK - 1) >> 1], gp(jb.wavenumber)[f0], 0);
+    // the wavenumbers of the band (at most 64 frequencies), lane i holding that of frequency
+    // f0 + i: read with readlane where needed, since the first use of a vector load in the band
+    // loop would wait for every U load issued before it
+    const double kband = f0 + lane < f1 ? gp(jb.wavenumber)[f0 + lane] : 0.0;
+    auto kof = [&](int ff) __attribute__((always_inline)) {
+        const int i = ff - f0;
+        const uint64_t b = __builtin_bit_cast(uint64_t, kband);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, i);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), i);
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    };
+    // first block of the band (by the waves of its parity); U of its first kWdRing - 1 sub-steps
+    // into LDS slots 0 ..
+    if (rgw == ((NK - 1) & 1)) generate(rr[(NK - 1) >> 1], kof(f0), 0);
     dma_sub(std::integral_constant<int, 0>{}, f0, 0);
+    if constexpr (kWdRing == 3) dma_sub(std::integral_constant<int, 1>{}, f0, 1);
     __syncthreads();
     // frequency-independent normalisation of every grid point (fastFuncs.py:293-310):
     // result = q * fac, normalizeSteer = nst
@@ -214,8 +228,7 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     int par = 0; // parity of the steering buffer in use at the first step of frequency f
     for (int f = f0; f < f1; ++f) {
         const int fn = min(f + 1, f1 - 1); // next frequency (clamped: the loads stay in bounds)
-        const double knext = gp(jb.wavenumber)[fn];
-        const double kcur = gp(jb.wavenumber)[f];
+        const double kcur = kof(f), knext = kof(fn);
         // maps of the previous frequency: its q partials were written before the last barrier
         if (f > f0 && wave == 0) store(f - 1);
         float qacc = 0.f;
@@ -224,7 +237,9 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
             constexpr int t = decltype(tc)::value;
             constexpr int J = sub.J[t];
             constexpr int JG = J > 0 ? J - 1 : NK - 1; // block generated in this step
-            constexpr int ub = t & 1;                  // LDS U buffer of this sub-step
+            constexpr int ub = t % kWdRing;                 // LDS U slot of this sub-step
+            constexpr int tp = (t + kWdRing - 1) % NS;      // sub-step whose U loads are issued here
+            constexpr int up = (t + kWdRing - 1) % kWdRing; // ... into this slot
             const int bcur = (par + (NK - 1 - J)) & 1;
             if constexpr (sub.first[t]) {
                 // steering fragment of block J, column tile c (K-slots of lane group lane >> 4)
@@ -239,7 +254,7 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
                 }
             }
             // U of sub-step t + 1 (of the next frequency after the last one) into the other buffer
-            dma_sub(std::integral_constant<int, (t + 1 < NS ? t + 1 : 0)>{}, t + 1 < NS ? f : fn, ub ^ 1);
+            dma_sub(std::integral_constant<int, tp>{}, t + kWdRing - 1 < NS ? f : fn, up);
             // the MFMAs of this wave's pairs (parity rgw) of the sub-step, then the generation
             const char *ua = us + ub * (kWdGS * 8192) + p * 4096 + lane * 16;
 #pragma unroll
@@ -291,11 +306,22 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
                 qacc += __shfl_xor(qacc, 32);
                 if (lane < 16) s_q[((f - f0) & 1) * 4 * kWdBN + grp * kWdBN + c * 16 + lane] = qacc;
             }
-            __syncthreads();
+            if constexpr (kWdRing == 2) {
+                __syncthreads();
+            } else {
+                // the U loads of sub-step t + 1 (issued one sub-step earlier) have landed when at
+                // most the ones issued in this sub-step are outstanding (a wave's loads complete in
+                // order); the LDS writes and reads of this sub-step are done (lgkmcnt); then the
+                // barrier.  The memory clobber keeps the compiler from moving LDS accesses across.
+                constexpr int nd = sub.I0[tp] - sub.I1[tp];
+                asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(nd) : "memory");
+            }
         };
         st_for(substep, std::make_integer_sequence<int, NS>{});
         par ^= NK & 1;
     }
+    // the U loads issued ahead for a next frequency land before the workgroup's LDS is released
+    if constexpr (kWdRing == 3) __syncthreads();
     if (wave == 0) store(f1 - 1);
 }
 
