@@ -79,6 +79,22 @@ int main(void)
         abf_grid_job j = job(131072, 1024, 256, ABF_FORM_MFMA_BF16X3, ABF_STEER_TRUE_LEVEL);
         CHECK(abf_beamform_workspace_size(&j) > ((size_t)1 << 30));
     }
+    /* the streamed kernels take bands of at most 64 frequencies (their wavenumbers sit in the 64
+     * lanes of a register); the row-pair kernel takes the band asked for */
+    {
+        const int kernels[] = {3, 6, 7, 8};
+        for (int i = 0; i < 4; ++i) {
+            abf_grid_job j = job(40401, 256, 513, ABF_FORM_MFMA_BF16X3, ABF_STEER_TRUE_LEVEL);
+            j.phase_mode = ABF_PHASE_REFERENCE | ABF_PHASE_FAST_TRIG;
+            j.tile_kernel = kernels[i];
+            j.tile_freqs = 128;
+            int32_t kern = -1, cols = -1, fb = -1;
+            int64_t wgs = -1;
+            CHECK(abf_beamform_plan(&j, &kern, &cols, &fb, &wgs) == ABF_OK);
+            CHECK(kern == kernels[i]);
+            CHECK(fb == (kern >= 6 ? 64 : 128));
+        }
+    }
     /* unknown steer type / form: size 0 and a message */
     {
         abf_grid_job j = job(100, 8, 2, ABF_FORM_MFMA_BF16X3, 9);
