@@ -114,7 +114,8 @@ typedef struct abf_grid_job {
     float *out;               /* [num_freqs][num_grid] result */
     float *out_norm;          /* [num_freqs][num_grid] normalizeSteer, may be NULL */
     int32_t tile_cols;        /* MFMA forms: grid points per workgroup (16..256, power of two); 0 = auto */
-    int32_t tile_freqs;       /* MFMA forms: frequencies per workgroup; 0 = auto */
+    int32_t tile_freqs;       /* MFMA forms: frequencies per workgroup (at most 64 for the
+                                 streamed kernels); 0 = auto */
     int32_t tile_kernel;      /* MFMA forms: enum abf_mfma_kernel; 0 = auto */
     int32_t pad1_;
 } abf_grid_job;
