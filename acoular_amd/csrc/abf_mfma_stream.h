@@ -182,7 +182,10 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const bf16x2 hp = {(__bf16)src[2 * q], (__bf16)src[2 * q + 1]};
-                const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
+                uint32_t hb = __builtin_bit_cast(uint32_t, hp);
+                // opaque to the compiler, which otherwise converts the first element a second time
+                // to get its bf16 value back instead of shifting it out of the pair
+                asm("" : "+v"(hb));
                 const float h0 = __uint_as_float(hb << 16), h1 = __uint_as_float(hb & 0xffff0000u);
                 const bf16x2 lp = {(__bf16)(src[2 * q] - h0), (__bf16)(src[2 * q + 1] - h1)};
                 h[2 * d + q] = hb;
