@@ -21,6 +21,31 @@ template <typename T> __device__ __forceinline__ ABF_GLOBAL T *gpw(T *p) { retur
 
 __device__ __forceinline__ int upper_index(int I, int J, int nK) { return I * nK - (I * (I - 1)) / 2 + (J - I); }
 
+// The wavenumbers of a band of at most 64 frequencies f0 .. f1 - 1, lane i holding that of
+// frequency f0 + i, read where needed with readlane (a uniform frequency index).  A vector load of
+// the wavenumber inside the frequency loop of the streamed kernels would make its first use wait
+// for every U load in flight.  (In the alternating kernels, whose loop starts with no loads in
+// flight, the same change measured within noise: 59.94-59.97 vs 60.01-60.13 ms on the fused
+// 512-mic workload.)
+struct BandWavenumbers {
+    double v;
+    int f0;
+    __device__ __forceinline__ BandWavenumbers(const double *wavenumber, int f0_, int f1, int lane) : f0(f0_)
+    {
+        v = f0_ + lane < f1 ? gp(wavenumber)[f0_ + lane] : 0.0;
+        // used here, so the load is waited for once, where the band starts, and not at the top of
+        // every frequency (where the compiler cannot tell the first iteration from the others)
+        asm volatile("" ::"v"(v));
+    }
+    __device__ __forceinline__ double operator()(int f) const
+    {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, f - f0);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), f - f0);
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    }
+};
+
 __device__ __forceinline__ int find_job(const MfJob *__restrict__ jobs, int njobs, int64_t wg)
 {
     int lo = 0, hi = njobs - 1;

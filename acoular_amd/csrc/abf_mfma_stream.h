@@ -231,17 +231,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
         }
     };
 
-    // the wavenumbers of the band (at most 64 frequencies), lane i holding that of frequency
-    // f0 + i: read with readlane where needed, since the first use of a vector load in the band
-    // loop would wait for every U load issued before it
-    const double kband = f0 + lane < f1 ? gp(jb.wavenumber)[f0 + lane] : 0.0;
-    auto kof = [&](int ff) __attribute__((always_inline)) {
-        const int i = ff - f0;
-        const uint64_t b = __builtin_bit_cast(uint64_t, kband);
-        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, i);
-        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), i);
-        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-    };
+    const BandWavenumbers kof(jb.wavenumber, f0, f1, lane);
     // first block of the band; U of its first kStRing - 1 sub-steps into LDS slots 0 ..
     generate(rr[NK - 1], kof(f0), 0, ev);
     dma_sub(std::integral_constant<int, 0>{}, f0, 0);
