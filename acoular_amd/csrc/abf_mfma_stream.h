@@ -339,7 +339,7 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
                 }
             }
             };
-            // U of sub-step t + 1 (of the next frequency after the last one) into the other buffer
+            // U of sub-step t + kStRing - 1 (of the next frequency past the last) into its slot
             dma_sub(std::integral_constant<int, tp>{}, t + kStRing - 1 < NS ? f : fn, up);
             // every wave issues its MFMAs first and generates while they drain: 14.14-14.66 ms on
             // the 256-mic, 201 x 201, 513-bin workload, against 15.22-15.89 ms with the two waves of

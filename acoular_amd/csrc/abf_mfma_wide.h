@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
                     xnl = xil ^ sg;
                 }
             }
-            // U of sub-step t + 1 (of the next frequency after the last one) into the other buffer
+            // U of sub-step t + kWdRing - 1 (of the next frequency past the last) into its slot
             dma_sub(std::integral_constant<int, tp>{}, t + kWdRing - 1 < NS ? f : fn, up);
             // the MFMAs of this wave's pairs (parity rgw) of the sub-step, then the generation
             const char *ua = us + ub * (kWdGS * 8192) + p * 4096 + lane * 16;
