@@ -518,7 +518,7 @@ void launch_prep_mfma(const abf_grid_job *j, const MfPlan &p, void *prep, hipStr
         hipLaunchKernelGGL(uscale_kernel, dim3(j->num_freqs), dim3(256), 0, s,
                            reinterpret_cast<const float2 *>(j->csm), j->geom.num_mics, j->r_diag, uscale);
     }
-    hipLaunchKernelGGL(prep_mfma_kernel, dim3(p.nK * p.nK, 2, j->num_freqs), dim3(64), 0, s,
+    hipLaunchKernelGGL(prep_mfma_kernel, dim3(p.nUp, j->num_freqs), dim3(128), 0, s,
                        reinterpret_cast<const float2 *>(j->csm), j->geom.num_mics, p.Mp, j->r_diag, p.prec, uscale,
                        reinterpret_cast<char *>(prep));
 }

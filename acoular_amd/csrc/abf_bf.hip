@@ -431,29 +431,43 @@ __global__ __launch_bounds__(256) void uscale_kernel(const float2 *__restrict__ 
     }
 }
 
-__global__ __launch_bounds__(64) void prep_mfma_kernel(const float2 *__restrict__ csm, int M, int Mp, int r_diag,
-                                                       int prec, const double *__restrict__ uscale,
-                                                       char *__restrict__ prep)
+// One workgroup of 128 threads per (upper block (I, J), frequency): the 32 x 32 block of the CSM
+// is read row-contiguously (32 threads per 256-byte row) into LDS, then thread (t, l) gathers the
+// 8 entries of its fragment (row I 32 + 16 t + (l & 15), K-slots 8 (l >> 4) + e) from there.
+__global__ __launch_bounds__(128) void prep_mfma_kernel(const float2 *__restrict__ csm, int M, int Mp, int r_diag,
+                                                        int prec, const double *__restrict__ uscale,
+                                                        char *__restrict__ prep)
 {
+    __shared__ float2 blk[32][33];
     const int nK = Mp / 32;
-    const int I = blockIdx.x / nK, J = blockIdx.x % nK;
-    if (J < I) return;
-    const int t = blockIdx.y, f = blockIdx.z;
-    const int l = threadIdx.x;
-    const int i = I * 32 + t * 16 + (l & 15);
+    // upper block u = upper_index(I, J, nK) of the grid's x index
+    const int u = blockIdx.x;
+    int I = 0;
+    while (upper_index(I + 1, I + 1, nK) <= u) ++I;
+    const int J = I + (u - upper_index(I, I, nK));
+    const int f = blockIdx.y;
+    const int k = threadIdx.x;
+    const float2 *Cf = csm + (int64_t)f * M * M;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int r = q * 4 + (k >> 5), c = k & 31;
+        const int i = I * 32 + r, j = J * 32 + c;
+        blk[r][c] = (i < M && j < M) ? Cf[(int64_t)i * M + j] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int t = k >> 6, l = k & 63;
+    const int rl = t * 16 + (l & 15), i = I * 32 + rl;
     float ur[8], ui[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const int j = J * 32 + kslot_mic(l >> 4, e);
+        const int cl = kslot_mic(l >> 4, e), j = J * 32 + cl;
+        const float2 c = blk[rl][cl];
         float x = 0.f, y = 0.f;
-        if (i < M && j < M) {
-            const float2 c = csm[((int64_t)f * M + i) * M + j];
-            if (j > i) {
-                x = 2.f * c.x;
-                y = 2.f * c.y;
-            } else if (j == i && !r_diag) {
-                x = c.x;
-            }
+        if (j > i) {
+            x = 2.f * c.x;
+            y = 2.f * c.y;
+        } else if (j == i && !r_diag) {
+            x = c.x;
         }
         ur[e] = x;
         ui[e] = y;
