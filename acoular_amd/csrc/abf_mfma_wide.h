@@ -107,51 +107,7 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     float ev[8];
     // phasors of distances rb for wavenumber k into steering buffer b
     auto generate = [&](const double (&rb)[4], double k, int b) __attribute__((always_inline)) {
-        float *xr = ev, *xi = ev + 4;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            double r = rb[i];
-            if constexpr (WK != 0) asm volatile("" : "+v"(r)); // keeps the weight in the band loop
-            // the reference's float32 phase (fastFuncs.py:260), reduced to [-1/2, 1/2] revolution
-            // in float64 for the hardware sin/cos
-            const double phi = (double)(float)(k * r);
-            const double u = phi * kInvTwoPi;
-            const float t = (float)(u - rint(u));
-            const float sn = __builtin_amdgcn_sinf(t);
-            const float cs = __builtin_amdgcn_cosf(t);
-            if constexpr (WK == 0) {
-                xr[i] = cs;
-                xi[i] = -sn;
-            } else {
-                const float rf = (float)r;
-                const float w = WK == 1 ? rf : __builtin_amdgcn_rcpf(rf);
-                xr[i] = w * cs;
-                xi[i] = -w * sn;
-            }
-        }
-        // hi = RNE bf16 pair, lo = RNE bf16 of the residuals; 8 bytes per plane
-        uint32_t h[4], l[4];
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-            const float *src = d == 0 ? xr : xi;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const bf16x2 hp = {(__bf16)src[2 * q], (__bf16)src[2 * q + 1]};
-                uint32_t hb = __builtin_bit_cast(uint32_t, hp);
-                // opaque to the compiler, which otherwise converts the first element a second time
-                // to get its bf16 value back instead of shifting it out of the pair
-                asm("" : "+v"(hb));
-                const float h0 = __uint_as_float(hb << 16), h1 = __uint_as_float(hb & 0xffff0000u);
-                const bf16x2 lp = {(__bf16)(src[2 * q] - h0), (__bf16)(src[2 * q + 1] - h1)};
-                h[2 * d + q] = hb;
-                l[2 * d + q] = __builtin_bit_cast(uint32_t, lp);
-            }
-        }
-        uint16_t *dst = xs + b * kWdBlock + col * kWdKSB + 8 * gq + 4 * p;
-        *reinterpret_cast<uint2 *>(dst) = make_uint2(h[0], h[1]);
-        *reinterpret_cast<uint2 *>(dst + kWdPlane) = make_uint2(l[0], l[1]);
-        *reinterpret_cast<uint2 *>(dst + 2 * kWdPlane) = make_uint2(h[2], h[3]);
-        *reinterpret_cast<uint2 *>(dst + 3 * kWdPlane) = make_uint2(l[2], l[3]);
+        steer4<WK>(rb, k, ev, xs + b * kWdBlock + col * kWdKSB + 8 * gq + 4 * p, kWdPlane);
     };
 
     // ---- MFMA role: column tile c, row half p of the row blocks of parity rgw ----
