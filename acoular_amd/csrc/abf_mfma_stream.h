@@ -84,15 +84,18 @@ template <int NK, int GS = kStGS, int R = 2> struct StSub {
 // Four steering values of one grid point for wavenumber k from float64 distances rb (weight kind
 // WK: 0 none, 1 r, 2 1/r): their float32 values into xe (re 0..3, im 4..7), their split-bf16 form
 // (hi = RNE bf16, lo = RNE bf16 of the residual) into the four planes at dst (plane stride
-// `plane` elements), 8 bytes per plane.
-template <int WK>
+// `plane` elements), 8 bytes per plane.  PIN keeps the weights from being hoisted out of the band
+// loop (an opaque copy of the distance): the wide kernel needs it for its register budget; the
+// streamed kernel runs faster without it (inverse steering 14.40-14.48 against 14.83-15.13 ms,
+// true level 14.39-14.49 against 14.74-14.77 ms on the 256-mic, 201 x 201, 513-bin shape).
+template <int WK, bool PIN>
 __device__ __forceinline__ void steer4(const double (&rb)[4], double k, float (&xe)[8], uint16_t *dst, int plane)
 {
     float *xr = xe, *xi = xe + 4;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         double r = rb[i];
-        if constexpr (WK != 0) asm volatile("" : "+v"(r)); // keeps the weight in the band loop
+        if constexpr (WK != 0 && PIN) asm volatile("" : "+v"(r));
         // the reference's float32 phase (fastFuncs.py:260), reduced to [-1/2, 1/2] revolution in
         // float64 for the hardware sin/cos
         const double phi = (double)(float)(k * r);
@@ -205,7 +208,9 @@ __device__ __forceinline__ void stream_item(const MfJob *__restrict__ jobs, int 
     // phasors of a block (distances rb) for wavenumber k into steering buffer b: K-slots
     // 8 gq + 4 pg .. + 3 of column col, as bf16 hi/lo planes (MFMA operand) and float32 (epilogue)
     auto generate = [&](const double (&rb)[4], double k, int b, float (&xe)[8]) __attribute__((always_inline)) {
-        steer4<WK>(rb, k, xe, xs + b * kStBlock + col * kStKSB + 8 * gq + 4 * pg, kStPlane);
+        // unpinned weights, except at 6 blocks, where the weighted instantiations would spill 7
+        // registers without the pin
+        steer4<WK, NK == 6>(rb, k, xe, xs + b * kStBlock + col * kStKSB + 8 * gq + 4 * pg, kStPlane);
     };
 
     // ---- MFMA role: column tile c, row half p of every row block ----

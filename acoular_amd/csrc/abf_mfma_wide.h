@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kWdWaves * 64) __attribute__((amdgpu_waves_per_eu(2
     float ev[8];
     // phasors of distances rb for wavenumber k into steering buffer b
     auto generate = [&](const double (&rb)[4], double k, int b) __attribute__((always_inline)) {
-        steer4<WK>(rb, k, ev, xs + b * kWdBlock + col * kWdKSB + 8 * gq + 4 * p, kWdPlane);
+        steer4<WK, true>(rb, k, ev, xs + b * kWdBlock + col * kWdKSB + 8 * gq + 4 * p, kWdPlane);
     };
 
     // ---- MFMA role: column tile c, row half p of the row blocks of parity rgw ----
